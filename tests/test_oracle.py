@@ -1,0 +1,389 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+Every check here is fixed by something other than the oracle's own formulas: the
+reference's golden vectors (test/test_fields.txt, test/test_demag.jl), closed forms
+from the reference's tests (test_exch.jl, test_dmi.jl, test_anis.jl,
+eigen/test_demag.jl, test_llg.jl), library cross-checks (numpy.fft) and
+invariants (energy/field consistency by finite differences, demag symmetry).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+MU0 = 4e-7 * np.pi
+
+
+def load_field_data():
+    d = np.loadtxt(os.path.join(GOLDEN, "test_fields.txt"), comments="#")
+    n3 = d.shape[0] - 1
+    N = n3 // 3
+    return (d[:N, 0], d[:n3, 1], d[:n3, 2], d[:n3, 3], d[:n3, 4], d[:n3, 5], d[:n3, 6], d[-1, 2:7])
+
+
+@pytest.mark.parametrize("fft", [True, False])
+def test_fields_golden(oracle, fft):
+    """test/test_fields.jl:25-59: Fidimag fields on 20x5x3 with 16 Ms=0 cells."""
+    Ms, m0, fd, fe, fdmi, fdmi2, fan, E = load_field_data()
+    s = oracle.OracleSim(20, 5, 3, 2.5e-9, 2.5e-9, 3e-9)
+    s.set_Ms(Ms)
+    s.init_m0(m0, norm=False)
+    te = s.add_exch(1.3e-11)
+    td = s.add_demag(fft=fft)
+    tb = s.add_dmi(4e-3)
+    ti = s.add_dmi(2e-3, type="interfacial")
+    ta = s.add_anis(-3e4, axis=(0, 0, 1))
+    s.effective_field()
+    Es = s.term_energies()
+    for t, ref in [(te, fe), (td, fd), (tb, fdmi), (ti, fdmi2), (ta, fan)]:
+        f, _ = s.term_field(t)
+        assert np.allclose(f, ref, rtol=0, atol=1e-5)
+    assert abs((Es[td] - E[0]) / E[0]) < 1e-11
+    assert abs((Es[te] - E[1]) / E[1]) < 1e-15
+    assert abs((Es[tb] - E[2]) / E[2]) < 1e-15
+    assert abs((Es[ti] - E[3]) / E[3]) < 1e-15
+    assert abs((Es[ta] - E[4]) / E[4]) < 1e-15
+    # the aggregate is the insertion-order sum of the term fields
+    tot = s.effective_field()
+    parts = sum(s.term_field(t)[0] for t in (te, td, tb, ti, ta))
+    assert np.allclose(tot, parts, rtol=1e-14, atol=1e-9)
+
+
+@pytest.mark.parametrize("fft", [True, False])
+def test_demag_golden(oracle, fft):
+    """test/test_demag.jl:8-57: hard-coded demag fields."""
+    cases = json.load(open(os.path.join(GOLDEN, "demag_fields.json")))["cases"]
+    for c in cases:
+        nx, ny, nz = c["mesh"]
+        s = oracle.OracleSim(nx, ny, nz, *c["d"])
+        s.set_Ms(np.full(nx * ny * nz, c["Ms"]))
+        s.init_m0(np.tile(c["m0"], nx * ny * nz))
+        s.add_demag(fft=fft)
+        f = s.effective_field()
+        assert np.allclose(f, c["expected"], rtol=1e-8, atol=1e-6)
+
+
+def _macro_uniform(oracle, nx, ny, nz, Nx, Ny, Nz):
+    s = oracle.OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9)
+    s.set_mu0Ms(np.full(nx * ny * nz, MU0 * 8.6e5))
+    s.init_m0(np.tile([1.0, 1, 1], nx * ny * nz), norm=False)
+    s.add_demag(Nx, Ny, Nz)
+    return s.effective_field() / 8.6e5
+
+
+def test_demag_macro_pbc(oracle):
+    """test/test_demag.jl:95-132: image-sum limits and self-consistency."""
+    f1 = _macro_uniform(oracle, 1, 1, 1, 500, 0, 0)
+    assert abs(f1[0]) < 1e-6 and abs(f1[1] + 0.5) < 1e-6 and abs(f1[2] + 0.5) < 1e-6
+    f2 = _macro_uniform(oracle, 101, 101, 1, 5, 5, 0).reshape(1, 101, 101, 3)
+    c = f2[0, 50, 50]
+    assert abs(c[0]) < 1e-3 and abs(c[1]) < 1e-3 and abs(c[2] + 1) < 1e-3
+    f3 = _macro_uniform(oracle, 1, 1, 1, 31, 31, 0)
+    f4 = _macro_uniform(oracle, 3, 3, 1, 10, 10, 0).reshape(1, 3, 3, 3)
+    assert np.allclose(f3, f4[0, 1, 1], atol=1e-6)
+
+    def rand_m(nx, ny):
+        m = np.zeros((1, ny, nx, 3))
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 1):
+                m[0, j - 1, i - 1] = (0.2, np.sin((i % 3) * 0.9), np.cos((j % 3) * 1.1))
+        return m.reshape(-1)
+
+    def macro(nx, ny, Nx, Ny):
+        s = oracle.OracleSim(nx, ny, 1, 2e-9, 2e-9, 2e-9)
+        s.set_mu0Ms(np.full(nx * ny, MU0 * 8.6e5))
+        s.init_m0(rand_m(nx, ny))
+        s.add_demag(Nx, Ny, 0)
+        return s.effective_field() / 8.6e5
+
+    f5 = macro(3, 3, 5, 5)
+    f6 = macro(33, 33, 0, 0).reshape(33, 33, 3)[15:18, 15:18].reshape(-1)
+    assert np.max(np.abs(f5 - f6)) < 5e-7
+
+
+def test_demag_tensor_closed_forms(oracle):
+    """Cube self term N = 1/3 (symmetry); 5:1:1 prism Nxx from eigen/test_demag.jl:29."""
+    s = oracle.OracleSim(1, 1, 1, 2e-9, 2e-9, 2e-9)
+    s.set_Ms([8e5])
+    t = s.add_demag()
+    T = s.demag_tensor(t)[:, 0]
+    assert np.allclose(T[:3], 1 / 3, rtol=0, atol=1e-14)
+    assert np.allclose(T[3:], 0, atol=1e-15)
+    s = oracle.OracleSim(1, 1, 1, 5e-9, 1e-9, 1e-9)
+    s.set_Ms([8e5])
+    T = s.demag_tensor(s.add_demag())[:, 0]
+    assert abs(T[0] - 0.08831574004542228) < 1e-12
+    assert abs(T[1] - (1 - T[0]) / 2) < 1e-12 and abs(T[0] + T[1] + T[2] - 1) < 1e-12
+
+
+def test_demag_operator_symmetric(oracle):
+    """<a, N b> = <N a, b>: the Newell operator is symmetric (with Ms = 1/mu0)."""
+    rng = np.random.default_rng(7)
+    nx, ny, nz = 6, 5, 4
+    N = nx * ny * nz
+    s = oracle.OracleSim(nx, ny, nz, 2e-9, 3e-9, 1.5e-9)
+    s.set_mu0Ms(np.ones(N))
+    s.add_demag()
+    a = rng.standard_normal(3 * N)
+    b = rng.standard_normal(3 * N)
+    Na = s.effective_field(a)
+    Nb = s.effective_field(b)
+    assert abs(a @ Nb - Na @ b) < 1e-12 * np.abs(a @ Nb)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 10, 12, 40, 100, 128, 243, 400, 512])
+def test_fft_vs_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    assert np.allclose(oracle.fft(x, -1), ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+    assert np.allclose(oracle.fft(x, +1), np.fft.ifft(x) * n, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
+def test_mesh_neighbours(oracle):
+    """test/test_mesh.jl:8-32 (neighbour table vs the _x_plus_one helpers)."""
+    nx, ny, nz = 3, 4, 2
+    g = oracle.OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9, pbc="xz").neighbours()
+
+    def idx(i, j, k):
+        return (k - 1) * nx * ny + (j - 1) * nx + i
+
+    for k in range(1, nz + 1):
+        for j in range(1, ny + 1):
+            for i in range(1, nx + 1):
+                I = idx(i, j, k)
+                row = g[I - 1]
+                assert row[0] == (idx(nx, j, k) if i == 1 else I - 1)
+                assert row[1] == (idx(1, j, k) if i == nx else I + 1)
+                assert row[2] == (-1 if j == 1 else I - nx)
+                assert row[3] == (-1 if j == ny else I + nx)
+                assert row[4] == (idx(i, j, nz) if k == 1 else I - nx * ny)
+                assert row[5] == (idx(i, j, 1) if k == nz else I + nx * ny)
+    g = oracle.OracleSim(50, 1, 1, 2e-9, pbc="x").neighbours()
+    assert g[0, 0] == 50 and g[49, 0] == 49 and g[0, 1] == 2 and g[49, 1] == 1
+
+
+def test_exchange_sine_wave(oracle):
+    """test/test_exch.jl:12-56: periodic sine wave vs -2A/(mu0 Ms) k^2 m."""
+    Nx, A, Ms = 50, 1.3e-11, 8.6e5
+    L = Nx * 2e-9
+    xs = np.arange(1, Nx + 1) * 2e-9
+    m = np.stack([np.sin(2 * np.pi * xs / L), np.sin(2 * np.pi * xs / L + 1.2),
+                  np.sin(2 * np.pi * xs / L + 2.3)], axis=1).reshape(-1)
+    for spatial in (False, True):
+        s = oracle.OracleSim(Nx, 1, 1, 2e-9, pbc="x")
+        s.set_Ms(np.full(Nx, Ms))
+        s.init_m0(m, norm=False)
+        s.add_exch(np.full(Nx, A) if spatial else A)
+        b = s.effective_field().reshape(Nx, 3)
+        for c, ph in enumerate((0, 1.2, 2.3)):
+            exp = -2 * A / (MU0 * Ms) * (2 * np.pi / L) ** 2 * np.sin(2 * np.pi / L * xs + ph)
+            assert np.max(b[:, c] - exp) / Ms < 2e-4
+
+
+def test_exchange_uniform_vs_spatial(oracle):
+    """test/test_exch.jl:70-101: single-axis A tuples vs a spatial A on a 3x3x3 cross."""
+    def m_fun():
+        m = np.zeros((3, 3, 3, 3))
+        for k in range(1, 4):
+            for j in range(1, 4):
+                for i in range(1, 4):
+                    m[k - 1, j - 1, i - 1] = (i ** 2, j + 1.0, k * j)
+        return m.reshape(-1)
+
+    for axis in range(3):
+        Ms = np.zeros((3, 3, 3))
+        if axis == 0:
+            Ms[1, 1, :] = 1e5
+        elif axis == 1:
+            Ms[1, :, 1] = 1e5
+        else:
+            Ms[:, 1, 1] = 1e5
+        A = [0.0, 0.0, 0.0]
+        A[axis] = 1.3e-11
+        s = oracle.OracleSim(3, 3, 3)
+        s.set_Ms(Ms.reshape(-1))
+        s.init_m0(m_fun())
+        t1 = s.add_exch(np.full(27, 1.3e-11))
+        t2 = s.add_exch(tuple(A))
+        f1, _ = s.term_field(t1)
+        f2, _ = s.term_field(t2)
+        assert np.allclose(f1, f2, rtol=1e-7)
+
+
+def test_dmi_closed_forms(oracle):
+    """test/test_dmi.jl:8-96: two-cell bulk and interfacial DMI fields."""
+    Ms, D, dx = 8.6e5, 1.23e-3, 2e-9
+    m = np.array([1, 2, 3, 4, 5, 6], dtype=float)
+    s = oracle.OracleSim(2, 1, 1, dx)
+    s.set_Ms(np.full(2, Ms))
+    s.init_m0(m, norm=False)
+    t1 = s.add_dmi(D)
+    t2 = s.add_dmi(np.full(2, D))
+    t3 = s.add_dmi((0.1 * D, D, 0.0))
+    t4 = s.add_dmi((np.full(2, 0.1 * D), np.full(2, D), np.zeros(2)))
+    ti = s.add_dmi(D, type="interfacial")
+    f1, f2, f3, f4, fi = (s.term_field(t)[0] for t in (t1, t2, t3, t4, ti))
+    assert np.allclose(f1, f2, rtol=1e-8)
+    assert np.allclose(f3, f4, rtol=1e-8)
+    c = D / dx / (MU0 * Ms)
+    assert np.allclose(f1[:3], -c * np.cross([1, 0, 0], [4, 5, 6]), rtol=1e-12)
+    assert np.allclose(fi[:3], c * np.cross([0, -1, 0], [4, 5, 6]), rtol=1e-12)
+
+
+def test_anisotropy_closed_forms(oracle):
+    """test/test_anis.jl:10-58 (uniaxial, fixed and per-cell axis) and a cubic case."""
+    Ms, Ku = 8.6e5, 1e5
+    m = np.array([1.1, 2.3, 4.3])
+    axis = np.array([0, 0.6, 0.8])
+    for spatial in (False, True):
+        s = oracle.OracleSim(10, 1, 1)
+        s.set_Ms(np.full(10, Ms))
+        s.init_m0(np.tile(m, 10), norm=False)
+        t = s.add_anis(Ku, axis=np.tile(axis, 10) if spatial else tuple(axis))
+        f, e = s.term_field(t)
+        assert abs(f[0]) < 1e-12
+        assert np.isclose(f[1], 2 * Ku / (MU0 * Ms) * (m @ axis) * axis[1], rtol=1e-14)
+        assert np.isclose(f[2], 2 * Ku / (MU0 * Ms) * (m @ axis) * axis[2], rtol=1e-14)
+        assert np.isclose(e[9], Ku * (1 - (m @ axis) ** 2) * 1e-27, rtol=1e-14)
+    # cubic: H = 4Kc/(mu0 Ms) sum_a (m.a)^3 a with a1,a2 rotated about z
+    Kc = 5e4
+    a1 = np.array([1.0, 1.0, 0]) / np.sqrt(2)
+    a2 = np.array([1.0, -1.0, 0]) / np.sqrt(2)
+    a3 = np.cross(a1, a2)
+    mu = np.array([0.3, 0.5, 0.81])
+    s = oracle.OracleSim(1, 1, 1)
+    s.set_Ms([Ms])
+    s.init_m0(mu, norm=False)
+    f, e = s.term_field(s.add_cubic_anis(Kc, tuple(a1), tuple(a2)))
+    exp = 4 * Kc / (MU0 * Ms) * sum((mu @ a) ** 3 * a for a in (a1, a2, a3))
+    assert np.allclose(f, exp, rtol=1e-13)
+    assert np.isclose(e[0], -Kc * sum((mu @ a) ** 4 for a in (a1, a2, a3)) * 1e-27, rtol=1e-13)
+
+
+def test_energy_field_consistency(oracle):
+    """H_i = -(1/(mu0 Ms_i V)) dE/dm_i for every term, by central differences on E."""
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 5, 4, 3
+    N = nx * ny * nz
+    dx, dy, dz = 2e-9, 2.5e-9, 3e-9
+    Ms = rng.uniform(4e5, 9e5, N)
+    Ms[[3, 17]] = 0.0
+    m = rng.standard_normal(3 * N)
+    m /= np.repeat(np.linalg.norm(m.reshape(N, 3), axis=1), 3)
+    makers = [
+        lambda s: s.add_exch(1.3e-11),
+        lambda s: s.add_exch(rng.uniform(1e-11, 2e-11, N)),
+        lambda s: s.add_dmi(3e-3),
+        lambda s: s.add_dmi((rng.uniform(1e-3, 3e-3, N), rng.uniform(1e-3, 3e-3, N), rng.uniform(1e-3, 3e-3, N))),
+        lambda s: s.add_dmi(rng.uniform(1e-3, 3e-3, N), type="interfacial"),
+        lambda s: s.add_anis(rng.uniform(1e5, 5e5, N), axis=(0.2, 0.3, 1.0)),
+        lambda s: s.add_cubic_anis(rng.uniform(-5e4, 5e4, N), (1, 0, 0), (0, 1, 0)),
+        lambda s: s.add_demag(),
+    ]
+    V = dx * dy * dz
+    for mk in makers:
+        s = oracle.OracleSim(nx, ny, nz, dx, dy, dz, pbc="x")
+        s.set_Ms(Ms)
+        s.init_m0(m, norm=False)
+        t = mk(s)
+        f, _ = s.term_field(t, m)
+        for q in rng.choice([i for i in range(3 * N) if Ms[i // 3] > 0], 12, replace=False):
+            h = 1e-6
+            mp, mm = m.copy(), m.copy()
+            mp[q] += h
+            mm[q] -= h
+            dE = (s.term_energies(mp)[t] - s.term_energies(mm)[t]) / (2 * h)
+            Hfd = -dE / (MU0 * Ms[q // 3] * V)
+            assert abs(Hfd - f[q]) <= 1e-6 * max(1.0, np.abs(f).max())
+
+
+def _analytical(alpha, gamma, H0, t):
+    p = gamma / (1 + alpha * alpha)
+    b = p * H0 * t
+    return np.array([np.cos(b) / np.cosh(alpha * b), np.sin(b) / np.cosh(alpha * b), np.tanh(alpha * b)])
+
+
+@pytest.mark.parametrize("kind,tol,err", [(2, 1e-8, 8e-7), (3, 1e-7, 2e-6)])
+def test_llg_adaptive_macrospin(oracle, kind, tol, err):
+    """test/test_llg.jl:23-60: damped precession in 1e5 A/m vs the analytic solution."""
+    s = oracle.OracleSim(1, 1, 1)
+    s.set_Ms([8e5])
+    s.set_llg(0.05, 2.21e5, True)
+    s.add_zeeman([0, 0, 1e5])
+    s.init_m0([1.0, 0, 0])
+    s.set_integrator(kind, 0, tol)
+    for i in range(1, 11):
+        assert s.run_until(1e-10 * i) == 0
+    assert np.all(np.abs(s.spin - _analytical(0.05, 2.21e5, 1e5, 1e-9)) < err)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_llg_fixed_macrospin(oracle, kind):
+    """test/test_llg.jl:62-92: RK4 and Heun, h = 1e-13, 500 steps."""
+    s = oracle.OracleSim(1, 1, 1)
+    s.set_Ms([8e5])
+    s.set_llg(0.05, 2.21e5, True)
+    s.add_exch(1.3e-11)
+    s.add_zeeman([0, 0, 1e5])
+    s.init_m0([1.0, 0, 0])
+    s.set_integrator(kind, 1e-13, 0)
+    for _ in range(500):
+        s.advance_step()
+    assert np.all(np.abs(s.spin - _analytical(0.05, 2.21e5, 1e5, 5e-11)) < 8e-6)
+    assert abs(s.state()["t"] - 5e-11) < 1e-24
+
+
+def test_llg_rhs_invariants(oracle):
+    """dm/dt is perpendicular to m; pinned cells get zero; precession off gives pure damping."""
+    rng = np.random.default_rng(3)
+    N = 40
+    s = oracle.OracleSim(8, 5, 1)
+    s.set_Ms(np.full(N, 8e5))
+    m = rng.standard_normal(3 * N)
+    s.init_m0(m)
+    pins = np.zeros(N, dtype=np.uint8)
+    pins[[0, 7]] = 1
+    s.set_pins(pins)
+    s.add_exch(1.3e-11)
+    s.add_zeeman(rng.standard_normal(3 * N) * 1e5)
+    s.set_llg(0.3, 2.21e5, True)
+    mm = s.spin
+    d = s.llg_rhs().reshape(N, 3)
+    assert np.all(d[[0, 7]] == 0)
+    assert np.max(np.abs(np.sum(d * mm.reshape(N, 3), axis=1))) < 1e-6 * np.abs(d).max()
+    s.set_llg(0.3, 2.21e5, False)
+    H = s.effective_field().reshape(N, 3)
+    d2 = s.llg_rhs().reshape(N, 3)
+    # pure damping: dm/dt . H >= 0 for every free cell
+    assert np.all(np.sum(d2 * H, axis=1)[1:7] >= -1e-9)
+
+
+def test_relax_pins(oracle):
+    """test/test_pins.jl:25-46 (LLG, precession off): pinned cells keep m exactly."""
+    N = 100
+    s = oracle.OracleSim(N, 1, 1, 2e-9, 2e-9, 1e-9)
+    s.set_llg(0.5, 2.21e5, False)
+    s.set_Ms(np.full(N, 8.6e5))
+    pins = np.zeros(N, dtype=np.uint8)
+    pins[[0, N - 1]] = 1
+    s.set_pins(pins)
+    s.add_exch(1.3e-11)
+    m = np.tile([1.0, 1, 1], N)
+    m[0:3] = (0, 0, 1)
+    m[3 * (N - 1):] = (0, 1, 0)
+    s.init_m0(m)
+    s.set_integrator(2, 0, 1e-6)
+    rc, steps, md = s.relax(max_steps=10000, stopping_dmdt=0.1)
+    assert rc == 0
+    sp = s.spin
+    assert abs(sp[0]) < 1e-15 and abs(sp[1]) < 1e-15 and abs(sp[2] - 1) < 1e-15
+    assert md < 0.1 * (2 * np.pi / 360) * 1e9
+
+
+def test_philox_normal_statistics(oracle):
+    z = oracle.philox_normal(1234, 5, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
+    assert not np.array_equal(z[:100], oracle.philox_normal(1234, 6, 100))
