@@ -1,0 +1,140 @@
+/*
+ * mmx.h -- C ABI of libmmx, the MI355X (gfx950) finite-difference field + LLG
+ * hot path that sits behind MicroMagnetic.jl's API (Sim / FDMesh / add_exch /
+ * add_dmi / add_anis / add_cubic_anis / add_zeeman / add_demag /
+ * effective_field / run_until / run_sim / relax).
+ *
+ * Conventions (the reference's, src/micro/mesh.jl:24-29):
+ *   - cell I = (k-1) nx ny + (j-1) nx + i, x fastest; vectors are AoS
+ *     [mx,my,mz] per cell, so a vector array holds 3N doubles;
+ *   - host pointers are borrowed for the duration of the call only;
+ *   - every call is synchronous at the boundary and returns 0 (or a term index
+ *     >= 0 for the add_* calls) on success and a negative MMX_E* status on
+ *     failure; mmx_last_error() then describes it (thread-local);
+ *   - a context owns its device memory and one HIP stream; it is not thread-
+ *     safe, distinct contexts are independent.
+ */
+#ifndef MMX_H
+#define MMX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMX_OK 0
+#define MMX_EINVAL (-1)  /* invalid argument (error(...) / ArgumentError in the reference) */
+#define MMX_EHIP (-2)    /* HIP runtime error */
+#define MMX_ENAN (-3)    /* NaN error norm: src/integrator/adaptive_rk.jl:112-117 */
+#define MMX_ETRIES (-4)  /* more than 50 tries: src/integrator/adaptive_rk.jl:119-122 */
+#define MMX_ESTATE (-5)  /* call not valid in the current state (e.g. run_until with RK4) */
+#define MMX_ENOMEM (-6)  /* device allocation failed */
+
+#define MMX_F64 64
+#define MMX_F32 32
+
+#define MMX_DMI_BULK 0
+#define MMX_DMI_INTERFACIAL 1
+
+#define MMX_RK4 0     /* "RungeKutta"   src/integrator/rk.jl */
+#define MMX_HEUN 1    /* "Heun"         src/integrator/heun.jl */
+#define MMX_DOPRI54 2 /* "DormandPrince" src/integrator/adaptive_rk_steps.jl:49 */
+#define MMX_BS23 3    /* "BS23"         src/integrator/adaptive_rk_steps.jl:11 */
+
+typedef struct mmx_ctx mmx_ctx;
+
+/* FDMesh(; dx, dy, dz, nx, ny, nz, pbc): src/micro/mesh.jl:69-98 */
+typedef struct mmx_mesh_desc {
+    int32_t nx, ny, nz;
+    int32_t pbc;       /* bit 0: x periodic, bit 1: y, bit 2: z */
+    double dx, dy, dz; /* m */
+    int32_t precision; /* MMX_F64 or MMX_F32 (set_precision, src/MicroMagnetic.jl:35) */
+    int32_t device;    /* HIP device ordinal */
+} mmx_mesh_desc;
+
+/* Sim(mesh): src/sim.jl:48-107 (LLG driver, DormandPrince integrator, alpha 0.1, gamma 2.21e5) */
+int mmx_create(mmx_ctx **out, const mmx_mesh_desc *mesh);
+int mmx_destroy(mmx_ctx *ctx);
+const char *mmx_last_error(void);
+
+/* set_Ms(sim, Ms): src/sim.jl:129-143 -- Ms in A/m, N values, stored as mu0*Ms */
+int mmx_set_Ms(mmx_ctx *ctx, const double *Ms);
+/* set_pinning(sim, ids): src/sim.jl:172-177 -- N bytes, nonzero = pinned */
+int mmx_set_pinning(mmx_ctx *ctx, const uint8_t *pins);
+/* init_m0(sim, m0; norm): src/sim.jl:278-309 -- 3N values; normalised if norm,
+ * zeroed where Ms == 0; MMX_EINVAL on NaN */
+int mmx_init_m0(mmx_ctx *ctx, const double *m, int norm);
+/* Array(sim.spin) */
+int mmx_get_m(mmx_ctx *ctx, double *m);
+
+/* add_exch(sim, A): src/micro/add_field.jl:233-263.  A != NULL: per-cell A
+ * (SpatialExchange, harmonic mean); else uniform (Ax, Ay, Az) (UniformExchange). */
+int mmx_add_exch(mmx_ctx *ctx, const double *A, double Ax, double Ay, double Az);
+/* add_dmi(sim, D; type): src/micro/add_field.jl:293-345.
+ * bulk: Dx/Dy/Dz == NULL -> BulkDMI(Dx0, Dy0, Dz0); else SpatialVectorBulkDMI (all three arrays).
+ * interfacial: Dx = per-cell D, or NULL with the uniform value Dx0. */
+int mmx_add_dmi(mmx_ctx *ctx, int type, const double *Dx, const double *Dy, const double *Dz, double Dx0,
+                double Dy0, double Dz0);
+/* add_anis(sim, Ku; axis): src/micro/add_field.jl:430-465.  Ku per cell (or NULL
+ * with the uniform Ku0); axes = 3N per-cell axis (normalised here) or NULL with (ax, ay, az). */
+int mmx_add_anis(mmx_ctx *ctx, const double *Ku, double Ku0, const double *axes, double ax, double ay,
+                 double az);
+/* add_cubic_anis(sim, Kc; axis1, axis2): src/micro/add_field.jl:541-574 */
+int mmx_add_cubic_anis(mmx_ctx *ctx, const double *Kc, double Kc0, const double axis1[3],
+                       const double axis2[3]);
+/* add_zeeman(sim, H0): src/micro/add_field.jl:11-48 -- 3N per-cell field or NULL with (Hx,Hy,Hz) */
+int mmx_add_zeeman(mmx_ctx *ctx, const double *H, double Hx, double Hy, double Hz);
+/* update_zeeman(sim, H0): src/micro/add_field.jl:60-100 */
+int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double Hy, double Hz);
+/* add_demag(sim; Nx, Ny, Nz): src/micro/add_field.jl:403-419 -> init_demag, src/micro/demag.jl:30-100.
+ * The FP64 Newell tensor is built on the host; its transform runs on the GPU in FP64. */
+int mmx_add_demag(mmx_ctx *ctx, int Nx, int Ny, int Nz);
+/* add_thermal_noise(sim, T; T0): src/micro/add_field.jl:734-771.  T per cell (or NULL
+ * with T_uniform).  The noise is Philox4x32-10 keyed by (seed, integrator step, index). */
+int mmx_add_thermal_noise(mmx_ctx *ctx, const double *T, double T_uniform, double T0, uint64_t seed);
+int mmx_num_terms(mmx_ctx *ctx);
+
+/* effective_field(sim, spin, t): src/micro/field.jl:496-505 -- sim.field (3N) */
+int mmx_effective_field(mmx_ctx *ctx, double t, double *H);
+/* effective_field(term, sim, spin, t): term.field (3N) and term.energy (N), either nullable */
+int mmx_term_field(mmx_ctx *ctx, int term, double t, double *H, double *E);
+/* compute_system_energy: src/micro/field.jl:520-536 -- sum(term.energy) per term */
+int mmx_compute_energies(mmx_ctx *ctx, double t, double *E);
+
+/* LLG driver fields alpha, gamma, precession: src/driver.jl:15-21 */
+int mmx_set_llg(mmx_ctx *ctx, double alpha, double gamma, int precession);
+/* llg_call_back(sim, dm_dt, spin, t): src/llg/llg.jl:45-58; m == NULL uses the current state */
+int mmx_llg_rhs(mmx_ctx *ctx, double t, const double *m, double *dmdt);
+
+/* create_driver("LLG", integrator): src/driver.jl:126-145.  step: fixed step of
+ * RK4/Heun; tol: tolerance of DOPRI54/BS23.  Resets t, nsteps and nfevals. */
+int mmx_set_integrator(mmx_ctx *ctx, int kind, double step, double tol);
+/* advance_step(sim) n times: src/integrator/rk.jl:23, heun.jl:28, adaptive_rk.jl:93 */
+int mmx_advance_step(mmx_ctx *ctx, int64_t n);
+/* run_until(sim, t_end): src/sim.jl:547-579 + integrate_to_time, adaptive_rk.jl:168-186 */
+int mmx_run_until(mmx_ctx *ctx, double t_end);
+/* relax(sim; max_steps, stopping_dmdt) with the LLG driver: src/sim.jl:392-490 */
+int mmx_relax(mmx_ctx *ctx, int64_t max_steps, double stopping_dmdt, int64_t *steps, double *max_dmdt);
+/* integrator.t, .step, .step_next, .nsteps, .nfevals: src/integrator/adaptive_rk.jl:37-55 */
+int mmx_get_state(mmx_ctx *ctx, double *t, double *step, double *step_next, int64_t *nsteps,
+                  int64_t *nfevals);
+
+/* diagnostics: the compact FP64 real-space tensor of a demag term (6N values,
+ * xx,yy,zz,xy,xz,yz), the padded FFT geometry and per-pass kernel timers */
+int mmx_demag_tensor(mmx_ctx *ctx, int term, double *out);
+int mmx_demag_geometry(mmx_ctx *ctx, int32_t *Px, int32_t *Py, int32_t *Pz, int32_t *KS);
+int mmx_synchronize(mmx_ctx *ctx);
+#define MMX_PASS_P1 0    /* x real->half-spectrum */
+#define MMX_PASS_P2 1    /* y forward */
+#define MMX_PASS_P3 2    /* z forward, tensor multiply, z inverse */
+#define MMX_PASS_P4 3    /* y inverse */
+#define MMX_PASS_P5 4    /* x inverse + fused local field, LLG and stage update */
+#define MMX_PASS_LOCAL 5 /* local field + LLG + stage update (no demag) */
+#define MMX_NPASS 6
+int mmx_profile(mmx_ctx *ctx, int enable);
+int mmx_profile_read(mmx_ctx *ctx, double *ms /* MMX_NPASS */, int64_t *launches /* MMX_NPASS */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

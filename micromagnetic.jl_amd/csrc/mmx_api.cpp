@@ -1,0 +1,1360 @@
+// mmx_api.cpp -- the C ABI of libmmx (include/mmx.h): context and parameter
+// management, demag set-up, and the integrators that drive the fused kernels.
+//
+// The steppers restate src/integrator/rk.jl (RK4), heun.jl (Heun),
+// adaptive_rk.jl + adaptive_rk_steps.jl (DOPRI54, BS23) and the run loops of
+// src/sim.jl (run_until, relax).  Every stage combination is fused into the
+// last kernel of the RHS evaluation that produces its k (mmx_device.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/mmx.h"
+#include "mmx_common.h"
+#include "mmx_kernels.h"
+
+namespace mmx {
+void newell_tensor(int nx, int ny, int nz, double Dx, double Dy, double Dz, int Nx, int Ny, int Nz,
+                   double *out, int nthreads);
+}
+
+using namespace mmx;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+struct HipError {
+    int code;
+};
+
+#define HIPCHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) {                                                                  \
+            fail(MMX_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            throw HipError{MMX_EHIP};                                                            \
+        }                                                                                        \
+    } while (0)
+
+static constexpr double MU0 = 4.0 * M_PI * 1e-7;  // src/const.jl:1
+static constexpr double KB = 1.3806505e-23;       // src/const.jl:3
+
+template <class X>
+struct DevBuf {
+    X *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(X));
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(MMX_ENOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(X), hipGetErrorString(e));
+            throw HipError{MMX_ENOMEM};
+        }
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+// host-side description of one interaction term (insertion order = index)
+struct HostTerm {
+    int kind = K_NONE;
+    int uniform = 0;
+    double s[12] = {0};
+    int narrays = 0;  // number of device arrays owned
+    double thermal_T0 = 0;
+    uint64_t seed = 0;
+    std::vector<double> rtensor;  // demag: compact real-space tensor (6N)
+};
+
+struct CtxBase {
+    virtual ~CtxBase() = default;
+};
+
+struct mmx_ctx {
+    mmx_mesh_desc mesh;
+    int64_t N;
+    double volume;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<CtxBase> impl;
+    std::vector<HostTerm> terms;
+    std::vector<double> mu0Ms;  // host copy (FP64)
+    // LLG driver and integrator state (src/driver.jl:148, adaptive_rk.jl:37-55)
+    double alpha = 0.1, gamma = 2.21e5;
+    int precession = 1;
+    int ikind = MMX_DOPRI54;
+    double t = 0, step = 0, step_next = 0, tol = 1e-6;
+    int64_t nsteps = 0, nfevals = 0;
+    int succeed = 0;
+    int demag_term = -1;
+    bool profile = false;
+};
+
+namespace {
+
+template <class T>
+struct Engine : CtxBase {
+    using value_type = T;
+    mmx_ctx *C;
+    int64_t N;
+    DevBuf<T> mu0Ms;
+    DevBuf<uint8_t> pins;
+    bool have_pins = false;
+    // state: current spin, three stage buffers, seven k buffers (SoA 3N each)
+    DevBuf<T> ybuf[4];
+    T *y = nullptr;
+    T *stg[3] = {nullptr, nullptr, nullptr};
+    DevBuf<T> kbuf[7];
+    T *k[7] = {nullptr};
+    // per-term device arrays
+    std::vector<std::unique_ptr<DevBuf<T>>> arrays;
+    std::vector<const T *> tA, tB, tC;
+    // demag
+    bool demag = false;
+    DemagGeom G{};
+    DevBuf<cplx<T>> Abuf, Bbuf;
+    DevBuf<T> Kt, Lline;
+    DevBuf<cplx<T>> tw;
+    // reductions and scratch
+    DevBuf<double> partial, red;
+    DevBuf<double> dscratch;  // 3N doubles for AoS transfers
+    // profiling
+    struct Ev {
+        int pass;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> evs;
+    double prof_ms[MMX_NPASS] = {0};
+    int64_t prof_n[MMX_NPASS] = {0};
+
+    explicit Engine(mmx_ctx *c) : C(c), N(c->N)
+    {
+        mu0Ms.alloc(N);
+        HIPCHECK(hipMemsetAsync(mu0Ms.p, 0, N * sizeof(T), C->stream));
+        for (auto &b : ybuf) {
+            b.alloc(3 * N);
+            HIPCHECK(hipMemsetAsync(b.p, 0, 3 * N * sizeof(T), C->stream));
+        }
+        y = ybuf[0].p;
+        for (int i = 0; i < 3; i++) stg[i] = ybuf[i + 1].p;
+        partial.alloc((size_t)max_local_blocks() * MAX_TERMS + 64);
+        red.alloc(64);
+        dscratch.alloc(3 * N);
+    }
+    ~Engine() override
+    {
+        for (auto &e : evs) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+    }
+
+    void ensure_k()
+    {
+        for (int i = 0; i < 7; i++)
+            if (!kbuf[i].p) {
+                kbuf[i].alloc(3 * N);
+                HIPCHECK(hipMemsetAsync(kbuf[i].p, 0, 3 * N * sizeof(T), C->stream));
+                k[i] = kbuf[i].p;
+            }
+    }
+
+    // ---- transfers ---------------------------------------------------------
+    void upload_soa(const double *aos, T *dst)
+    {
+        HIPCHECK(hipMemcpyAsync(dscratch.p, aos, 3 * N * sizeof(double), hipMemcpyHostToDevice, C->stream));
+        launch_aos_to_soa<T>(dscratch.p, dst, N, C->stream);
+        HIPCHECK(hipGetLastError());
+    }
+    void download_aos(const T *src, double *aos)
+    {
+        launch_soa_to_aos<T>(src, dscratch.p, N, C->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(aos, dscratch.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+        HIPCHECK(hipStreamSynchronize(C->stream));
+    }
+    void upload_scalar(const double *h, T *dst, int64_t n)
+    {
+        HIPCHECK(hipMemcpyAsync(dscratch.p, h, n * sizeof(double), hipMemcpyHostToDevice, C->stream));
+        launch_cast<T>(dscratch.p, dst, n, C->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(C->stream));
+    }
+    void download_scalar(const T *src, double *h, int64_t n)
+    {
+        launch_to_double<T>(src, dscratch.p, n, C->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(h, dscratch.p, n * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+        HIPCHECK(hipStreamSynchronize(C->stream));
+    }
+    // a device copy of a per-cell parameter array (n = N or 3N, AoS -> SoA for 3N)
+    const T *new_array(const double *h, int64_t n, bool vec)
+    {
+        auto b = std::make_unique<DevBuf<T>>();
+        b->alloc(n);
+        if (vec) {
+            upload_soa(h, b->p);
+            HIPCHECK(hipStreamSynchronize(C->stream));
+        } else {
+            upload_scalar(h, b->p, n);
+        }
+        const T *p = b->p;
+        arrays.push_back(std::move(b));
+        return p;
+    }
+
+    // ---- parameters of the kernels ----------------------------------------
+    FieldParams<T> params() const
+    {
+        FieldParams<T> P{};
+        P.nx = C->mesh.nx;
+        P.ny = C->mesh.ny;
+        P.nz = C->mesh.nz;
+        P.px = (C->mesh.pbc & 1) != 0;
+        P.py = (C->mesh.pbc & 2) != 0;
+        P.pz = (C->mesh.pbc & 4) != 0;
+        P.N = N;
+        P.volume = (T)C->volume;
+        P.mu0Ms = mu0Ms.p;
+        P.pins = have_pins ? pins.p : nullptr;
+        P.nterms = (int)C->terms.size();
+        for (int i = 0; i < P.nterms; i++) {
+            const HostTerm &h = C->terms[i];
+            TermDev<T> &d = P.t[i];
+            d.kind = h.kind;
+            d.uniform = h.uniform;
+            for (int q = 0; q < 12; q++) d.s[q] = (T)h.s[q];
+            d.a = tA[i];
+            d.b = tB[i];
+            d.c = tC[i];
+        }
+        P.alpha = (T)C->alpha;
+        P.gamma = (T)C->gamma;
+        P.precession = C->precession;
+        // stochastic field factor, src/micro/field.jl:261 (dt = integrator.step)
+        P.thermal_factor = (C->step > 0) ? 2 * C->alpha * KB / (C->volume * C->gamma * C->step) : 0.0;
+        P.noise_step = (uint64_t)C->nsteps;
+        P.seed = 0;
+        for (const auto &h : C->terms)
+            if (h.kind == K_THERMAL) P.seed = h.seed;
+        return P;
+    }
+
+    uint32_t all_mask() const { return (uint32_t)((1ull << C->terms.size()) - 1); }
+
+    // ---- profiling ---------------------------------------------------------
+    void mark(int pass, bool begin)
+    {
+        if (!C->profile) return;
+        if (begin) {
+            Ev e;
+            e.pass = pass;
+            HIPCHECK(hipEventCreate(&e.a));
+            HIPCHECK(hipEventCreate(&e.b));
+            HIPCHECK(hipEventRecord(e.a, C->stream));
+            evs.push_back(e);
+        } else {
+            HIPCHECK(hipEventRecord(evs.back().b, C->stream));
+        }
+    }
+    void collect_profile()
+    {
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        for (auto &e : evs) {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, e.a, e.b));
+            prof_ms[e.pass] += ms;
+            prof_n[e.pass] += 1;
+            HIPCHECK(hipEventDestroy(e.a));
+            HIPCHECK(hipEventDestroy(e.b));
+        }
+        evs.clear();
+    }
+
+    // ---- one RHS evaluation with a fused epilogue; returns partial blocks --
+    int eval(const T *m, const Epi<T> &E)
+    {
+        FieldParams<T> P = params();
+        int nb;
+        if (!demag) {
+            mark(MMX_PASS_LOCAL, true);
+            nb = launch_local<T>(P, E, m, C->stream);
+            mark(MMX_PASS_LOCAL, false);
+        } else {
+            const int nx = G.nx, ny = G.ny, nz = G.nz;
+            mark(MMX_PASS_P1, true);
+            launch_p1<T>(G, m, N, mu0Ms.p, nx, ny, nz, 3, Abuf.p, tw.p, C->stream);
+            mark(MMX_PASS_P1, false);
+            mark(MMX_PASS_P2, true);
+            launch_coly<T>(G, Abuf.p, Bbuf.p, ny, ny, G.Py, G.Py, nz, 3, 0, tw.p, C->stream);
+            mark(MMX_PASS_P2, false);
+            mark(MMX_PASS_P3, true);
+            launch_convz<T>(G, Bbuf.p, Kt.p, Lline.p, nz, nz, 3, 0, tw.p, C->stream);
+            mark(MMX_PASS_P3, false);
+            mark(MMX_PASS_P4, true);
+            launch_coly<T>(G, Bbuf.p, Abuf.p, G.Py, G.Py, ny, ny, nz, 3, 1, tw.p, C->stream);
+            mark(MMX_PASS_P4, false);
+            mark(MMX_PASS_P5, true);
+            nb = launch_p5<T>(G, Abuf.p, P, E, m, tw.p, C->stream);
+            mark(MMX_PASS_P5, false);
+        }
+        HIPCHECK(hipGetLastError());
+        return nb;
+    }
+
+    double reduce_scalar(int nblocks, bool is_max)
+    {
+        launch_reduce(partial.p, nblocks, 1, 1, is_max ? 1 : 0, red.p, C->stream);
+        HIPCHECK(hipGetLastError());
+        double v = 0;
+        HIPCHECK(hipMemcpyAsync(&v, red.p, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        return v;
+    }
+
+    Epi<T> epi(int mode) const
+    {
+        Epi<T> E{};
+        E.mode = mode;
+        E.term_mask = all_mask();
+        E.partial = partial.p;
+        return E;
+    }
+
+    // ---- demag set-up -------------------------------------------------------
+    static int next_pow2(int v)
+    {
+        int p = 1;
+        while (p < v) p <<= 1;
+        return p;
+    }
+    static int ilog2(int v)
+    {
+        int l = 0;
+        while ((1 << l) < v) l++;
+        return l;
+    }
+
+    void setup_demag(const std::vector<double> &rt)
+    {
+        const int nx = C->mesh.nx, ny = C->mesh.ny, nz = C->mesh.nz;
+        G.nx = nx;
+        G.ny = ny;
+        G.nz = nz;
+        G.Px = std::max(2, next_pow2(2 * nx - 1));
+        G.Py = ny == 1 ? 1 : next_pow2(2 * ny - 1);
+        G.Pz = nz == 1 ? 1 : next_pow2(2 * nz - 1);
+        G.H = G.Px / 2;
+        G.Kx = G.H + 1;
+        G.KS = (G.Kx + 15) / 16 * 16;
+        G.lgH = ilog2(G.H);
+        G.lgPy = ilog2(G.Py);
+        G.lgPz = ilog2(G.Pz);
+        G.Lmax = std::max(G.Px, std::max(G.Py, G.Pz));
+        G.lgLmax = ilog2(G.Lmax);
+        G.Py2 = G.Py / 2 + 1;
+        G.Pz2 = G.Pz / 2 + 1;
+        const int cap = sizeof(T) == 8 ? 4096 : 8192;
+        if (3 * (G.H + 1) > cap || G.Py + 1 > cap + 64 || 3 * (G.Pz + 1) > cap) {
+            fail(MMX_EINVAL, "mesh %dx%dx%d exceeds the demag transform limits of this build", nx, ny, nz);
+            throw HipError{MMX_EINVAL};
+        }
+        // FP64 twiddle table W_Lmax^t (long double sincos, rounded once)
+        std::vector<cplx<double>> twd(G.Lmax);
+        for (int t = 0; t < G.Lmax; t++) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)G.Lmax;
+            twd[t] = {(double)cosl(a), (double)sinl(a)};
+        }
+        DevBuf<cplx<double>> twD;
+        twD.alloc(G.Lmax);
+        HIPCHECK(hipMemcpyAsync(twD.p, twd.data(), G.Lmax * sizeof(cplx<double>), hipMemcpyHostToDevice, C->stream));
+        tw.alloc(G.Lmax);
+        if (sizeof(T) == 8) {
+            HIPCHECK(hipMemcpyAsync(tw.p, twD.p, G.Lmax * sizeof(cplx<double>), hipMemcpyDeviceToDevice, C->stream));
+        } else {
+            std::vector<cplx<T>> twf(G.Lmax);
+            for (int t = 0; t < G.Lmax; t++) twf[t] = {(T)twd[t].x, (T)twd[t].y};
+            HIPCHECK(hipMemcpyAsync(tw.p, twf.data(), G.Lmax * sizeof(cplx<T>), hipMemcpyHostToDevice, C->stream));
+            HIPCHECK(hipStreamSynchronize(C->stream));
+        }
+        // k-space tensor: pad + 3-D r2c transform in FP64 on the device, keep the
+        // real part folded to ky <= Py/2, kz <= Pz/2, scaled by -1/(mu0 Px Py Pz)
+        // (irfft normalisation and the -1/mu0 of collect_h, demag.jl:441).
+        const int64_t Ptot = (int64_t)G.Px * G.Py * G.Pz;
+        const int64_t nA = (int64_t)G.Pz * G.Py * G.KS;
+        DevBuf<double> rtD, pad;
+        DevBuf<cplx<double>> At, Bt;
+        rtD.alloc(6 * N);
+        pad.alloc(Ptot);
+        At.alloc(nA);
+        Bt.alloc(nA);
+        HIPCHECK(hipMemcpyAsync(rtD.p, rt.data(), 6 * N * sizeof(double), hipMemcpyHostToDevice, C->stream));
+        Kt.alloc((int64_t)6 * G.Pz2 * G.Py2 * G.KS);
+        HIPCHECK(hipMemsetAsync(Kt.p, 0, Kt.n * sizeof(T), C->stream));
+        const double scale = -1.0 / (MU0 * (double)Ptot);
+        for (int c = 0; c < 6; c++) {
+            launch_tensor_pad(G, rtD.p, c, pad.p, C->stream);
+            launch_p1<double>(G, pad.p, 0, nullptr, G.Px, G.Py, G.Pz, 1, At.p, twD.p, C->stream);
+            launch_coly<double>(G, At.p, Bt.p, G.Py, G.Py, G.Py, G.Py, G.Pz, 1, 0, twD.p, C->stream);
+            launch_convz<double>(G, Bt.p, nullptr, nullptr, G.Pz, G.Pz, 1, 1, twD.p, C->stream);
+            launch_tensor_extract<T>(G, Bt.p, c, scale, Kt.p, C->stream);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        // even cosine transforms of the odd components' on-axis lines (see
+        // k_convz): L(k) = T(0) + 2 sum_{d>=1} T(d) cos(2 pi k d / P), scaled
+        auto line_dft = [&](int comp, int n, int P, int nk, int64_t stride, double *out) {
+            const double *t0 = rt.data() + (int64_t)comp * N;
+            for (int kk = 0; kk < nk; kk++) {
+                long double acc = 0;
+                for (int d = 1; d < n; d++) {
+                    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)((int64_t)kk * d % P) / P;
+                    acc += (long double)t0[d * stride] * cosl(a);
+                }
+                out[kk] = (double)((long double)t0[0] + 2.0L * acc) * scale;
+            }
+        };
+        {
+            std::vector<double> L(G.Pz2 + G.Py2 + G.Kx);
+            line_dft(3, nz, G.Pz, G.Pz2, (int64_t)nx * ny, L.data());                // xy line along z
+            line_dft(4, ny, G.Py, G.Py2, nx, L.data() + G.Pz2);                     // xz line along y
+            line_dft(5, nx, G.Px, G.Kx, 1, L.data() + G.Pz2 + G.Py2);               // yz line along x
+            DevBuf<double> Ld;  // staging sized for L (may exceed the 3N scratch on tiny meshes)
+            Ld.alloc(L.size());
+            HIPCHECK(hipMemcpyAsync(Ld.p, L.data(), L.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+            Lline.alloc(L.size());
+            launch_cast<T>(Ld.p, Lline.p, (int64_t)L.size(), C->stream);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(C->stream));
+        }
+        // the last pass writes one reduction slot group per block
+        const int rpb = std::max(1, cap / (3 * (G.H + 1)));
+        const int64_t p5_blocks = ((int64_t)ny * nz + rpb - 1) / rpb;
+        const int64_t slots = std::max<int64_t>(max_local_blocks(), p5_blocks) * MAX_TERMS + 64;
+        if ((int64_t)partial.n < slots) partial.alloc(slots);
+        Abuf.alloc((int64_t)3 * nz * ny * G.KS);
+        Bbuf.alloc((int64_t)3 * nz * G.Py * G.KS);
+        HIPCHECK(hipMemsetAsync(Abuf.p, 0, Abuf.n * sizeof(cplx<T>), C->stream));
+        HIPCHECK(hipMemsetAsync(Bbuf.p, 0, Bbuf.n * sizeof(cplx<T>), C->stream));
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        demag = true;
+    }
+
+    // ---- integrators --------------------------------------------------------
+    void swap_y(T *&other)
+    {
+        T *t = y;
+        y = other;
+        other = t;
+    }
+
+    // RungeKutta.advance_step: src/integrator/rk.jl:23-52
+    void rk4_step()
+    {
+        ensure_k();
+        const double h = C->step;
+        T *a = stg[0], *b = stg[1];
+        Epi<T> E = epi(EPI_STAGE);
+        E.y0 = y;
+        E.kstore = k[0];
+        E.cself = 0.5 * h;
+        E.yout = a;
+        eval(y, E);
+        E.kstore = k[1];
+        E.yout = b;
+        eval(a, E);
+        E.kstore = k[2];
+        E.cself = h;
+        E.yout = a;
+        eval(b, E);
+        Epi<T> F = epi(EPI_RK4_FINAL);
+        F.y0 = y;
+        F.kp[0] = k[0];
+        F.kp[1] = k[1];
+        F.kp[2] = k[2];
+        F.np = 3;
+        F.cself = 1.0 / 6 * h;
+        F.yout = b;
+        F.normalize = 1;
+        eval(a, F);
+        swap_y(stg[1]);  // new state in b; the previous state stays in stg[1] (prespin)
+        C->nfevals += 4;
+        C->nsteps += 1;
+        C->t = C->nsteps * h;
+    }
+
+    // ModifiedEuler.advance_step: src/integrator/heun.jl:28-49
+    void heun_step()
+    {
+        ensure_k();
+        const double h = C->step;
+        T *a = stg[0], *b = stg[1];
+        Epi<T> E = epi(EPI_STAGE);
+        E.y0 = y;
+        E.kstore = k[0];
+        E.cself = h;
+        E.yout = a;
+        eval(y, E);
+        Epi<T> F = epi(EPI_STAGE);
+        F.y0 = y;
+        F.kp[0] = k[0];
+        F.cp[0] = 0.5 * h;
+        F.np = 1;
+        F.cself = 0.5 * h;
+        F.yout = b;
+        F.normalize = 1;
+        eval(a, F);
+        swap_y(stg[1]);
+        C->nfevals += 2;
+        C->nsteps += 1;
+        C->t = C->nsteps * h;
+    }
+
+    // rk_step_dopri54!: src/integrator/adaptive_rk_steps.jl:49-105; the new
+    // state is left in stg[1]; returns the scaled error norm
+    double dopri54_try(double step)
+    {
+        static const double a1 = 1.0 / 5;
+        static const double a2[2] = {3.0 / 40, 9.0 / 40};
+        static const double a3[3] = {44.0 / 45, -56.0 / 15, 32.0 / 9};
+        static const double a4[4] = {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729};
+        static const double a5[5] = {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656};
+        static const double a6[6] = {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84};
+        static const double w[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525,
+                                    -1.0 / 40};
+        ensure_k();
+        T *A = stg[0], *B = stg[1];
+        Epi<T> E = epi(EPI_STAGE);
+        E.y0 = y;
+        // k1 -> y + a1 h k1
+        E.kstore = k[0];
+        E.np = 0;
+        E.cself = a1 * step;
+        E.yout = A;
+        eval(y, E);
+        // k2
+        E.kstore = k[1];
+        E.kp[0] = k[0];
+        E.cp[0] = a2[0] * step;
+        E.np = 1;
+        E.cself = a2[1] * step;
+        E.yout = B;
+        eval(A, E);
+        // k3
+        E.kstore = k[2];
+        E.kp[1] = k[1];
+        E.cp[0] = a3[0] * step;
+        E.cp[1] = a3[1] * step;
+        E.np = 2;
+        E.cself = a3[2] * step;
+        E.yout = A;
+        eval(B, E);
+        // k4
+        E.kstore = k[3];
+        E.kp[2] = k[2];
+        for (int q = 0; q < 3; q++) E.cp[q] = a4[q] * step;
+        E.np = 3;
+        E.cself = a4[3] * step;
+        E.yout = B;
+        eval(A, E);
+        // k5
+        E.kstore = k[4];
+        E.kp[3] = k[3];
+        for (int q = 0; q < 4; q++) E.cp[q] = a5[q] * step;
+        E.np = 4;
+        E.cself = a5[4] * step;
+        E.yout = A;
+        eval(B, E);
+        // k6 -> y_next = y + a61 k1 + a63 k3 + a64 k4 + a65 k5 + a66 k6, normalised
+        E.kstore = k[5];
+        E.kp[0] = k[0];
+        E.kp[1] = k[2];
+        E.kp[2] = k[3];
+        E.kp[3] = k[4];
+        E.cp[0] = a6[0] * step;
+        E.cp[1] = a6[2] * step;
+        E.cp[2] = a6[3] * step;
+        E.cp[3] = a6[4] * step;
+        E.np = 4;
+        E.cself = a6[5] * step;
+        E.yout = B;
+        E.normalize = 1;
+        eval(A, E);
+        // k7 at y_next and the error combination (vector_add6b)
+        Epi<T> R = epi(EPI_ERR);
+        R.kstore = k[6];
+        R.kp[0] = k[0];
+        R.kp[1] = k[2];
+        R.kp[2] = k[3];
+        R.kp[3] = k[4];
+        R.kp[4] = k[5];
+        R.cp[0] = w[0];
+        R.cp[1] = w[2];
+        R.cp[2] = w[3];
+        R.cp[3] = w[4];
+        R.cp[4] = w[5];
+        R.np = 5;
+        R.cself = w[6];
+        int nb = eval(B, R);
+        C->nfevals += 7;
+        return reduce_scalar(nb, true) * step / C->tol;
+    }
+
+    // rk_step_bs23!: src/integrator/adaptive_rk_steps.jl:11-47 (FSAL: k1 <- k4)
+    double bs23_try(double step, double t)
+    {
+        ensure_k();
+        T *A = stg[0], *B = stg[1];
+        Epi<T> E = epi(EPI_STAGE);
+        E.y0 = y;
+        if (C->succeed && t >= C->t) {
+            std::swap(k[0], k[3]);
+            // y_temp = y + (step*0.5) k1 without a new evaluation
+            axpy(y, step * 0.5, k[0], A);
+        } else {
+            E.kstore = k[0];
+            E.cself = step * 0.5;
+            E.yout = A;
+            eval(y, E);
+            C->nfevals += 1;
+        }
+        // k2 -> y + (step*0.75) k2
+        E.kstore = k[1];
+        E.np = 0;
+        E.cself = step * 0.75;
+        E.yout = B;
+        eval(A, E);
+        // k3 -> y_next = y + step (2/9 k1 + 1/3 k2 + 4/9 k3), normalised
+        Epi<T> S = epi(EPI_STAGE_SCALED);
+        S.y0 = y;
+        S.kstore = k[2];
+        S.kp[0] = k[0];
+        S.kp[1] = k[1];
+        S.cp[0] = 2.0 / 9;
+        S.cp[1] = 1.0 / 3;
+        S.np = 2;
+        S.cself = 4.0 / 9;
+        S.outer = step;
+        S.yout = A;
+        S.normalize = 1;
+        eval(B, S);
+        // k4 at y_next and the error combination
+        Epi<T> R = epi(EPI_ERR);
+        R.kstore = k[3];
+        R.kp[0] = k[0];
+        R.kp[1] = k[1];
+        R.kp[2] = k[2];
+        R.cp[0] = -(5.0 / 72);
+        R.cp[1] = 1.0 / 12;
+        R.cp[2] = 1.0 / 9;
+        R.np = 3;
+        R.cself = -1.0 / 8;
+        int nb = eval(A, R);
+        C->nfevals += 3;
+        // keep the candidate in stg[1] as the DOPRI path does
+        std::swap(stg[0], stg[1]);
+        return reduce_scalar(nb, true) * step / C->tol;
+    }
+
+    void axpy(const T *yv, double c, const T *kv, T *out);
+
+    // compute_initial_step: src/integrator/adaptive_rk.jl:151-165
+    double initial_step(double dt, int order_main)
+    {
+        ensure_k();
+        double abs_step = dt, abs_step_tmp = dt;
+        C->step = 1e-15;
+        Epi<T> E = epi(EPI_RHS);
+        E.out = k[6];
+        eval(y, E);
+        double mx = maxabs(k[6]);
+        double r_step = mx / (0.824 * std::pow(C->tol, 1.0 / order_main));
+        C->nfevals += 1;
+        if (abs_step * r_step > 0.001) abs_step_tmp = 0.001 / r_step;
+        return std::min(abs_step, abs_step_tmp);
+    }
+
+    double maxabs(const T *v);
+
+    // AdaptiveRK.advance_step: src/integrator/adaptive_rk.jl:93-148
+    int adaptive_step()
+    {
+        const int max_nsteps = 50;
+        const double t = C->t;
+        int order_main = 5, order = 4;
+        if (C->ikind == MMX_BS23) {
+            order_main = 3;
+            order = 2;
+        }
+        if (C->step_next <= 0) C->step_next = initial_step(1e-12, order_main);
+        C->step = C->step_next;
+        int nstep = 1;
+        for (;;) {
+            double max_error = C->ikind == MMX_BS23 ? bs23_try(C->step, t) : dopri54_try(C->step);
+            if (std::isnan(max_error)) {
+                C->step_next = 1e-14;
+                return fail(MMX_ENAN, "NaN error detected, reducing step size");
+            }
+            nstep += 1;
+            if (nstep > max_nsteps) return fail(MMX_ETRIES, "Too many inner steps (%d), system may not converge!", nstep);
+            C->succeed = (max_error <= 1.0);
+            if (C->succeed) {
+                swap_y(stg[1]);
+                C->nsteps += 1;
+                C->t += C->step;
+                double factor = 0.824 * std::pow(1.0 / max_error, 1.0 / (order + 1));
+                C->step_next = C->step * std::min(5.0, std::max(0.2, factor));
+                break;
+            } else {
+                double factor = 0.824 * std::pow(1.0 / max_error, 1.0 / order);
+                C->step = C->step * std::min(5.0, std::max(0.2, factor));
+            }
+        }
+        return MMX_OK;
+    }
+
+    int advance(int64_t n)
+    {
+        for (int64_t i = 0; i < n; i++) {
+            if (C->ikind == MMX_RK4) rk4_step();
+            else if (C->ikind == MMX_HEUN) heun_step();
+            else {
+                int rc = adaptive_step();
+                if (rc != MMX_OK) return rc;
+            }
+        }
+        return MMX_OK;
+    }
+
+    double max_dm()
+    {
+        int nb = launch_maxdiff<T>(stg[1], y, N, partial.p, C->stream);
+        HIPCHECK(hipGetLastError());
+        return reduce_scalar(nb, true);
+    }
+};
+
+template <class T>
+void Engine<T>::axpy(const T *yv, double c, const T *kv, T *out)
+{
+    launch_axpy<T>(yv, c, kv, out, 3 * N, C->stream);
+    HIPCHECK(hipGetLastError());
+}
+
+template <class T>
+double Engine<T>::maxabs(const T *v)
+{
+    int g = launch_maxabs<T>(v, 3 * N, partial.p, C->stream);
+    HIPCHECK(hipGetLastError());
+    return reduce_scalar(g, true);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// dispatch helpers
+// ---------------------------------------------------------------------------
+template <class F>
+static int with_engine(mmx_ctx *c, F &&f)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    try {
+        if (c->mesh.precision == MMX_F64) return f(*static_cast<Engine<double> *>(c->impl.get()));
+        return f(*static_cast<Engine<float> *>(c->impl.get()));
+    } catch (const HipError &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return fail(MMX_ENOMEM, "host allocation failed");
+    }
+}
+
+static bool finite_all(const double *a, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++)
+        if (std::isnan(a[i])) return false;
+    return true;
+}
+
+template <class T>
+static int add_term(mmx_ctx *c, Engine<T> &eng, HostTerm h, const typename Engine<T>::value_type *a,
+                    const typename Engine<T>::value_type *b, const typename Engine<T>::value_type *cc)
+{
+    if ((int)c->terms.size() >= MAX_TERMS) return fail(MMX_EINVAL, "at most %d interaction terms", MAX_TERMS);
+    c->terms.push_back(h);
+    eng.tA.push_back(a);
+    eng.tB.push_back(b);
+    eng.tC.push_back(cc);
+    return (int)c->terms.size() - 1;
+}
+
+template <class T>
+static void ensure_out(Engine<T> &e)
+{
+    e.ensure_k();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *mmx_last_error(void) { return g_err.c_str(); }
+
+int mmx_create(mmx_ctx **out, const mmx_mesh_desc *mesh)
+{
+    if (!out || !mesh) return fail(MMX_EINVAL, "null argument");
+    *out = nullptr;
+    if (mesh->nx < 1 || mesh->ny < 1 || mesh->nz < 1) return fail(MMX_EINVAL, "mesh sizes must be >= 1");
+    if (!(mesh->dx > 0 && mesh->dy > 0 && mesh->dz > 0)) return fail(MMX_EINVAL, "cell sizes must be > 0");
+    if (mesh->precision != MMX_F64 && mesh->precision != MMX_F32)
+        return fail(MMX_EINVAL, "precision must be 64 or 32");
+    int64_t N = (int64_t)mesh->nx * mesh->ny * mesh->nz;
+    if (N > (int64_t)1 << 31) return fail(MMX_EINVAL, "mesh too large");
+    auto c = std::make_unique<mmx_ctx>();
+    c->mesh = *mesh;
+    c->N = N;
+    c->volume = mesh->dx * mesh->dy * mesh->dz;  // src/micro/mesh.jl:86
+    c->mu0Ms.assign(N, 0.0);
+    try {
+        HIPCHECK(hipSetDevice(mesh->device));
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        if (mesh->precision == MMX_F64) c->impl.reset(new Engine<double>(c.get()));
+        else c->impl.reset(new Engine<float>(c.get()));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        return e.code;
+    }
+    *out = c.release();
+    return MMX_OK;
+}
+
+int mmx_destroy(mmx_ctx *c)
+{
+    if (!c) return MMX_OK;
+    (void)hipSetDevice(c->mesh.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->impl.reset();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MMX_OK;
+}
+
+int mmx_set_Ms(mmx_ctx *c, const double *Ms)
+{
+    if (!c || !Ms) return fail(MMX_EINVAL, "null argument");
+    if (!finite_all(Ms, c->N)) return fail(MMX_EINVAL, "NaN is given by the input Ms!");
+    return with_engine(c, [&](auto &e) {
+        for (int64_t i = 0; i < c->N; i++) c->mu0Ms[i] = Ms[i] * MU0;
+        e.upload_scalar(c->mu0Ms.data(), e.mu0Ms.p, c->N);
+        return MMX_OK;
+    });
+}
+
+int mmx_set_pinning(mmx_ctx *c, const uint8_t *pins)
+{
+    if (!c || !pins) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) {
+        if (!e.pins.p) e.pins.alloc(c->N);
+        HIPCHECK(hipMemcpyAsync(e.pins.p, pins, c->N, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        e.have_pins = true;
+        return MMX_OK;
+    });
+}
+
+int mmx_init_m0(mmx_ctx *c, const double *m, int norm)
+{
+    if (!c || !m) return fail(MMX_EINVAL, "null argument");
+    const int64_t N = c->N;
+    std::vector<double> s(m, m + 3 * N);
+    if (norm) {  // normalise: src/helper.jl:255-271
+        for (int64_t i = 0; i < N; i++) {
+            double *v = &s[3 * i];
+            double m2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (m2 > 0) {
+                double len = 1 / std::sqrt(m2);
+                v[0] *= len;
+                v[1] *= len;
+                v[2] *= len;
+            }
+        }
+    }
+    for (int64_t i = 0; i < N; i++)
+        if (c->mu0Ms[i] == 0) s[3 * i] = s[3 * i + 1] = s[3 * i + 2] = 0;
+    if (!finite_all(s.data(), 3 * N)) return fail(MMX_EINVAL, "NaN is given by the input m0!");
+    return with_engine(c, [&](auto &e) {
+        e.upload_soa(s.data(), e.y);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->succeed = 0;
+        return MMX_OK;
+    });
+}
+
+int mmx_get_m(mmx_ctx *c, double *m)
+{
+    if (!c || !m) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) {
+        e.download_aos(e.y, m);
+        return MMX_OK;
+    });
+}
+
+int mmx_add_exch(mmx_ctx *c, const double *A, double Ax, double Ay, double Az)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;
+        const double dx = c->mesh.dx, dy = c->mesh.dy, dz = c->mesh.dz;
+        if (A) {  // SpatialExchange: nabla = 2/d^2 (kernels.jl:231-233)
+            h.kind = K_EXCH_S;
+            h.s[0] = (double)(TT(2) / ((TT)dx * (TT)dx));
+            h.s[1] = (double)(TT(2) / ((TT)dy * (TT)dy));
+            h.s[2] = (double)(TT(2) / ((TT)dz * (TT)dz));
+            const TT *a = e.new_array(A, c->N, false);
+            return add_term(c, e, h, a, nullptr, nullptr);
+        }
+        h.kind = K_EXCH;  // UniformExchange: 2A/d^2 (kernels.jl:277-279)
+        h.s[0] = (double)(TT(2) * (TT)Ax / ((TT)dx * (TT)dx));
+        h.s[1] = (double)(TT(2) * (TT)Ay / ((TT)dy * (TT)dy));
+        h.s[2] = (double)(TT(2) * (TT)Az / ((TT)dz * (TT)dz));
+        return add_term(c, e, h, nullptr, nullptr, nullptr);
+    });
+}
+
+int mmx_add_dmi(mmx_ctx *c, int type, const double *Dx, const double *Dy, const double *Dz, double Dx0,
+                double Dy0, double Dz0)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (type != MMX_DMI_BULK && type != MMX_DMI_INTERFACIAL)
+        return fail(MMX_EINVAL, "Supported DMI type: interfacial bulk");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;
+        const TT dx = (TT)c->mesh.dx, dy = (TT)c->mesh.dy, dz = (TT)c->mesh.dz;
+        if (type == MMX_DMI_INTERFACIAL) {  // kernels.jl:426-466, Dd = 1/d
+            h.kind = K_DMI_INT;
+            h.s[0] = (double)(TT(1) / dx);
+            h.s[1] = (double)(TT(1) / dy);
+            if (!Dx) {
+                h.uniform = 1;
+                h.s[3] = (double)(TT)Dx0;
+                return add_term(c, e, h, nullptr, nullptr, nullptr);
+            }
+            const TT *a = e.new_array(Dx, c->N, false);
+            return add_term(c, e, h, a, nullptr, nullptr);
+        }
+        if (!Dx && !Dy && !Dz) {  // BulkDMI: Ds = D/d (kernels.jl:313)
+            h.kind = K_DMI_BULK;
+            h.s[0] = (double)((TT)Dx0 / dx);
+            h.s[1] = (double)((TT)Dy0 / dy);
+            h.s[2] = (double)((TT)Dz0 / dz);
+            return add_term(c, e, h, nullptr, nullptr, nullptr);
+        }
+        if (!Dx || !Dy || !Dz) return fail(MMX_EINVAL, "spatial bulk DMI needs all three arrays");
+        h.kind = K_DMI_BULK_S;  // axes = 1/d (kernels.jl:356)
+        h.s[0] = (double)(TT(1) / dx);
+        h.s[1] = (double)(TT(1) / dy);
+        h.s[2] = (double)(TT(1) / dz);
+        const TT *a = e.new_array(Dx, c->N, false);
+        const TT *b = e.new_array(Dy, c->N, false);
+        const TT *cc = e.new_array(Dz, c->N, false);
+        return add_term(c, e, h, a, b, cc);
+    });
+}
+
+int mmx_add_anis(mmx_ctx *c, const double *Ku, double Ku0, const double *axes, double ax, double ay, double az)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;
+        const TT *ka = nullptr;
+        if (Ku) ka = e.new_array(Ku, c->N, false);
+        else {
+            h.uniform = 1;
+            h.s[3] = (double)(TT)Ku0;
+        }
+        if (axes) {  // spatial axis, normalised (add_field.jl:447-453)
+            std::vector<double> ax3(axes, axes + 3 * c->N);
+            for (int64_t i = 0; i < c->N; i++) {
+                double *v = &ax3[3 * i];
+                double m2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (m2 > 0) {
+                    double len = 1 / std::sqrt(m2);
+                    v[0] *= len;
+                    v[1] *= len;
+                    v[2] *= len;
+                }
+            }
+            h.kind = K_ANIS_S;
+            const TT *b = e.new_array(ax3.data(), 3 * c->N, true);
+            return add_term(c, e, h, ka, b, nullptr);
+        }
+        double lt = std::sqrt(ax * ax + ay * ay + az * az);  // add_field.jl:441-443
+        if (!(lt > 0)) return fail(MMX_EINVAL, "anisotropy axis must be nonzero");
+        h.kind = K_ANIS;
+        h.s[0] = (double)(TT)(ax / lt);
+        h.s[1] = (double)(TT)(ay / lt);
+        h.s[2] = (double)(TT)(az / lt);
+        return add_term(c, e, h, ka, nullptr, nullptr);
+    });
+}
+
+int mmx_add_cubic_anis(mmx_ctx *c, const double *Kc, double Kc0, const double a1[3], const double a2[3])
+{
+    if (!c || !a1 || !a2) return fail(MMX_EINVAL, "null argument");
+    double n1 = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+    double n2 = std::sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+    double na1[3] = {a1[0] / n1, a1[1] / n1, a1[2] / n1};
+    double na2[3] = {a2[0] / n2, a2[1] / n2, a2[2] / n2};
+    if (std::fabs(na1[0] * na2[0] + na1[1] * na2[1] + na1[2] * na2[2]) > 1e-10)
+        return fail(MMX_EINVAL, "The axis1 and axis2 are not perpendicular to each other!");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;
+        h.kind = K_CUBIC;
+        // axis3 = axis1 x axis2 of the inputs as given (add_field.jl:555)
+        double a3[3] = {a1[1] * a2[2] - a1[2] * a2[1], a1[2] * a2[0] - a1[0] * a2[2], a1[0] * a2[1] - a1[1] * a2[0]};
+        for (int q = 0; q < 3; q++) {
+            h.s[q] = (double)(TT)na1[q];
+            h.s[3 + q] = (double)(TT)na2[q];
+            h.s[6 + q] = (double)(TT)a3[q];
+        }
+        const TT *ka = nullptr;
+        if (Kc) ka = e.new_array(Kc, c->N, false);
+        else {
+            h.uniform = 1;
+            h.s[9] = (double)(TT)Kc0;
+        }
+        return add_term(c, e, h, ka, nullptr, nullptr);
+    });
+}
+
+int mmx_add_zeeman(mmx_ctx *c, const double *H, double Hx, double Hy, double Hz)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;
+        if (!H) {
+            h.kind = K_ZEEMAN_U;
+            h.uniform = 1;
+            h.s[0] = (double)(TT)Hx;
+            h.s[1] = (double)(TT)Hy;
+            h.s[2] = (double)(TT)Hz;
+            return add_term(c, e, h, nullptr, nullptr, nullptr);
+        }
+        h.kind = K_ZEEMAN;
+        const TT *p = e.new_array(H, 3 * c->N, true);
+        return add_term(c, e, h, p, p + c->N, p + 2 * c->N);
+    });
+}
+
+int mmx_update_zeeman(mmx_ctx *c, int term, const double *H, double Hx, double Hy, double Hz)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (term < 0 || term >= (int)c->terms.size()) return fail(MMX_EINVAL, "no such term");
+    HostTerm &h = c->terms[term];
+    if (h.kind != K_ZEEMAN && h.kind != K_ZEEMAN_U) return fail(MMX_EINVAL, "term %d is not a Zeeman term", term);
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        if (!H) {
+            h.kind = K_ZEEMAN_U;
+            h.uniform = 1;
+            h.s[0] = (double)(TT)Hx;
+            h.s[1] = (double)(TT)Hy;
+            h.s[2] = (double)(TT)Hz;
+            return MMX_OK;
+        }
+        const TT *p = e.new_array(H, 3 * c->N, true);
+        h.kind = K_ZEEMAN;
+        h.uniform = 0;
+        e.tA[term] = p;
+        e.tB[term] = p + c->N;
+        e.tC[term] = p + 2 * c->N;
+        return MMX_OK;
+    });
+}
+
+int mmx_add_demag(mmx_ctx *c, int Nx, int Ny, int Nz)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (c->demag_term >= 0) return fail(MMX_EINVAL, "only one demag term per simulation");
+    if (Nx < 0 || Ny < 0 || Nz < 0) return fail(MMX_EINVAL, "image counts must be >= 0");
+    return with_engine(c, [&](auto &e) {
+        HostTerm h;
+        h.kind = K_DEMAG;
+        h.rtensor.resize(6 * c->N);
+        newell_tensor(c->mesh.nx, c->mesh.ny, c->mesh.nz, c->mesh.dx, c->mesh.dy, c->mesh.dz, Nx, Ny, Nz,
+                      h.rtensor.data(), 0);
+        e.setup_demag(h.rtensor);
+        int r = add_term(c, e, h, nullptr, nullptr, nullptr);
+        if (r >= 0) c->demag_term = r;
+        return r;
+    });
+}
+
+int mmx_add_thermal_noise(mmx_ctx *c, const double *T, double T_uniform, double T0, uint64_t seed)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    for (const auto &h : c->terms)
+        if (h.kind == K_THERMAL) return fail(MMX_EINVAL, "only one thermal term per simulation");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;
+        h.kind = K_THERMAL;
+        h.s[0] = (double)(TT)T0;
+        h.seed = seed;
+        const TT *a = nullptr;
+        if (T) a = e.new_array(T, c->N, false);
+        else {
+            h.uniform = 1;
+            h.s[1] = (double)(TT)T_uniform;
+        }
+        return add_term(c, e, h, a, nullptr, nullptr);
+    });
+}
+
+int mmx_num_terms(mmx_ctx *c) { return c ? (int)c->terms.size() : fail(MMX_EINVAL, "null context"); }
+
+int mmx_effective_field(mmx_ctx *c, double t, double *H)
+{
+    (void)t;
+    return with_engine(c, [&](auto &e) {
+        ensure_out(e);
+        auto E = e.epi(EPI_FIELD);
+        E.out = e.k[6];
+        e.eval(e.y, E);
+        if (H) e.download_aos(e.k[6], H);
+        else HIPCHECK(hipStreamSynchronize(c->stream));
+        return MMX_OK;
+    });
+}
+
+int mmx_term_field(mmx_ctx *c, int term, double t, double *H, double *Eo)
+{
+    (void)t;
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (term < 0 || term >= (int)c->terms.size()) return fail(MMX_EINVAL, "no such term");
+    return with_engine(c, [&](auto &e) {
+        ensure_out(e);
+        auto E = e.epi(EPI_FIELD);
+        E.term_mask = 1u << term;
+        E.out = e.k[6];
+        E.e_out = e.k[5];
+        e.eval(e.y, E);
+        if (H) e.download_aos(e.k[6], H);
+        if (Eo) e.download_scalar(e.k[5], Eo, c->N);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return MMX_OK;
+    });
+}
+
+int mmx_compute_energies(mmx_ctx *c, double t, double *Eo)
+{
+    (void)t;
+    if (!c || !Eo) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) {
+        auto E = e.epi(EPI_ENERGY);
+        int nb = e.eval(e.y, E);
+        int nt = (int)c->terms.size();
+        launch_reduce(e.partial.p, nb, MAX_TERMS, nt, 0, e.red.p, c->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(Eo, e.red.p, nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return MMX_OK;
+    });
+}
+
+int mmx_set_llg(mmx_ctx *c, double alpha, double gamma, int precession)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    c->alpha = alpha;
+    c->gamma = gamma;
+    c->precession = precession ? 1 : 0;
+    return MMX_OK;
+}
+
+int mmx_llg_rhs(mmx_ctx *c, double t, const double *m, double *dmdt)
+{
+    (void)t;
+    if (!c || !dmdt) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) {
+        ensure_out(e);
+        const auto *src = e.y;
+        if (m) {
+            e.upload_soa(m, e.stg[2]);
+            src = e.stg[2];
+        }
+        auto E = e.epi(EPI_RHS);
+        E.out = e.k[6];
+        e.eval(src, E);
+        e.download_aos(e.k[6], dmdt);
+        return MMX_OK;
+    });
+}
+
+int mmx_set_integrator(mmx_ctx *c, int kind, double step, double tol)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (kind < MMX_RK4 || kind > MMX_BS23) return fail(MMX_EINVAL, "unknown integrator %d", kind);
+    if ((kind == MMX_RK4 || kind == MMX_HEUN) && !(step > 0)) return fail(MMX_EINVAL, "fixed-step integrators need step > 0");
+    if ((kind == MMX_DOPRI54 || kind == MMX_BS23) && !(tol > 0)) return fail(MMX_EINVAL, "tol must be > 0");
+    c->ikind = kind;
+    c->t = 0;
+    c->nsteps = 0;
+    c->nfevals = 0;
+    c->succeed = 0;
+    c->step_next = 0;
+    c->tol = tol;
+    c->step = (kind == MMX_RK4 || kind == MMX_HEUN) ? step : 0.0;
+    return MMX_OK;
+}
+
+int mmx_advance_step(mmx_ctx *c, int64_t n)
+{
+    return with_engine(c, [&](auto &e) {
+        int rc = e.advance(n);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return rc;
+    });
+}
+
+int mmx_run_until(mmx_ctx *c, double t_end)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (c->ikind == MMX_RK4 || c->ikind == MMX_HEUN)
+        return fail(MMX_ESTATE, "run_until needs an adaptive integrator (integrate_to_time is defined for AdaptiveRK only)");
+    return with_engine(c, [&](auto &e) {
+        if (t_end < c->t - c->step) return MMX_OK;  // sim.jl:549-551
+        if (t_end == c->t) return MMX_OK;
+        while (c->t < t_end) {  // integrate_to_time: adaptive_rk.jl:168-186
+            if (c->step_next + c->t > t_end) c->step_next = t_end - c->t;
+            int rc = e.adaptive_step();
+            if (rc != MMX_OK) return rc;
+        }
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return MMX_OK;
+    });
+}
+
+int mmx_relax(mmx_ctx *c, int64_t max_steps, double stopping_dmdt, int64_t *steps, double *max_dmdt_out)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    return with_engine(c, [&](auto &e) {
+        const double dmdt_factor = (2 * M_PI / 360) * 1e9;  // sim.jl:406
+        double max_dmdt = 0;
+        int64_t i;
+        int rc = MMX_OK;
+        for (i = 1; i <= max_steps; i++) {
+            rc = e.advance(1);
+            if (rc != MMX_OK) break;
+            max_dmdt = e.max_dm() / c->step;  // compute_dm!: helper.jl:273-286
+            if (max_dmdt < stopping_dmdt * dmdt_factor) break;
+        }
+        if (steps) *steps = i > max_steps ? max_steps : i;
+        if (max_dmdt_out) *max_dmdt_out = max_dmdt;
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return rc;
+    });
+}
+
+int mmx_get_state(mmx_ctx *c, double *t, double *step, double *step_next, int64_t *nsteps, int64_t *nfevals)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (t) *t = c->t;
+    if (step) *step = c->step;
+    if (step_next) *step_next = c->step_next;
+    if (nsteps) *nsteps = c->nsteps;
+    if (nfevals) *nfevals = c->nfevals;
+    return MMX_OK;
+}
+
+int mmx_demag_tensor(mmx_ctx *c, int term, double *out)
+{
+    if (!c || !out) return fail(MMX_EINVAL, "null argument");
+    if (term < 0 || term >= (int)c->terms.size() || c->terms[term].kind != K_DEMAG)
+        return fail(MMX_EINVAL, "term %d is not a demag term", term);
+    std::memcpy(out, c->terms[term].rtensor.data(), 6 * c->N * sizeof(double));
+    return MMX_OK;
+}
+
+int mmx_demag_geometry(mmx_ctx *c, int32_t *Px, int32_t *Py, int32_t *Pz, int32_t *KS)
+{
+    return with_engine(c, [&](auto &e) {
+        if (!e.demag) return fail(MMX_ESTATE, "no demag term");
+        if (Px) *Px = e.G.Px;
+        if (Py) *Py = e.G.Py;
+        if (Pz) *Pz = e.G.Pz;
+        if (KS) *KS = e.G.KS;
+        return MMX_OK;
+    });
+}
+
+int mmx_synchronize(mmx_ctx *c)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    try {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    } catch (const HipError &e) {
+        return e.code;
+    }
+    return MMX_OK;
+}
+
+int mmx_profile(mmx_ctx *c, int enable)
+{
+    return with_engine(c, [&](auto &e) {
+        e.collect_profile();
+        for (int i = 0; i < MMX_NPASS; i++) {
+            e.prof_ms[i] = 0;
+            e.prof_n[i] = 0;
+        }
+        c->profile = enable != 0;
+        return MMX_OK;
+    });
+}
+
+int mmx_profile_read(mmx_ctx *c, double *ms, int64_t *launches)
+{
+    return with_engine(c, [&](auto &e) {
+        e.collect_profile();
+        for (int i = 0; i < MMX_NPASS; i++) {
+            if (ms) ms[i] = e.prof_ms[i];
+            if (launches) launches[i] = e.prof_n[i];
+        }
+        return MMX_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// mmx_common.h -- types shared by the HIP kernels and the host library.
+//
+// Device data layout (all per-cell arrays are x-fastest, then y, then z, the
+// cell order of src/micro/mesh.jl:24-29):
+//   m, k_s, y  : structure-of-arrays, three planes of N values (mx | my | mz)
+//   mu0Ms      : N values (mu0 * Ms, as src/sim.jl:138 stores it)
+//   pins       : N bytes
+// Precision T is double (FP64 path) or float (FP32 path).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mmx {
+
+constexpr int MAX_TERMS = 10;
+constexpr int BLOCK = 256;
+
+enum TermKind : int {
+    K_NONE = 0,
+    K_ZEEMAN,       // static per-cell field (three planes in a/b/c)
+    K_ZEEMAN_U,     // static uniform field s[0..2]
+    K_EXCH,         // uniform exchange, s = {ax, ay, az} = 2A/d^2
+    K_EXCH_S,       // spatial A (a), s = {2/dx^2, 2/dy^2, 2/dz^2}
+    K_DMI_BULK,     // uniform bulk DMI, s = {Dx/dx, Dy/dy, Dz/dz}
+    K_DMI_BULK_S,   // spatial bulk DMI (a,b,c = Dx,Dy,Dz), s = {1/dx, 1/dy, 1/dz}
+    K_DMI_INT,      // interfacial DMI, per-cell D (a) or uniform s[3] when a == null
+    K_ANIS,         // uniaxial, Ku (a) or uniform s[3]; axis s[0..2]
+    K_ANIS_S,       // uniaxial with per-cell axis planes (b), Ku (a) or s[3]
+    K_CUBIC,        // cubic, Kc (a) or uniform s[9]; axes s[0..8]
+    K_DEMAG,        // FFT demag; its field comes from the convolution passes
+    K_THERMAL,      // stochastic field, T (a) or uniform s[1]; s[0] = T0
+};
+
+template <class T>
+struct TermDev {
+    int kind;
+    int uniform;     // parameter given as a scalar instead of an array
+    T s[12];
+    const T *a, *b, *c;
+};
+
+template <class T>
+struct FieldParams {
+    int nx, ny, nz;
+    int px, py, pz;  // periodic flags
+    int64_t N;
+    T volume;
+    const T *mu0Ms;
+    const uint8_t *pins;
+    int nterms;
+    TermDev<T> t[MAX_TERMS];
+    // LLG driver (src/llg/llg.jl:6-40)
+    T alpha, gamma;
+    int precession;
+    // thermal noise stream: factor = 2 alpha kB / (V gamma dt) (src/micro/field.jl:261)
+    double thermal_factor;
+    uint64_t seed;
+    uint64_t noise_step;
+};
+
+// What the last pass does with the field of a cell.
+enum EpiMode : int {
+    EPI_FIELD = 0,   // write H (3 planes) and, if e != null, per-cell energy of the masked terms
+    EPI_RHS,         // write dm/dt
+    EPI_STAGE,       // k = dm/dt; optionally store k; y_out = y0 + sum cp_j kp_j + cself k
+    EPI_STAGE_SCALED,// y_out = y0 + outer * (sum cp_j kp_j + cself k)    (BS23 stage 3)
+    EPI_RK4_FINAL,   // y_out = y0 + cself * (((k1 + 2 k2) + 2 k3) + k)  (src/integrator/rk.jl:46)
+    EPI_ERR,         // store k; err = sum cp_j kp_j + cself k; block max of |err|/(1+|y_in|)
+    EPI_ENERGY,      // per-term energy block sums (FP64)
+};
+
+template <class T>
+struct Epi {
+    int mode;
+    uint32_t term_mask;  // which terms enter the field (bit per term index)
+    T *out;              // FIELD: H planes; RHS: dm/dt planes
+    T *e_out;            // FIELD: per-cell energy (nullable)
+    T *kstore;           // store k (nullable)
+    const T *y0;         // base state of the stage combination
+    const T *kp[6];
+    double cp[6];
+    int np;
+    double cself;
+    double outer;
+    T *yout;
+    int normalize;
+    double *partial;     // one slot per block (ERR: max; ENERGY: nterms slots per block)
+};
+
+template <class T>
+struct cplx {
+    T x, y;
+};
+
+template <class T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+template <class T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b)
+{
+    return {a.x + b.x, a.y + b.y};
+}
+
+template <class T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b)
+{
+    return {a.x - b.x, a.y - b.y};
+}
+
+// Demag convolution geometry.  Padded lengths are powers of two >= 2n-1
+// (any such length gives the same linear convolution as the reference's
+// 2n / 2n-1 padding of src/micro/demag.jl:41-44).  x is transformed real-to-
+// complex: H = Px/2 complex points per row, Kx = H+1 output columns, rows of
+// the intermediate arrays are KS >= Kx complex values apart.
+struct DemagGeom {
+    int nx, ny, nz;
+    int Px, Py, Pz;
+    int H, Kx, KS;
+    int lgH, lgPy, lgPz;
+    int Lmax, lgLmax;   // twiddle table length
+    int Py2, Pz2;       // Py/2+1, Pz/2+1 (folded tensor extents)
+};
+
+}  // namespace mmx

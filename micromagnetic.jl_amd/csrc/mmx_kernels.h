@@ -1,0 +1,59 @@
+// mmx_kernels.h -- host-side launchers of the HIP kernels (mmx_kernels.hip).
+#pragma once
+#include "mmx_common.h"
+
+namespace mmx {
+
+// conversions between the caller's AoS layout and the device SoA planes
+template <class T>
+void launch_aos_to_soa(const double *aos_dev, T *soa, int64_t N, hipStream_t s);
+template <class T>
+void launch_soa_to_aos(const T *soa, double *aos_dev, int64_t N, hipStream_t s);
+template <class T>
+void launch_cast(const double *src, T *dst, int64_t n, hipStream_t s);
+template <class T>
+void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s);
+
+// local field (no demag) + epilogue over all cells; returns number of blocks
+// used (the number of partial-reduction slots written).
+template <class T>
+int launch_local(const FieldParams<T> &P, const Epi<T> &E, const T *m, hipStream_t s);
+
+// demag passes
+template <class T>
+void launch_p1(const DemagGeom &G, const T *in, int64_t cstride, const T *scale, int n_in, int Ry,
+               int Rz, int ncomp, cplx<T> *A, const cplx<T> *tw, hipStream_t s);
+template <class T>
+void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, int Rin, int n_out,
+                 int Rout, int nplanes, int ncomp, int inverse, const cplx<T> *tw, hipStream_t s);
+template <class T>
+void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in,
+                  int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s);
+template <class T>
+int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> &P, const Epi<T> &E,
+              const T *m, const cplx<T> *tw, hipStream_t s);
+
+// tensor set-up (FP64)
+void launch_tensor_pad(const DemagGeom &G, const double *rt, int comp, double *pad, hipStream_t s);
+template <class T>
+void launch_tensor_extract(const DemagGeom &G, const cplx<double> *Bt, int comp, double scale, T *Kt,
+                           hipStream_t s);
+
+// reductions of per-block partials: out[v] = reduce over blocks of partial[b*stride+v]
+void launch_reduce(const double *partial, int nblocks, int stride, int nval, int is_max, double *out,
+                   hipStream_t s);
+
+// max_i |a_i - b_i| over cells (norm of the 3-vector difference), for relax
+template <class T>
+int launch_maxdiff(const T *a, const T *b, int64_t N, double *partial, hipStream_t s);
+
+// out = y + c k over n values (BS23 first-same-as-last stage)
+template <class T>
+void launch_axpy(const T *y, double c, const T *k, T *out, int64_t n, hipStream_t s);
+// per-block max |v_i| over n values; returns the number of blocks
+template <class T>
+int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s);
+
+int max_local_blocks();
+
+}  // namespace mmx

@@ -1,0 +1,417 @@
+"""mmx -- MicroMagnetic.jl's finite-difference API over the MI355X library.
+
+The names, argument meanings and error behaviour mirror the reference's Julia
+API for this path (src/micro/mesh.jl FDMesh, src/sim.jl Sim / set_Ms / init_m0
+/ set_pinning / relax / run_until / run_sim, src/micro/add_field.jl add_*,
+src/micro/field.jl effective_field / compute_system_energy), so that code
+written against the reference reads the same here.  All arithmetic happens in
+libmmx (HIP kernels for gfx950); this module only expands parameters the way
+the reference's init_scalar! / init_vector! do (src/helper.jl:5-102) and moves
+arrays across the C ABI.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import BS23, DOPRI54, F32, F64, HEUN, RK4, MmxError, check, dptr
+
+mu_0 = 4 * math.pi * 1e-7       # src/const.jl:1
+k_B = 1.3806505e-23             # src/const.jl:3
+mT = 0.001 / mu_0               # src/const.jl:15
+
+_INTEGRATORS = {"RungeKutta": RK4, "Heun": HEUN, "DormandPrince": DOPRI54, "DOPRI54": DOPRI54,
+                "BS23": BS23}
+# create_driver defaults: src/driver.jl:126-145
+_DEFAULT_STEP = {RK4: 5e-14, HEUN: 1e-14}
+
+
+class FDMesh:
+    """FDMesh(; dx, dy, dz, nx, ny, nz, pbc, x0, y0, z0): src/micro/mesh.jl:69-98."""
+
+    def __init__(self, dx=1e-9, dy=1e-9, dz=1e-9, nx=1, ny=1, nz=1, pbc="open", x0=None, y0=None, z0=None):
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.xperiodic, self.yperiodic, self.zperiodic = "x" in pbc, "y" in pbc, "z" in pbc
+        self.x0 = -self.nx * self.dx / 2 if x0 is None else float(x0)
+        self.y0 = -self.ny * self.dy / 2 if y0 is None else float(y0)
+        self.z0 = -self.nz * self.dz / 2 if z0 is None else float(z0)
+        self.n_total = self.nx * self.ny * self.nz
+        self.volume = self.dx * self.dy * self.dz
+        self.regions = np.zeros(self.n_total, dtype=np.int32)
+
+    @property
+    def pbc_mask(self):
+        return int(self.xperiodic) | (int(self.yperiodic) << 1) | (int(self.zperiodic) << 2)
+
+
+def _ijk(mesh):
+    k, j, i = np.meshgrid(np.arange(1, mesh.nz + 1), np.arange(1, mesh.ny + 1), np.arange(1, mesh.nx + 1),
+                          indexing="ij")
+    return i.reshape(-1), j.reshape(-1), k.reshape(-1)
+
+
+def init_scalar(mesh, v, dtype=np.float64):
+    """init_scalar!: src/helper.jl:5-44 (number, array, f(i,j,k,dx,dy,dz), f(x,y,z), f(region))."""
+    N = mesh.n_total
+    if callable(v):
+        import inspect
+        nargs = len(inspect.signature(v).parameters)
+        i, j, k = _ijk(mesh)
+        if nargs == 6:
+            out = [v(a, b, c, mesh.dx, mesh.dy, mesh.dz) for a, b, c in zip(i, j, k)]
+        elif nargs == 3:
+            out = [v(mesh.x0 + (a - 0.5) * mesh.dx, mesh.y0 + (b - 0.5) * mesh.dy, mesh.z0 + (c - 0.5) * mesh.dz)
+                   for a, b, c in zip(i, j, k)]
+        elif nargs == 1:
+            out = [v(r) for r in mesh.regions]
+        else:
+            raise ValueError("The input function should have 6, 3 or 1 arguments.")
+        return np.asarray(out, dtype=dtype)
+    a = np.asarray(v, dtype=dtype)
+    if a.ndim == 0:
+        return np.full(N, a, dtype=dtype)
+    a = np.ascontiguousarray(a.reshape(-1))
+    if a.size != N:
+        raise ValueError(f"expected {N} values, got {a.size}")
+    return a
+
+
+def init_vector(mesh, v):
+    """init_vector!: src/helper.jl:64-102, :239-253 (tuple, array, f(i,j,k,...), f(x,y,z))."""
+    N = mesh.n_total
+    if callable(v):
+        import inspect
+        nargs = len(inspect.signature(v).parameters)
+        i, j, k = _ijk(mesh)
+        out = np.zeros((N, 3))
+        for q, (a, b, c) in enumerate(zip(i, j, k)):
+            if nargs == 6:
+                val = v(a, b, c, mesh.dx, mesh.dy, mesh.dz)
+            elif nargs == 3:  # centred coordinates, helper.jl:84-86
+                val = v((a - 0.5 - mesh.nx / 2) * mesh.dx, (b - 0.5 - mesh.ny / 2) * mesh.dy,
+                        (c - 0.5 - mesh.nz / 2) * mesh.dz)
+            else:
+                raise ValueError("The input function should have either 6 or 3 arguments.")
+            if val is not None:
+                out[q] = val
+        out = out.reshape(-1)
+        if np.isnan(out).any():
+            raise ValueError("NaN is given by the input function.")
+        return out
+    a = np.asarray(v, dtype=np.float64)
+    if a.size == 3:
+        return np.tile(a.reshape(3), N)
+    a = np.ascontiguousarray(a.reshape(-1))
+    if a.size != 3 * N:
+        raise ValueError(f"expected {3 * N} values, got {a.size}")
+    return a
+
+
+class Term:
+    """One interaction term (the reference's Zeeman / UniformExchange / ... structs)."""
+
+    def __init__(self, sim, index, name):
+        self.sim, self.index, self.name = sim, index, name
+
+    @property
+    def field(self):
+        """term.field after effective_field(term, sim, spin, t) (3N, AoS)."""
+        return self.sim._term_field(self.index)[0]
+
+    @property
+    def energy(self):
+        return self.sim._term_field(self.index)[1]
+
+
+class LLGDriver:
+    """The LLG driver fields (src/driver.jl:15-21); writes go to the library."""
+
+    def __init__(self, sim):
+        object.__setattr__(self, "_sim", sim)
+        object.__setattr__(self, "alpha", 0.1)
+        object.__setattr__(self, "gamma", 2.21e5)
+        object.__setattr__(self, "precession", True)
+
+    def __setattr__(self, k, v):
+        object.__setattr__(self, k, v)
+        if k in ("alpha", "gamma", "precession"):
+            self._sim._push_llg()
+
+    @property
+    def integrator(self):
+        return self._sim._integrator
+
+
+class Integrator:
+    def __init__(self, sim, kind, step, tol):
+        object.__setattr__(self, "_sim", sim)
+        object.__setattr__(self, "kind", kind)
+        object.__setattr__(self, "_step", step)
+        object.__setattr__(self, "_tol", tol)
+
+    def _state(self):
+        t, st, sn = C.c_double(), C.c_double(), C.c_double()
+        ns, nf = C.c_int64(), C.c_int64()
+        check(_lib.load().mmx_get_state(self._sim._h, C.byref(t), C.byref(st), C.byref(sn), C.byref(ns),
+                                        C.byref(nf)))
+        return t.value, st.value, sn.value, ns.value, nf.value
+
+    t = property(lambda self: self._state()[0])
+    step_next = property(lambda self: self._state()[2])
+    nsteps = property(lambda self: self._state()[3])
+    nfevals = property(lambda self: self._state()[4])
+
+    @property
+    def step(self):
+        return self._state()[1]
+
+    def __setattr__(self, k, v):
+        if k == "step":
+            object.__setattr__(self, "_step", float(v))
+            check(_lib.load().mmx_set_integrator(self._sim._h, self.kind, float(v), self._tol))
+        elif k == "tol":
+            object.__setattr__(self, "_tol", float(v))
+            check(_lib.load().mmx_set_integrator(self._sim._h, self.kind, self._step, float(v)))
+        else:
+            object.__setattr__(self, k, v)
+
+
+class Sim:
+    """Sim(mesh; driver="LLG", integrator="DormandPrince"): src/sim.jl:48-107."""
+
+    def __init__(self, mesh, driver="LLG", name="dyn", integrator="DormandPrince", precision=F64, device=0):
+        if driver != "LLG":
+            raise ValueError(f"driver {driver!r} is not available on this path (LLG only)")
+        if integrator not in _INTEGRATORS:
+            raise ValueError("Supported integrators: " + " ".join(_INTEGRATORS))
+        self.mesh, self.name = mesh, name
+        self.n_total = mesh.n_total
+        self.precision = precision
+        L = _lib.load()
+        d = _lib.MeshDesc(mesh.nx, mesh.ny, mesh.nz, mesh.pbc_mask, mesh.dx, mesh.dy, mesh.dz, int(precision),
+                          int(device))
+        h = C.c_void_p()
+        check(L.mmx_create(C.byref(h), C.byref(d)))
+        self._h = h
+        self.interactions = []
+        self.mu0_Ms = np.zeros(self.n_total)
+        self.driver = LLGDriver(self)
+        kind = _INTEGRATORS[integrator]
+        self._integrator = Integrator(self, kind, _DEFAULT_STEP.get(kind, 0.0), 1e-6)
+        check(L.mmx_set_integrator(self._h, kind, _DEFAULT_STEP.get(kind, 0.0), 1e-6))
+        self._push_llg()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().mmx_destroy(h)
+            self._h = None
+
+    def _push_llg(self):
+        d = self.driver
+        check(_lib.load().mmx_set_llg(self._h, float(d.alpha), float(d.gamma), int(bool(d.precession))))
+
+    def _term_field(self, index):
+        N = self.n_total
+        f = np.empty(3 * N)
+        e = np.empty(N)
+        check(_lib.load().mmx_term_field(self._h, index, 0.0, dptr(f), dptr(e)))
+        return f, e
+
+    def _add(self, index, name):
+        t = Term(self, check(index), name)
+        self.interactions.append(t)
+        return t
+
+    @property
+    def spin(self):
+        out = np.empty(3 * self.n_total)
+        check(_lib.load().mmx_get_m(self._h, dptr(out)))
+        return out
+
+    @property
+    def field(self):
+        return effective_field(self)
+
+
+# ---- the reference's free functions ----------------------------------------
+def set_Ms(sim, Ms):
+    a = init_scalar(sim.mesh, Ms)
+    if np.isnan(a).any():
+        raise ValueError("NaN is given by the input Ms!")
+    check(_lib.load().mmx_set_Ms(sim._h, dptr(a)))
+    sim.mu0_Ms = a * mu_0
+    return True
+
+
+def set_pinning(sim, ids):
+    p = np.ascontiguousarray(init_scalar(sim.mesh, ids, dtype=np.float64) != 0, dtype=np.uint8)
+    check(_lib.load().mmx_set_pinning(sim._h, p.ctypes.data))
+    return True
+
+
+def init_m0(sim, m0, norm=True):
+    m = np.ascontiguousarray(init_vector(sim.mesh, m0), dtype=np.float64)
+    check(_lib.load().mmx_init_m0(sim._h, dptr(m), int(bool(norm))))
+    return True
+
+
+def add_exch(sim, A, name="exch"):
+    L = _lib.load()
+    if np.ndim(A) == 0 and not callable(A):
+        return sim._add(L.mmx_add_exch(sim._h, None, float(A), float(A), float(A)), name)
+    if isinstance(A, tuple) and len(A) == 3:
+        return sim._add(L.mmx_add_exch(sim._h, None, *map(float, A)), name)
+    a = init_scalar(sim.mesh, A)
+    return sim._add(L.mmx_add_exch(sim._h, dptr(a), 0.0, 0.0, 0.0), name)
+
+
+def add_dmi(sim, D, name="dmi", type="bulk"):
+    L = _lib.load()
+    if type == "interfacial":
+        if np.ndim(D) == 0 and not callable(D):
+            return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_INTERFACIAL, None, None, None, float(D), 0.0, 0.0), name)
+        a = init_scalar(sim.mesh, D)
+        return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_INTERFACIAL, dptr(a), None, None, 0.0, 0.0, 0.0), name)
+    if type == "D2d":
+        if np.ndim(D) != 0:
+            raise ValueError("D2d only support uniform DMI!")
+        return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, -float(D), float(D), 0.0), name)
+    if type != "bulk":
+        raise ValueError("Supported DMI type: interfacial bulk D2d")
+    if np.ndim(D) == 0 and not callable(D):
+        return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, float(D), float(D), float(D)), name)
+    if isinstance(D, tuple) and len(D) == 3:
+        if all(np.ndim(d) == 0 and not callable(d) for d in D):
+            return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, *map(float, D)), name)
+        Dx, Dy, Dz = (init_scalar(sim.mesh, d) for d in D)
+    else:
+        Dx = Dy = Dz = init_scalar(sim.mesh, D)
+    return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, dptr(Dx), dptr(Dy), dptr(Dz), 0.0, 0.0, 0.0), name)
+
+
+def add_anis(sim, Ku, axis=(0, 0, 1), name="anis"):
+    L = _lib.load()
+    Kua = init_scalar(sim.mesh, Ku)
+    if isinstance(axis, tuple):
+        return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), 0.0, None, *map(float, axis)), name)
+    ax = np.ascontiguousarray(init_vector(sim.mesh, axis))
+    return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), 0.0, dptr(ax), 0.0, 0.0, 0.0), name)
+
+
+def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
+    L = _lib.load()
+    a = init_scalar(sim.mesh, Kc)
+    a1 = np.array(axis1, dtype=np.float64)
+    a2 = np.array(axis2, dtype=np.float64)
+    return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), 0.0, dptr(a1), dptr(a2)), name)
+
+
+def add_zeeman(sim, H0, name="zeeman"):
+    L = _lib.load()
+    if isinstance(H0, tuple) and len(H0) == 3:
+        return sim._add(L.mmx_add_zeeman(sim._h, None, *map(float, H0)), name)
+    H = np.ascontiguousarray(init_vector(sim.mesh, H0))
+    return sim._add(L.mmx_add_zeeman(sim._h, dptr(H), 0.0, 0.0, 0.0), name)
+
+
+def update_zeeman(sim, H0, name="zeeman"):
+    L = _lib.load()
+    for t in sim.interactions:
+        if t.name == name:
+            if isinstance(H0, tuple) and len(H0) == 3:
+                check(L.mmx_update_zeeman(sim._h, t.index, None, *map(float, H0)))
+            else:
+                H = np.ascontiguousarray(init_vector(sim.mesh, H0))
+                check(L.mmx_update_zeeman(sim._h, t.index, dptr(H), 0.0, 0.0, 0.0))
+            return True
+    raise ValueError(f"no Zeeman term named {name!r}")
+
+
+def add_demag(sim, name="demag", Nx=0, Ny=0, Nz=0, fft=True):
+    if not fft:
+        raise ValueError("the direct O(N^2) demag is not part of this path; use fft=True")
+    return sim._add(_lib.load().mmx_add_demag(sim._h, int(Nx), int(Ny), int(Nz)), name)
+
+
+def add_thermal_noise(sim, Temp, name="thermal", T0=0.0, seed=0):
+    L = _lib.load()
+    if np.ndim(Temp) == 0 and not callable(Temp):
+        return sim._add(L.mmx_add_thermal_noise(sim._h, None, float(Temp), float(T0), int(seed)), name)
+    a = init_scalar(sim.mesh, Temp)
+    return sim._add(L.mmx_add_thermal_noise(sim._h, dptr(a), 0.0, float(T0), int(seed)), name)
+
+
+def effective_field(sim, spin=None, t=0.0):
+    """effective_field(sim, spin, t): src/micro/field.jl:496-505; returns sim.field."""
+    if spin is not None:
+        raise ValueError("pass the state through init_m0; the library evaluates the current spin")
+    out = np.empty(3 * sim.n_total)
+    check(_lib.load().mmx_effective_field(sim._h, float(t), dptr(out)))
+    return out
+
+
+def compute_system_energy(sim, t=0.0):
+    """compute_system_energy: src/micro/field.jl:520-536; returns sum(term.energy) per term."""
+    n = len(sim.interactions)
+    out = np.empty(max(n, 1))
+    check(_lib.load().mmx_compute_energies(sim._h, float(t), dptr(out)))
+    return out[:n]
+
+
+def llg_rhs(sim, spin=None, t=0.0):
+    out = np.empty(3 * sim.n_total)
+    m = None if spin is None else np.ascontiguousarray(spin, dtype=np.float64)
+    check(_lib.load().mmx_llg_rhs(sim._h, float(t), dptr(m), dptr(out)))
+    return out
+
+
+def set_driver(sim, driver="LLG", integrator="DormandPrince", step=None, tol=1e-6):
+    if driver != "LLG":
+        raise ValueError("LLG only")
+    kind = _INTEGRATORS[integrator]
+    st = _DEFAULT_STEP.get(kind, 0.0) if step is None else float(step)
+    sim._integrator = Integrator(sim, kind, st, tol)
+    check(_lib.load().mmx_set_integrator(sim._h, kind, st, float(tol)))
+
+
+def run_step(sim, driver=None, n=1):
+    check(_lib.load().mmx_advance_step(sim._h, int(n)))
+    return True
+
+
+advance_step = run_step
+
+
+def run_until(sim, t_end, save_data=False):
+    check(_lib.load().mmx_run_until(sim._h, float(t_end)))
+
+
+def run_sim(sim, steps=10, dt=1e-10, save_data=False, call_back=None):
+    """run_sim: src/sim.jl:813-857 (data saving is not part of this path)."""
+    for i in range(0, steps + 1):
+        run_until(sim, i * dt)
+        if call_back is not None:
+            call_back(sim, i * dt)
+
+
+def relax(sim, max_steps=10000, stopping_dmdt=0.01):
+    st = C.c_int64()
+    md = C.c_double()
+    check(_lib.load().mmx_relax(sim._h, int(max_steps), float(stopping_dmdt), C.byref(st), C.byref(md)))
+    return st.value, md.value
+
+
+def average_m(sim):
+    """average_m: src/sim.jl:251-255 (weighted by mu0 Ms)."""
+    b = sim.spin.reshape(-1, 3)
+    ms = sim.mu0_Ms
+    return tuple((b * ms[:, None]).sum(axis=0) / ms.sum())
+
+
+def demag_tensor(sim, term):
+    out = np.empty(6 * sim.n_total)
+    check(_lib.load().mmx_demag_tensor(sim._h, term.index, dptr(out)))
+    return out.reshape(6, sim.n_total)
