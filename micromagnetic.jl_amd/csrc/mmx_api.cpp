@@ -143,6 +143,10 @@ struct Engine : CtxBase {
     DevBuf<cplx<T>> Abuf, Bbuf;
     DevBuf<T> Kt, Lline;
     DevBuf<cplx<T>> tw;
+    // kernel parameter block (device copy, re-uploaded when it changes)
+    DevBuf<FieldParams<T>> Pdev;
+    FieldParams<T> Plast{};
+    bool Pvalid = false;
     // reductions and scratch
     DevBuf<double> partial, red;
     DevBuf<double> dscratch;  // 3N doubles for AoS transfers
@@ -165,7 +169,7 @@ struct Engine : CtxBase {
         }
         y = ybuf[0].p;
         for (int i = 0; i < 3; i++) stg[i] = ybuf[i + 1].p;
-        partial.alloc((size_t)max_local_blocks() * MAX_TERMS + 64);
+        partial.alloc((size_t)max_local_blocks() + 64);
         red.alloc(64);
         dscratch.alloc(3 * N);
     }
@@ -259,13 +263,33 @@ struct Engine : CtxBase {
         P.alpha = (T)C->alpha;
         P.gamma = (T)C->gamma;
         P.precession = C->precession;
-        // stochastic field factor, src/micro/field.jl:261 (dt = integrator.step)
-        P.thermal_factor = (C->step > 0) ? 2 * C->alpha * KB / (C->volume * C->gamma * C->step) : 0.0;
-        P.noise_step = (uint64_t)C->nsteps;
         P.seed = 0;
         for (const auto &h : C->terms)
             if (h.kind == K_THERMAL) P.seed = h.seed;
         return P;
+    }
+
+    Dyn dyn() const
+    {
+        Dyn d{};
+        // stochastic field factor, src/micro/field.jl:261 (dt = integrator.step)
+        d.thermal_factor = (C->step > 0) ? 2 * C->alpha * KB / (C->volume * C->gamma * C->step) : 0.0;
+        d.noise_step = (uint64_t)C->nsteps;
+        return d;
+    }
+
+    // device copy of the parameter block, refreshed when it changed
+    const FieldParams<T> *params_dev()
+    {
+        FieldParams<T> P = params();
+        if (!Pvalid || std::memcmp(&P, &Plast, sizeof P) != 0) {
+            if (!Pdev.p) Pdev.alloc(1);
+            HIPCHECK(hipStreamSynchronize(C->stream));
+            HIPCHECK(hipMemcpy(Pdev.p, &P, sizeof P, hipMemcpyHostToDevice));
+            Plast = P;
+            Pvalid = true;
+        }
+        return Pdev.p;
     }
 
     uint32_t all_mask() const { return (uint32_t)((1ull << C->terms.size()) - 1); }
@@ -302,11 +326,12 @@ struct Engine : CtxBase {
     // ---- one RHS evaluation with a fused epilogue; returns partial blocks --
     int eval(const T *m, const Epi<T> &E)
     {
-        FieldParams<T> P = params();
+        const FieldParams<T> *P = params_dev();
+        const Dyn d = dyn();
         int nb;
         if (!demag) {
             mark(MMX_PASS_LOCAL, true);
-            nb = launch_local<T>(P, E, m, C->stream);
+            nb = launch_local<T>(N, P, d, E, m, C->stream);
             mark(MMX_PASS_LOCAL, false);
         } else {
             const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -323,7 +348,7 @@ struct Engine : CtxBase {
             launch_coly<T>(G, Bbuf.p, Abuf.p, G.Py, G.Py, ny, ny, nz, 3, 1, tw.p, C->stream);
             mark(MMX_PASS_P4, false);
             mark(MMX_PASS_P5, true);
-            nb = launch_p5<T>(G, Abuf.p, P, E, m, tw.p, C->stream);
+            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, tw.p, C->stream);
             mark(MMX_PASS_P5, false);
         }
         HIPCHECK(hipGetLastError());
@@ -382,8 +407,7 @@ struct Engine : CtxBase {
         G.lgLmax = ilog2(G.Lmax);
         G.Py2 = G.Py / 2 + 1;
         G.Pz2 = G.Pz / 2 + 1;
-        const int cap = sizeof(T) == 8 ? 4096 : 8192;
-        if (3 * (G.H + 1) > cap || G.Py + 1 > cap + 64 || 3 * (G.Pz + 1) > cap) {
+        if (G.H > 2048 || G.Py > 2048 || G.Pz > 2048) {
             fail(MMX_EINVAL, "mesh %dx%dx%d exceeds the demag transform limits of this build", nx, ny, nz);
             throw HipError{MMX_EINVAL};
         }
@@ -456,9 +480,7 @@ struct Engine : CtxBase {
             HIPCHECK(hipStreamSynchronize(C->stream));
         }
         // the last pass writes one reduction slot group per block
-        const int rpb = std::max(1, cap / (3 * (G.H + 1)));
-        const int64_t p5_blocks = ((int64_t)ny * nz + rpb - 1) / rpb;
-        const int64_t slots = std::max<int64_t>(max_local_blocks(), p5_blocks) * MAX_TERMS + 64;
+        const int64_t slots = std::max<int64_t>(max_local_blocks(), p5_blocks(G)) + 64;
         if ((int64_t)partial.n < slots) partial.alloc(slots);
         Abuf.alloc((int64_t)3 * nz * ny * G.KS);
         Bbuf.alloc((int64_t)3 * nz * G.Py * G.KS);
@@ -1185,13 +1207,12 @@ int mmx_compute_energies(mmx_ctx *c, double t, double *Eo)
     (void)t;
     if (!c || !Eo) return fail(MMX_EINVAL, "null argument");
     return with_engine(c, [&](auto &e) {
-        auto E = e.epi(EPI_ENERGY);
-        int nb = e.eval(e.y, E);
-        int nt = (int)c->terms.size();
-        launch_reduce(e.partial.p, nb, MAX_TERMS, nt, 0, e.red.p, c->stream);
-        HIPCHECK(hipGetLastError());
-        HIPCHECK(hipMemcpyAsync(Eo, e.red.p, nt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
+        // one masked evaluation per term; each block writes its FP64 partial sum
+        for (int t = 0; t < (int)c->terms.size(); t++) {
+            auto E = e.epi(EPI_ENERGY);
+            E.term_mask = 1u << t;
+            Eo[t] = e.reduce_scalar(e.eval(e.y, E), false);
+        }
         return MMX_OK;
     });
 }

@@ -52,10 +52,14 @@ struct FieldParams {
     // LLG driver (src/llg/llg.jl:6-40)
     T alpha, gamma;
     int precession;
-    // thermal noise stream: factor = 2 alpha kB / (V gamma dt) (src/micro/field.jl:261)
-    double thermal_factor;
-    uint64_t seed;
-    uint64_t noise_step;
+    uint64_t seed;  // thermal noise stream
+};
+
+// Per-launch values (by value; the parameter block above lives in device
+// memory and is read with wave-uniform scalar loads).
+struct Dyn {
+    double thermal_factor;  // 2 alpha kB / (V gamma dt), src/micro/field.jl:261
+    uint64_t noise_step;    // integrator step count keying the noise stream
 };
 
 // What the last pass does with the field of a cell.

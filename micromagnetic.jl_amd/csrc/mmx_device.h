@@ -90,11 +90,11 @@ __device__ __forceinline__ void neighbours(const FieldParams<T> &P, int i, int j
 
 // Field (A/m) of every term selected by `mask` at cell I, summed in insertion
 // order; hd = demag field of this cell when a demag term is present.
-// If E != nullptr, E[t] receives each selected term's energy (J) for this cell.
+// If esum != nullptr, it receives the summed energy (J) of the selected terms.
 template <class T>
-__device__ __forceinline__ void cell_field(const FieldParams<T> &P, uint32_t mask, int64_t I, int i,
-                                           int j, int k, const T *__restrict__ m, T hdx, T hdy,
-                                           T hdz, T &Hx, T &Hy, T &Hz, double *E)
+__device__ __forceinline__ void cell_field(const FieldParams<T> &P, const Dyn &dyn, uint32_t mask, int64_t I,
+                                           int i, int j, int k, const T *__restrict__ m, T hdx, T hdy,
+                                           T hdz, T &Hx, T &Hy, T &Hz, double *esum)
 {
 #pragma clang fp contract(off)
     const int64_t N = P.N;
@@ -310,11 +310,11 @@ __device__ __forceinline__ void cell_field(const FieldParams<T> &P, uint32_t mas
         case K_THERMAL: {  // stochastic_field_kernel!: kernels.jl:469-494
             if (!(Ms > T(0))) break;
             T Tl = (D.uniform ? D.s[1] : D.a[I]) + D.s[0];
-            T scale = (T)sqrt((T)P.thermal_factor * Tl / Ms);
+            T scale = (T)sqrt((T)dyn.thermal_factor * Tl / Ms);
             uint64_t base = 3 * (uint64_t)I;
-            hx = (T)philox_normal(P.seed, P.noise_step, base) * scale;
-            hy = (T)philox_normal(P.seed, P.noise_step, base + 1) * scale;
-            hz = (T)philox_normal(P.seed, P.noise_step, base + 2) * scale;
+            hx = (T)philox_normal(P.seed, dyn.noise_step, base) * scale;
+            hy = (T)philox_normal(P.seed, dyn.noise_step, base + 1) * scale;
+            hz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
             e = (double)(-Ms * V * (mx * hx + my * hy + mz * hz));
         } break;
         default: break;
@@ -322,7 +322,7 @@ __device__ __forceinline__ void cell_field(const FieldParams<T> &P, uint32_t mas
         Hx = Hx + hx;
         Hy = Hy + hy;
         Hz = Hz + hz;
-        if (E) E[tt] = e;
+        if (esum) *esum += e;
     }
 }
 
@@ -363,39 +363,35 @@ __device__ __forceinline__ double pmax(double a, double b)
 }
 
 // The fused last step of one RHS evaluation for cell I.  `red` accumulates the
-// thread's reduction values (ERR: red[0] = running max; ENERGY: red[t] sums).
+// thread's reduction value (ERR: running max; ENERGY: sum of the masked terms).
 template <class T>
-__device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Epi<T> &E, int64_t I, int i,
-                                            int j, int k, const T *__restrict__ m, T hdx, T hdy,
-                                            T hdz, double *red)
+__device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
+                                            int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
+                                            double &red)
 {
 #pragma clang fp contract(off)
     const int64_t N = P.N;
     T Hx, Hy, Hz;
     if (E.mode == EPI_ENERGY) {
-        double e[MAX_TERMS];
-        cell_field(P, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, e);
-        for (int t = 0; t < P.nterms; t++)
-            if ((E.term_mask >> t) & 1u) red[t] += e[t];
+        double es = 0;
+        cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, &es);
+        red += es;
         return;
     }
     if (E.mode == EPI_FIELD) {
         if (E.e_out) {
-            double e[MAX_TERMS];
-            cell_field(P, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, e);
-            double s = 0;
-            for (int t = 0; t < P.nterms; t++)
-                if ((E.term_mask >> t) & 1u) s += e[t];
-            E.e_out[I] = (T)s;
+            double es = 0;
+            cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, &es);
+            E.e_out[I] = (T)es;
         } else {
-            cell_field(P, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
+            cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
         }
         E.out[I] = Hx;
         E.out[N + I] = Hy;
         E.out[2 * N + I] = Hz;
         return;
     }
-    cell_field(P, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
+    cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
     const T mx = m[I], my = m[N + I], mz = m[2 * N + I];
     T kx, ky, kz;
     llg_rhs(P, I, mx, my, mz, Hx, Hy, Hz, kx, ky, kz);
@@ -415,31 +411,37 @@ __device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Epi<T
         double mval = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            int64_t o = c * N + I;
+            const int64_t o = c * N + I;
             double err = 0;
-            for (int q = 0; q < E.np; q++) err = (q == 0) ? E.cp[0] * (double)E.kp[0][o] : err + E.cp[q] * (double)E.kp[q][o];
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (q < E.np) err = (q == 0) ? E.cp[0] * (double)E.kp[0][o] : err + E.cp[q] * (double)E.kp[q][o];
             err = err + E.cself * (double)kk[c];
-            double v = fabs(err / (1.0 + fabs((double)m[o])));
+            const double v = fabs(err / (1.0 + fabs((double)m[o])));
             mval = pmax(mval, v);
         }
-        red[0] = pmax(red[0], mval);
+        red = pmax(red, mval);
         return;
     }
     T y[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        int64_t o = c * N + I;
+        const int64_t o = c * N + I;
         if (E.mode == EPI_RK4_FINAL) {  // src/integrator/rk.jl:46
             T s = E.kp[0][o] + T(2) * E.kp[1][o] + T(2) * E.kp[2][o] + kk[c];
             y[c] = E.y0[o] + (T)E.cself * s;
         } else if (E.mode == EPI_STAGE_SCALED) {  // adaptive_rk_steps.jl:35
             T s = T(0);
-            for (int q = 0; q < E.np; q++) s = (q == 0) ? (T)E.cp[0] * E.kp[0][o] : s + (T)E.cp[q] * E.kp[q][o];
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (q < E.np) s = (q == 0) ? (T)E.cp[0] * E.kp[0][o] : s + (T)E.cp[q] * E.kp[q][o];
             s = s + (T)E.cself * kk[c];
             y[c] = E.y0[o] + (T)E.outer * s;
         } else {  // vector_add2..6: a1 + c2 a2 + ... (util.jl:86-166)
             T s = E.y0[o];
-            for (int q = 0; q < E.np; q++) s = s + (T)E.cp[q] * E.kp[q][o];
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (q < E.np) s = s + (T)E.cp[q] * E.kp[q][o];
             y[c] = s + (T)E.cself * kk[c];
         }
     }
@@ -457,23 +459,28 @@ __device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Epi<T
     E.yout[2 * N + I] = y[2];
 }
 
-// Block-wide reduction of `nval` doubles per thread into partial[blockIdx*stride + v].
+// Block-wide reduction of `nval` doubles per thread into
+// partial[blockIdx*stride + v]; works for any block size up to 1024.
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double *red, int nval, bool is_max, double *partial,
                                                    int stride)
 {
-    __shared__ double sred[BLOCK];
+    __shared__ double sred[1024];
+    const int nt = blockDim.x * blockDim.y;
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    int p2 = 1;
+    while (p2 < nt) p2 <<= 1;
     for (int v = 0; v < nval; v++) {
-        sred[threadIdx.x] = red[v];
+        sred[tid] = red[v];
         __syncthreads();
-        for (int s = BLOCK / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) {
-                double a = sred[threadIdx.x], b = sred[threadIdx.x + s];
-                sred[threadIdx.x] = is_max ? pmax(a, b) : a + b;
+        for (int s = p2 / 2; s > 0; s >>= 1) {
+            if (tid < s && tid + s < nt) {
+                double a = sred[tid], b = sred[tid + s];
+                sred[tid] = is_max ? pmax(a, b) : a + b;
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * stride + v] = sred[0];
+        if (tid == 0) partial[(size_t)blockIdx.x * stride + v] = sred[0];
         __syncthreads();
     }
 }

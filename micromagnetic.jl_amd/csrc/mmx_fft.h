@@ -1,0 +1,155 @@
+// mmx_fft.h -- register-resident Stockham FFT for power-of-two lengths.
+//
+// One sequence of length L is spread over TPS(L) = L / EPT(L) threads; thread
+// j in [0, TPS) holds EPT(L) = min(L, 8) elements v[m] = x[j + m*TPS] on entry
+// and X[j + m*TPS] on exit, so a column can be loaded from and stored to
+// global memory in the same pattern.  Stages are radix 8 (a final radix 4 or 2
+// when log2 L is not a multiple of 3); values move between stages through an
+// LDS exchange buffer of L elements per sequence.  Twiddles come from an FP64-
+// accurate table tw[t] = exp(-2 pi i t / 2^lgLmax) indexed at t << (lgLmax - lgL).
+#pragma once
+#include "mmx_common.h"
+
+namespace mmx {
+
+template <int L>
+struct FftShape {
+    static constexpr int EPT = L >= 8 ? 8 : L;
+    static constexpr int TPS = L / EPT;
+    static constexpr int LG = L <= 1 ? 0 : (L <= 2 ? 1 : (L <= 4 ? 2 : (L <= 8 ? 3 : (L <= 16 ? 4 : (L <= 32 ? 5 : (L <= 64 ? 6 : (L <= 128 ? 7 : (L <= 256 ? 8 : (L <= 512 ? 9 : (L <= 1024 ? 10 : (L <= 2048 ? 11 : 12)))))))))));
+};
+
+template <class T>
+__device__ __forceinline__ cplx<T> mul_mi(cplx<T> a, bool inv)  // a * (-i) forward, a * (+i) inverse
+{
+    return inv ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
+}
+
+template <class T, bool INV>
+__device__ __forceinline__ void dft2(cplx<T> *a)
+{
+    cplx<T> t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+}
+
+template <class T, bool INV>
+__device__ __forceinline__ void dft4(cplx<T> *a)
+{
+    cplx<T> b0 = cadd(a[0], a[2]), b1 = csub(a[0], a[2]);
+    cplx<T> b2 = cadd(a[1], a[3]), b3 = mul_mi(csub(a[1], a[3]), INV);
+    a[0] = cadd(b0, b2);
+    a[2] = csub(b0, b2);
+    a[1] = cadd(b1, b3);
+    a[3] = csub(b1, b3);
+}
+
+// 8-point DFT as two 4-point DFTs (even/odd inputs) and the W8 combination
+template <class T, bool INV>
+__device__ __forceinline__ void dft8(cplx<T> *a)
+{
+    const T r = T(0.70710678118654752440084436210485);
+    cplx<T> b0 = cadd(a[0], a[4]), b1 = csub(a[0], a[4]);
+    cplx<T> b2 = cadd(a[2], a[6]), b3 = mul_mi(csub(a[2], a[6]), INV);
+    cplx<T> b4 = cadd(a[1], a[5]), b5 = csub(a[1], a[5]);
+    cplx<T> b6 = cadd(a[3], a[7]), b7 = mul_mi(csub(a[3], a[7]), INV);
+    cplx<T> c0 = cadd(b0, b2), c2 = csub(b0, b2), c1 = cadd(b1, b3), c3 = csub(b1, b3);
+    cplx<T> c4 = cadd(b4, b6), c6 = csub(b4, b6), c5 = cadd(b5, b7), c7 = csub(b5, b7);
+    // W8 c5, -i c6, W8^3 c7 (forward); conjugate roots for the inverse
+    cplx<T> d5, d7;
+    if (!INV) {
+        d5 = {(c5.x + c5.y) * r, (c5.y - c5.x) * r};
+        d7 = {(c7.y - c7.x) * r, -(c7.x + c7.y) * r};
+    } else {
+        d5 = {(c5.x - c5.y) * r, (c5.y + c5.x) * r};
+        d7 = {-(c7.x + c7.y) * r, (c7.x - c7.y) * r};
+    }
+    cplx<T> d6 = mul_mi(c6, INV);
+    a[0] = cadd(c0, c4);
+    a[4] = csub(c0, c4);
+    a[1] = cadd(c1, d5);
+    a[5] = csub(c1, d5);
+    a[2] = cadd(c2, d6);
+    a[6] = csub(c2, d6);
+    a[3] = cadd(c3, d7);
+    a[7] = csub(c3, d7);
+}
+
+template <class T, int R, bool INV>
+__device__ __forceinline__ void dftR(cplx<T> *a)
+{
+    if constexpr (R == 8) dft8<T, INV>(a);
+    else if constexpr (R == 4) dft4<T, INV>(a);
+    else if constexpr (R == 2) dft2<T, INV>(a);
+}
+
+// One Stockham stage of radix Rs at span Ns.  Butterfly jb = j + q*TPS
+// (q < EPT/Rs) reads v[q + r*EPT/Rs] = x[jb + r*L/Rs], applies W_{Ns Rs}^{r (jb mod Ns)},
+// and writes y[(jb/Ns) Ns Rs + jb mod Ns + r Ns].  The last stage (Ns Rs = L)
+// leaves its outputs in v at the input positions.
+template <class T, int L, int Rs, int Ns, bool INV>
+__device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw,
+                                          int twsh)
+{
+    constexpr int EPT = FftShape<L>::EPT, TPS = FftShape<L>::TPS;
+    constexpr int NQ = EPT / Rs;
+    constexpr bool LAST = (Ns * Rs == L);
+    constexpr int TWSTEP = L / (Ns * Rs);
+    cplx<T> o[EPT];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int jb = j + q * TPS;
+        const int kk = jb & (Ns - 1);
+        cplx<T> a[Rs];
+#pragma unroll
+        for (int r = 0; r < Rs; r++) a[r] = v[q + r * NQ];
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < Rs; r++) {
+                cplx<T> w = tw[(r * kk * TWSTEP) << twsh];
+                if (INV) w.y = -w.y;
+                a[r] = cmul(a[r], w);
+            }
+        }
+        dftR<T, Rs, INV>(a);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < Rs; r++) o[q + r * NQ] = a[r];
+        } else {
+            const int base = (jb / Ns) * Ns * Rs + kk;
+#pragma unroll
+            for (int r = 0; r < Rs; r++) lds[base + r * Ns] = a[r];
+        }
+    }
+    if constexpr (LAST) {
+#pragma unroll
+        for (int m = 0; m < EPT; m++) v[m] = o[m];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < EPT; m++) v[m] = lds[j + m * TPS];
+        __syncthreads();
+    }
+}
+
+template <class T, int L, int Ns, bool INV>
+__device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int twsh)
+{
+    if constexpr (Ns < L) {
+        constexpr int rem = L / Ns;
+        constexpr int Rs = rem >= 8 ? 8 : rem;
+        fft_stage<T, L, Rs, Ns, INV>(v, j, lds, tw, twsh);
+        fft_stages<T, L, Ns * Rs, INV>(v, j, lds, tw, twsh);
+    }
+}
+
+// Transform of one sequence.  Threads of the same block must all call this
+// together (the exchanges use __syncthreads); `lds` is this sequence's buffer
+// of L elements.  `active` threads hold data; inactive ones only join barriers.
+template <class T, int L, bool INV>
+__device__ __forceinline__ void fft_reg(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int lgLmax)
+{
+    if constexpr (L > 1) fft_stages<T, L, 1, INV>(v, j, lds, tw, lgLmax - FftShape<L>::LG);
+}
+
+}  // namespace mmx

@@ -17,7 +17,7 @@ void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s);
 // local field (no demag) + epilogue over all cells; returns number of blocks
 // used (the number of partial-reduction slots written).
 template <class T>
-int launch_local(const FieldParams<T> &P, const Epi<T> &E, const T *m, hipStream_t s);
+int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, hipStream_t s);
 
 // demag passes
 template <class T>
@@ -30,7 +30,7 @@ template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in,
                   int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s);
 template <class T>
-int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> &P, const Epi<T> &E,
+int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
               const T *m, const cplx<T> *tw, hipStream_t s);
 
 // tensor set-up (FP64)
@@ -55,5 +55,7 @@ template <class T>
 int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s);
 
 int max_local_blocks();
+// number of blocks (reduction slot groups) the last demag pass launches
+int p5_blocks(const DemagGeom &G);
 
 }  // namespace mmx

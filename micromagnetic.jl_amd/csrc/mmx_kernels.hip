@@ -12,158 +12,82 @@
 // Each pass stages a batch of 1-D transforms in LDS and runs a radix-4/2
 // Stockham FFT over them with FP64-accurate twiddles from a table.
 #include "mmx_device.h"
+#include "mmx_fft.h"
 #include "mmx_kernels.h"
 
 namespace mmx {
 
-template <class T>
-struct Cap;
-template <>
-struct Cap<double> {
-    static constexpr int E = 4096;  // complex elements per workgroup batch (64 KB)
-    static constexpr int Q = 4;     // radix-4 butterflies per thread per stage
-};
-template <>
-struct Cap<float> {
-    static constexpr int E = 8192;
-    static constexpr int Q = 8;
+// Block shapes.  A pass works on batches of sequences; each sequence of length
+// L uses TPS(L) threads holding EPT(L) elements (mmx_fft.h).
+template <int L>
+struct Shape : FftShape<L> {
+    using FftShape<L>::TPS;
+    // y columns: TY consecutive kx columns per block (>= 8 for whole cache lines)
+    static constexpr int TY = (256 / TPS) >= 8 ? (256 / TPS) : ((1024 / TPS) < 8 ? (1024 / TPS) : 8);
+    static constexpr int BY = TY * TPS;
+    // z columns: TZ columns, all three components in one thread
+    static constexpr int TZ = (256 / TPS) > 64 ? 64 : ((256 / TPS) < 1 ? 1 : (256 / TPS));
+    static constexpr int BZ = TZ * TPS;
+    // x rows: SX sequences per block
+    static constexpr int SX = (256 / TPS) < 1 ? 1 : (256 / TPS);
+    static constexpr int BX = SX * TPS;
+    // last pass: RX rows x 3 components per block
+    static constexpr int RX = (256 / (3 * TPS)) < 1 ? 1 : (256 / (3 * TPS));
+    static constexpr int B5 = 3 * RX * TPS;
 };
 
-// In-place batched Stockham FFT over LDS: nseq sequences of length 2^lgL,
-// sequence s at buf[s*LS ...].  tw[t] = exp(-2 pi i t / 2^lgLmax).
-// Stage with radix R and current span Ns: butterfly j reads x[j + r L/R],
-// multiplies by W_{Ns R}^{r (j mod Ns)}, and writes y[(j/Ns) Ns R + j mod Ns + r Ns].
-template <class T>
-__device__ void lds_fft(cplx<T> *buf, int nseq, int lgL, int LS, bool inv, const cplx<T> *__restrict__ tw,
-                        int lgLmax)
-{
-    constexpr int Q = Cap<T>::Q;
-    const int tid = threadIdx.x;
-    int lgNs = 0;
-    int rem = lgL;
-    while (rem > 0) {
-        const int lgR = (rem >= 2) ? 2 : 1;
-        const int lgM = lgL - lgR;
-        const int M = 1 << lgM;
-        const int NB = nseq << lgM;  // butterflies in this stage
-        const int shift = lgLmax - lgNs - lgR;
-        const int Nsm = (1 << lgNs) - 1;
-        if (lgR == 2) {
-            cplx<T> v[Q][4];
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                int b = tid + q * BLOCK;
-                if (b < NB) {
-                    int s = b >> lgM, jj = b & (M - 1);
-                    const cplx<T> *p = buf + s * LS + jj;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) v[q][r] = p[r << lgM];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                int b = tid + q * BLOCK;
-                if (b < NB) {
-                    int s = b >> lgM, jj = b & (M - 1);
-                    int kk = jj & Nsm;
-                    if (kk) {
-#pragma unroll
-                        for (int r = 1; r < 4; r++) {
-                            cplx<T> w = tw[(r * kk) << shift];
-                            if (inv) w.y = -w.y;
-                            v[q][r] = cmul(v[q][r], w);
-                        }
-                    }
-                    cplx<T> a0 = cadd(v[q][0], v[q][2]);
-                    cplx<T> a1 = csub(v[q][0], v[q][2]);
-                    cplx<T> a2 = cadd(v[q][1], v[q][3]);
-                    cplx<T> d = csub(v[q][1], v[q][3]);
-                    cplx<T> a3 = inv ? cplx<T>{-d.y, d.x} : cplx<T>{d.y, -d.x};
-                    int base = s * LS + ((jj >> lgNs) << (lgNs + 2)) + kk;
-                    buf[base] = cadd(a0, a2);
-                    buf[base + (1 << lgNs)] = cadd(a1, a3);
-                    buf[base + (2 << lgNs)] = csub(a0, a2);
-                    buf[base + (3 << lgNs)] = csub(a1, a3);
-                }
-            }
-        } else {
-            cplx<T> v[2 * Q][2];
-#pragma unroll
-            for (int q = 0; q < 2 * Q; q++) {
-                int b = tid + q * BLOCK;
-                if (b < NB) {
-                    int s = b >> lgM, jj = b & (M - 1);
-                    const cplx<T> *p = buf + s * LS + jj;
-                    v[q][0] = p[0];
-                    v[q][1] = p[M];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 2 * Q; q++) {
-                int b = tid + q * BLOCK;
-                if (b < NB) {
-                    int s = b >> lgM, jj = b & (M - 1);
-                    int kk = jj & Nsm;
-                    if (kk) {
-                        cplx<T> w = tw[kk << shift];
-                        if (inv) w.y = -w.y;
-                        v[q][1] = cmul(v[q][1], w);
-                    }
-                    int base = s * LS + ((jj >> lgNs) << (lgNs + 1)) + kk;
-                    buf[base] = cadd(v[q][0], v[q][1]);
-                    buf[base + (1 << lgNs)] = csub(v[q][0], v[q][1]);
-                }
-            }
-        }
-        __syncthreads();
-        lgNs += lgR;
-        rem -= lgR;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// P1: rows of n_in reals (zero-padded to Px = 2H) -> Kx = H+1 complex values.
-// Row r = z*Ry + y of component c starts at in[c*cstride + r*n_in]; inputs are
-// multiplied by scale[r*n_in + x] when scale != null (mu0 Ms, demag.jl:419-427).
+// P1: rows of n_in reals (zero-padded to Px = 2H) -> Kx = H+1 complex values
+// per row, via the half-length complex transform of z[j] = x[2j] + i x[2j+1].
+// Sequence s = c*nrows + row; row r of component c starts at in[c*cstride +
+// r*n_in]; inputs are multiplied by scale[r*n_in + x] when scale != null
+// (mu0 Ms, demag.jl:419-427).  Output A[(c*nrows + row)*KS + k].
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_p1(DemagGeom G, const T *__restrict__ in, int64_t cstride,
-                                              const T *__restrict__ scale, int n_in, int Ry, int Rz,
-                                              int ncomp, int rpb, cplx<T> *__restrict__ A,
-                                              const cplx<T> *__restrict__ tw)
+template <class T, int H>
+__global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__restrict__ in, int64_t cstride,
+                                                      const T *__restrict__ scale, int n_in, int nrows, int ncomp,
+                                                      cplx<T> *__restrict__ A, const cplx<T> *__restrict__ tw)
 {
-    __shared__ cplx<T> lds[Cap<T>::E + 64];
-    const int H = G.H, HS = H + 1;
-    const int nrows_tot = Ry * Rz;
-    const int row0 = blockIdx.x * rpb;
-    const int nrows = min(rpb, nrows_tot - row0);
-    const int nseq = ncomp * nrows;
-    for (int idx = threadIdx.x; idx < nseq * H; idx += BLOCK) {
-        int s = idx / H, j = idx - s * H;
-        int c = s / nrows, r = s - c * nrows;
-        int64_t rb = (int64_t)(row0 + r) * n_in;
-        int x0 = 2 * j, x1 = 2 * j + 1;
-        T v0 = 0, v1 = 0;
-        if (x0 < n_in) v0 = scale ? in[c * cstride + rb + x0] * scale[rb + x0] : in[c * cstride + rb + x0];
-        if (x1 < n_in) v1 = scale ? in[c * cstride + rb + x1] * scale[rb + x1] : in[c * cstride + rb + x1];
-        lds[s * HS + j] = {v0, v1};
+    using S = Shape<H>;
+    constexpr int EPT = S::EPT, TPS = S::TPS, SX = S::SX;
+    constexpr int HS = H + 1;
+    __shared__ cplx<T> lds[SX * HS];
+    const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
+    const int64_t seq = (int64_t)blockIdx.x * SX + ls;
+    const bool active = seq < (int64_t)ncomp * nrows;
+    const int c = active ? (int)(seq / nrows) : 0;
+    const int64_t row = active ? seq % nrows : 0;
+    const int64_t rb = row * n_in;
+    const T *src = in + c * cstride + rb;
+    cplx<T> v[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int k = j + m * TPS;
+        const int x0 = 2 * k, x1 = 2 * k + 1;
+        T a = 0, b = 0;
+        if (active && x0 < n_in) a = scale ? src[x0] * scale[rb + x0] : src[x0];
+        if (active && x1 < n_in) b = scale ? src[x1] * scale[rb + x1] : src[x1];
+        v[m] = {a, b};
     }
+    cplx<T> *buf = lds + ls * HS;
+    fft_reg<T, H, false>(v, j, buf, tw, G.lgLmax);
+#pragma unroll
+    for (int m = 0; m < EPT; m++) buf[j + m * TPS] = v[m];
     __syncthreads();
-    lds_fft<T>(lds, nseq, G.lgH, HS, false, tw, G.lgLmax);
-    const int sh = G.lgLmax - (G.lgH + 1);  // W_Px^k = tw[k << sh]
-    for (int idx = threadIdx.x; idx < nseq * (H + 1); idx += BLOCK) {
-        int s = idx / (H + 1), k = idx - s * (H + 1);
-        int c = s / nrows, r = s - c * nrows;
-        cplx<T> Zk = lds[s * HS + (k & (H - 1))];
-        cplx<T> Zc = lds[s * HS + ((H - k) & (H - 1))];
+    if (!active) return;
+    const int sh = G.lgLmax - (FftShape<H>::LG + 1);  // W_Px^k = tw[k << sh]
+    cplx<T> *dst = A + ((int64_t)c * nrows + row) * G.KS;
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int k = j + m * TPS;
+        cplx<T> Zk = v[m];
+        cplx<T> Zc = buf[(H - k) & (H - 1)];
         Zc.y = -Zc.y;
         cplx<T> Fe = {(Zk.x + Zc.x) * T(0.5), (Zk.y + Zc.y) * T(0.5)};
         cplx<T> d = csub(Zk, Zc);
         cplx<T> Fo = {d.y * T(0.5), -d.x * T(0.5)};
-        cplx<T> X = cadd(Fe, cmul(tw[k << sh], Fo));
-        A[((int64_t)c * nrows_tot + row0 + r) * G.KS + k] = X;
+        dst[k] = cadd(Fe, cmul(tw[k << sh], Fo));
+        if (k == 0) dst[H] = csub(Fe, Fo);  // W_Px^H = -1
     }
 }
 
@@ -172,178 +96,200 @@ __global__ void __launch_bounds__(BLOCK) k_p1(DemagGeom G, const T *__restrict__
 // Input column (plane p, comp c): in[((c*nplanes + p)*Rin + y)*KS + kx], y < n_in.
 // Output: out[((c*nplanes + p)*Rout + y)*KS + kx] for y < n_out.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_coly(DemagGeom G, const cplx<T> *__restrict__ in,
-                                                cplx<T> *__restrict__ out, int n_in, int Rin, int n_out,
-                                                int Rout, int nplanes, int lgL, int TY, int inverse,
-                                                const cplx<T> *__restrict__ tw)
+template <class T, int L, bool INV>
+__global__ void __launch_bounds__(Shape<L>::BY) k_coly(DemagGeom G, const cplx<T> *__restrict__ in,
+                                                        cplx<T> *__restrict__ out, int n_in, int Rin, int n_out,
+                                                        int Rout, int nplanes, const cplx<T> *__restrict__ tw)
 {
-    __shared__ cplx<T> lds[Cap<T>::E + 64];
-    const int L = 1 << lgL, LS = L + 1;
-    const int kx0 = blockIdx.x * TY;
+    using S = Shape<L>;
+    constexpr int EPT = S::EPT, TPS = S::TPS, TY = S::TY;
+    constexpr int LS = L + 1;
+    __shared__ cplx<T> lds[TY * LS];
+    const int t = threadIdx.x % TY, j = threadIdx.x / TY;
+    const int kx = blockIdx.x * TY + t;
     const int p = blockIdx.y, c = blockIdx.z;
-    const int KS = G.KS, Kx = G.Kx;
-    const cplx<T> *src = in + ((int64_t)c * nplanes + p) * Rin * KS;
-    for (int idx = threadIdx.x; idx < TY * L; idx += BLOCK) {
-        int t = idx % TY, y = idx / TY;
-        int kx = kx0 + t;
-        cplx<T> v = {0, 0};
-        if (y < n_in && kx < Kx) v = src[(int64_t)y * KS + kx];
-        lds[t * LS + y] = v;
+    const int KS = G.KS;
+    const bool ok = kx < G.Kx;
+    const cplx<T> *src = in + ((int64_t)c * nplanes + p) * Rin * KS + kx;
+    cplx<T> v[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int y = j + m * TPS;
+        v[m] = (ok && y < n_in) ? src[(int64_t)y * KS] : cplx<T>{0, 0};
     }
-    __syncthreads();
-    lds_fft<T>(lds, TY, lgL, LS, inverse != 0, tw, G.lgLmax);
-    cplx<T> *dst = out + ((int64_t)c * nplanes + p) * Rout * KS;
-    for (int idx = threadIdx.x; idx < TY * n_out; idx += BLOCK) {
-        int t = idx % TY, y = idx / TY;
-        int kx = kx0 + t;
-        if (kx < Kx) dst[(int64_t)y * KS + kx] = lds[t * LS + y];
+    fft_reg<T, L, INV>(v, j, lds + t * LS, tw, G.lgLmax);
+    if (!ok) return;
+    cplx<T> *dst = out + ((int64_t)c * nplanes + p) * Rout * KS + kx;
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int y = j + m * TPS;
+        if (y < n_out) dst[(int64_t)y * KS] = v[m];
     }
 }
 
 // ---------------------------------------------------------------------------
-// P3: z transform, per-k 3x3 symmetric tensor multiply, inverse (conv mode),
-// or forward only (tensor set-up).  B element (c, z, ky, kx) at
-// ((c*nzb + z)*Py + ky)*KS + kx.  Kt holds the real k-space tensor folded to
-// ky <= Py/2, kz <= Pz/2: (comp, kzf, kyf, kx) at ((comp*Pz2 + kzf)*Py2 + kyf)*KS + kx,
-// with the sign of the odd components restored on unfolding (xy odd in y,
-// xz odd in z, yz odd in y and z; src/micro/demag.jl:69-80).  The reference
-// keeps real(rfft(tensor)), whose odd components also carry the cosine
-// transform of the tensor's on-axis line (xy: x=y=0, a function of kz; xz:
-// x=z=0, of ky; yz: y=z=0, of kx) -- rounding residue of the Newell sums that
-// is not odd; Lline holds these three even transforms: [Pz2 | Py2 | Kx].
+// P3: z transform, per-k 3x3 symmetric tensor multiply, inverse, keep z < nz
+// (conv mode); or forward only over one component (tensor set-up).  B element
+// (c, z, ky, kx) at ((c*nzb + z)*Py + ky)*KS + kx.  Kt holds the real k-space
+// tensor folded to ky <= Py/2, kz <= Pz/2: (comp, kzf, kyf, kx) at
+// ((comp*Pz2 + kzf)*Py2 + kyf)*KS + kx, with the sign of the odd components
+// restored on unfolding (xy odd in y, xz odd in z, yz odd in y and z;
+// src/micro/demag.jl:69-80).  The reference keeps real(rfft(tensor)), whose
+// odd components also carry the cosine transform of the tensor's on-axis line
+// (xy: x=y=0, a function of kz; xz: x=z=0, of ky; yz: y=z=0, of kx) -- the
+// rounding residue of the Newell sums that is not odd; Lline holds these three
+// even transforms: [Pz2 | Py2 | Kx].
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_convz(DemagGeom G, cplx<T> *__restrict__ B,
-                                                 const T *__restrict__ Kt, const T *__restrict__ Lline,
-                                                 int nzb, int n_in, int ncomp, int TZ, int fwd_only,
-                                                 const cplx<T> *__restrict__ tw)
+template <class T, int L, bool FWD_ONLY>
+__global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__restrict__ B,
+                                                         const T *__restrict__ Kt, const T *__restrict__ Lline,
+                                                         int nzb, int n_in, const cplx<T> *__restrict__ tw)
 {
-    __shared__ cplx<T> lds[Cap<T>::E + 64];
-    const int Pz = G.Pz, LS = Pz + 1;
-    const int KS = G.KS, Kx = G.Kx, Py = G.Py;
-    const int kx0 = blockIdx.x * TZ;
+    using S = Shape<L>;
+    constexpr int EPT = S::EPT, TPS = S::TPS, TZ = S::TZ;
+    constexpr int LS = L + 1;
+    constexpr int NC = FWD_ONLY ? 1 : 3;
+    __shared__ cplx<T> lds[TZ * LS];
+    const int t = threadIdx.x % TZ, j = threadIdx.x / TZ;
+    const int kx = blockIdx.x * TZ + t;
     const int ky = blockIdx.y;
-    const int nseq = ncomp * TZ;
-    for (int idx = threadIdx.x; idx < nseq * Pz; idx += BLOCK) {
-        int t = idx % TZ, rest = idx / TZ;
-        int z = rest % Pz, c = rest / Pz;
-        int kx = kx0 + t;
-        cplx<T> v = {0, 0};
-        if (z < n_in && kx < Kx) v = B[(((int64_t)c * nzb + z) * Py + ky) * KS + kx];
-        lds[(c * TZ + t) * LS + z] = v;
-    }
-    __syncthreads();
-    lds_fft<T>(lds, nseq, G.lgPz, LS, false, tw, G.lgLmax);
-    if (fwd_only) {
-        for (int idx = threadIdx.x; idx < nseq * Pz; idx += BLOCK) {
-            int t = idx % TZ, rest = idx / TZ;
-            int z = rest % Pz, c = rest / Pz;
-            int kx = kx0 + t;
-            if (kx < Kx) B[(((int64_t)c * nzb + z) * Py + ky) * KS + kx] = lds[(c * TZ + t) * LS + z];
+    const int KS = G.KS, Py = G.Py;
+    const bool ok = kx < G.Kx;
+    const int64_t cs_b = (int64_t)nzb * Py * KS;  // component stride in B
+    const int64_t zs_b = (int64_t)Py * KS;        // z stride in B
+    cplx<T> *col = B + (int64_t)ky * KS + kx;
+    cplx<T> v[NC][EPT];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int z = j + m * TPS;
+            v[c][m] = (ok && z < n_in) ? col[c * cs_b + z * zs_b] : cplx<T>{0, 0};
         }
+    cplx<T> *buf = lds + t * LS;
+#pragma unroll
+    for (int c = 0; c < NC; c++) fft_reg<T, L, false>(v[c], j, buf, tw, G.lgLmax);
+    if constexpr (FWD_ONLY) {
+        if (!ok) return;
+#pragma unroll
+        for (int m = 0; m < EPT; m++) col[(int64_t)(j + m * TPS) * zs_b] = v[0][m];
         return;
-    }
-    const int kyf = (ky <= Py / 2) ? ky : Py - ky;
-    const T sy = (ky <= Py / 2) ? T(1) : T(-1);
-    const int64_t cs = (int64_t)G.Pz2 * G.Py2 * KS;  // component stride of Kt
-    for (int idx = threadIdx.x; idx < TZ * Pz; idx += BLOCK) {
-        int t = idx % TZ, kz = idx / TZ;
-        int kx = kx0 + t;
-        if (kx >= Kx) continue;
-        int kzf = (kz <= Pz / 2) ? kz : Pz - kz;
-        T sz = (kz <= Pz / 2) ? T(1) : T(-1);
-        const T *kp = Kt + ((int64_t)kzf * G.Py2 + kyf) * KS + kx;
-        T xx = kp[0], yy = kp[cs], zz = kp[2 * cs];
-        T xy = sy * kp[3 * cs] + Lline[kzf];
-        T xz = sz * kp[4 * cs] + Lline[G.Pz2 + kyf];
-        T yz = sy * sz * kp[5 * cs] + Lline[G.Pz2 + G.Py2 + kx];
-        cplx<T> Mx = lds[(0 * TZ + t) * LS + kz];
-        cplx<T> My = lds[(1 * TZ + t) * LS + kz];
-        cplx<T> Mz = lds[(2 * TZ + t) * LS + kz];
-        lds[(0 * TZ + t) * LS + kz] = {xx * Mx.x + xy * My.x + xz * Mz.x, xx * Mx.y + xy * My.y + xz * Mz.y};
-        lds[(1 * TZ + t) * LS + kz] = {xy * Mx.x + yy * My.x + yz * Mz.x, xy * Mx.y + yy * My.y + yz * Mz.y};
-        lds[(2 * TZ + t) * LS + kz] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
-    }
-    __syncthreads();
-    lds_fft<T>(lds, nseq, G.lgPz, LS, true, tw, G.lgLmax);
-    const int nz = G.nz;
-    for (int idx = threadIdx.x; idx < nseq * nz; idx += BLOCK) {
-        int t = idx % TZ, rest = idx / TZ;
-        int z = rest % nz, c = rest / nz;
-        int kx = kx0 + t;
-        if (kx < Kx) B[(((int64_t)c * nzb + z) * Py + ky) * KS + kx] = lds[(c * TZ + t) * LS + z];
+    } else {
+        const int kyf = (ky <= Py / 2) ? ky : Py - ky;
+        const T sy = (ky <= Py / 2) ? T(1) : T(-1);
+        const int64_t cs = (int64_t)G.Pz2 * G.Py2 * KS;  // component stride of Kt
+        const int kxs = ok ? kx : 0;
+        const T lyz = Lline[G.Pz2 + G.Py2 + kxs];
+        const T lxz = Lline[G.Pz2 + kyf];
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int kz = j + m * TPS;
+            const int kzf = (kz <= L / 2) ? kz : L - kz;
+            const T sz = (kz <= L / 2) ? T(1) : T(-1);
+            const T *kp = Kt + ((int64_t)kzf * G.Py2 + kyf) * KS + kxs;
+            const T xx = kp[0], yy = kp[cs], zz = kp[2 * cs];
+            const T xy = sy * kp[3 * cs] + Lline[kzf];
+            const T xz = sz * kp[4 * cs] + lxz;
+            const T yz = sy * sz * kp[5 * cs] + lyz;
+            const cplx<T> Mx = v[0][m], My = v[1][m], Mz = v[2][m];
+            v[0][m] = {xx * Mx.x + xy * My.x + xz * Mz.x, xx * Mx.y + xy * My.y + xz * Mz.y};
+            v[1][m] = {xy * Mx.x + yy * My.x + yz * Mz.x, xy * Mx.y + yy * My.y + yz * Mz.y};
+            v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) fft_reg<T, L, true>(v[c], j, buf, tw, G.lgLmax);
+        if (!ok) return;
+        const int nz = G.nz;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int m = 0; m < EPT; m++) {
+                const int z = j + m * TPS;
+                if (z < nz) col[c * cs_b + z * zs_b] = v[c][m];
+            }
     }
 }
 
 // ---------------------------------------------------------------------------
-// P5: half-spectrum -> real along x for rpb rows, then the fused epilogue for
-// every cell of those rows.  A element (c, row, kx) at (c*ny*nz + row)*KS + kx.
+// P5: half-spectrum -> real along x for RX rows x 3 components, then the
+// fused epilogue for every cell of those rows.  A element (c, row, kx) at
+// (c*ny*nz + row)*KS + kx.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_p5(DemagGeom G, const cplx<T> *__restrict__ A, FieldParams<T> P,
-                                              Epi<T> E, const T *__restrict__ m, int rpb,
-                                              const cplx<T> *__restrict__ tw)
+template <class T, int H>
+__global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
+                                                      const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
+                                                      const T *__restrict__ m, const cplx<T> *__restrict__ tw)
 {
-    __shared__ cplx<T> lds[Cap<T>::E + 64];
-    const int H = G.H, HS = H + 1, nx = G.nx, ny = G.ny;
+    using S = Shape<H>;
+    constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
+    constexpr int HS = H + 1;
+    __shared__ cplx<T> lds[3 * RX * HS];
+    const int nx = G.nx, ny = G.ny;
     const int nrows_tot = G.ny * G.nz;
-    const int row0 = blockIdx.x * rpb;
-    const int nrows = min(rpb, nrows_tot - row0);
-    const int nseq = 3 * nrows;
-    const int sh = G.lgLmax - (G.lgH + 1);
-    for (int idx = threadIdx.x; idx < nseq * H; idx += BLOCK) {
-        int s = idx / H, k = idx - s * H;
-        int c = s / nrows, r = s - c * nrows;
-        const cplx<T> *row = A + ((int64_t)c * nrows_tot + row0 + r) * G.KS;
-        cplx<T> Xk = row[k];
-        cplx<T> Xc = row[H - k];
-        Xc.y = -Xc.y;
-        cplx<T> Fe = cadd(Xk, Xc);
-        cplx<T> w = tw[k << sh];
-        w.y = -w.y;  // W_Px^{-k}
-        cplx<T> Fo = cmul(csub(Xk, Xc), w);
-        lds[s * HS + k] = {Fe.x - Fo.y, Fe.y + Fo.x};  // Fe + i Fo
+    const int row0 = blockIdx.x * RX;
+    const int nrows = min(RX, nrows_tot - row0);
+    const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
+    const int c = ls / RX, r = ls % RX;
+    const bool active = r < nrows;
+    const cplx<T> *row = A + ((int64_t)c * nrows_tot + row0 + (active ? r : 0)) * G.KS;
+    const int sh = G.lgLmax - (FftShape<H>::LG + 1);
+    cplx<T> v[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int k = j + q * TPS;
+        cplx<T> Z = {0, 0};
+        if (active) {
+            cplx<T> Xk = row[k];
+            cplx<T> Xc = row[H - k];
+            Xc.y = -Xc.y;
+            cplx<T> Fe = cadd(Xk, Xc);
+            cplx<T> w = tw[k << sh];
+            w.y = -w.y;  // W_Px^{-k}
+            cplx<T> Fo = cmul(csub(Xk, Xc), w);
+            Z = {Fe.x - Fo.y, Fe.y + Fo.x};  // Fe + i Fo
+        }
+        v[q] = Z;
     }
+    cplx<T> *buf = lds + (c * RX + r) * HS;
+    fft_reg<T, H, true>(v, j, buf, tw, G.lgLmax);
+#pragma unroll
+    for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
     __syncthreads();
-    lds_fft<T>(lds, nseq, G.lgH, HS, true, tw, G.lgLmax);
-    double red[MAX_TERMS];
-    for (int v = 0; v < MAX_TERMS; v++) red[v] = 0;
-    for (int idx = threadIdx.x; idx < nrows * nx; idx += BLOCK) {
-        int r = idx / nx, x = idx - r * nx;
-        int row = row0 + r;
-        int y = row % ny, z = row / ny;
-        cplx<T> h0 = lds[(0 * nrows + r) * HS + (x >> 1)];
-        cplx<T> h1 = lds[(1 * nrows + r) * HS + (x >> 1)];
-        cplx<T> h2 = lds[(2 * nrows + r) * HS + (x >> 1)];
-        T hx = (x & 1) ? h0.y : h0.x;
-        T hy = (x & 1) ? h1.y : h1.x;
-        T hz = (x & 1) ? h2.y : h2.x;
-        int64_t I = (int64_t)row * nx + x;
-        finish_cell(P, E, I, x, y, z, m, hx, hy, hz, red);
+    const FieldParams<T> &P = *Pp;
+    double red = 0;
+    for (int idx = threadIdx.x; idx < nrows * nx; idx += blockDim.x) {
+        const int rr = idx / nx, x = idx - rr * nx;
+        const int rw = row0 + rr;
+        const int y = rw % ny, z = rw / ny;
+        const cplx<T> h0 = lds[(0 * RX + rr) * HS + (x >> 1)];
+        const cplx<T> h1 = lds[(1 * RX + rr) * HS + (x >> 1)];
+        const cplx<T> h2 = lds[(2 * RX + rr) * HS + (x >> 1)];
+        const T hx = (x & 1) ? h0.y : h0.x;
+        const T hy = (x & 1) ? h1.y : h1.x;
+        const T hz = (x & 1) ? h2.y : h2.x;
+        finish_cell(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
     }
-    if (E.mode == EPI_ERR) block_reduce_store<1>(red, 1, true, E.partial, 1);
-    else if (E.mode == EPI_ENERGY) block_reduce_store<MAX_TERMS>(red, P.nterms, false, E.partial, MAX_TERMS);
+    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
 
 // ---------------------------------------------------------------------------
 // Local-field kernel (meshes without demag): one thread per cell.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(BLOCK) k_local(FieldParams<T> P, Epi<T> E, const T *__restrict__ m)
+__global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
+                                                 const T *__restrict__ m)
 {
-    double red[MAX_TERMS];
-    for (int v = 0; v < MAX_TERMS; v++) red[v] = 0;
+    const FieldParams<T> &P = *Pp;
+    double red = 0;
     const int nx = P.nx, ny = P.ny;
     for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < P.N; I += (int64_t)gridDim.x * BLOCK) {
         int i = (int)(I % nx);
         int64_t q = I / nx;
         int j = (int)(q % ny);
         int k = (int)(q / ny);
-        finish_cell(P, E, I, i, j, k, m, T(0), T(0), T(0), red);
+        finish_cell(P, dyn, E, I, i, j, k, m, T(0), T(0), T(0), red);
     }
-    if (E.mode == EPI_ERR) block_reduce_store<1>(red, 1, true, E.partial, 1);
-    else if (E.mode == EPI_ENERGY) block_reduce_store<MAX_TERMS>(red, P.nterms, false, E.partial, MAX_TERMS);
+    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,58 +465,103 @@ void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s)
 }
 
 template <class T>
-int launch_local(const FieldParams<T> &P, const Epi<T> &E, const T *m, hipStream_t s)
+int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, hipStream_t s)
 {
-    int g = grid_for(P.N, max_local_blocks());
-    hipLaunchKernelGGL(k_local<T>, dim3(g), dim3(BLOCK), 0, s, P, E, m);
+    int g = grid_for(N, max_local_blocks());
+    hipLaunchKernelGGL(k_local<T>, dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     return g;
 }
+
+#define MMX_LG_SWITCH(lg, CALL)                                                 \
+    switch (lg) {                                                              \
+    case 0: CALL(1); break;                                                    \
+    case 1: CALL(2); break;                                                    \
+    case 2: CALL(4); break;                                                    \
+    case 3: CALL(8); break;                                                    \
+    case 4: CALL(16); break;                                                   \
+    case 5: CALL(32); break;                                                   \
+    case 6: CALL(64); break;                                                   \
+    case 7: CALL(128); break;                                                  \
+    case 8: CALL(256); break;                                                  \
+    case 9: CALL(512); break;                                                  \
+    case 10: CALL(1024); break;                                                \
+    case 11: CALL(2048); break;                                                \
+    default: break;                                                            \
+    }
 
 template <class T>
 void launch_p1(const DemagGeom &G, const T *in, int64_t cstride, const T *scale, int n_in, int Ry, int Rz,
                int ncomp, cplx<T> *A, const cplx<T> *tw, hipStream_t s)
 {
-    int rpb = Cap<T>::E / (ncomp * (G.H + 1));
-    if (rpb < 1) rpb = 1;
-    int rows = Ry * Rz;
-    int g = (rows + rpb - 1) / rpb;
-    hipLaunchKernelGGL(k_p1<T>, dim3(g), dim3(BLOCK), 0, s, G, in, cstride, scale, n_in, Ry, Rz, ncomp, rpb, A, tw);
+    const int nrows = Ry * Rz;
+#define P1_CALL(H)                                                                                   \
+    {                                                                                                \
+        int g = (int)(((int64_t)ncomp * nrows + Shape<H>::SX - 1) / Shape<H>::SX);                   \
+        hipLaunchKernelGGL((k_p1<T, H>), dim3(g), dim3(Shape<H>::BX), 0, s, G, in, cstride, scale, n_in, nrows, \
+                           ncomp, A, tw);                                                            \
+    }
+    MMX_LG_SWITCH(G.lgH, P1_CALL)
+#undef P1_CALL
 }
 
 template <class T>
 void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, int Rin, int n_out, int Rout,
                  int nplanes, int ncomp, int inverse, const cplx<T> *tw, hipStream_t s)
 {
-    int L = G.Py;
-    int TY = Cap<T>::E / (L + 1);
-    if (TY > 16) TY = 16;
-    if (TY < 1) TY = 1;
-    dim3 grid((G.Kx + TY - 1) / TY, nplanes, ncomp);
-    hipLaunchKernelGGL(k_coly<T>, grid, dim3(BLOCK), 0, s, G, in, out, n_in, Rin, n_out, Rout, nplanes, G.lgPy,
-                       TY, inverse, tw);
+#define CY_CALL(L)                                                                                   \
+    {                                                                                                \
+        dim3 grid((G.Kx + Shape<L>::TY - 1) / Shape<L>::TY, nplanes, ncomp);                         \
+        if (inverse)                                                                                 \
+            hipLaunchKernelGGL((k_coly<T, L, true>), grid, dim3(Shape<L>::BY), 0, s, G, in, out, n_in, Rin, n_out, \
+                               Rout, nplanes, tw);                                                   \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_coly<T, L, false>), grid, dim3(Shape<L>::BY), 0, s, G, in, out, n_in, Rin, n_out, \
+                               Rout, nplanes, tw);                                                   \
+    }
+    MMX_LG_SWITCH(G.lgPy, CY_CALL)
+#undef CY_CALL
 }
 
 template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in, int ncomp,
                   int fwd_only, const cplx<T> *tw, hipStream_t s)
 {
-    int TZ = Cap<T>::E / (ncomp * (G.Pz + 1));
-    if (TZ > 16) TZ = 16;
-    if (TZ < 1) TZ = 1;
-    dim3 grid((G.Kx + TZ - 1) / TZ, G.Py);
-    hipLaunchKernelGGL(k_convz<T>, grid, dim3(BLOCK), 0, s, G, B, Kt, Lline, nzb, n_in, ncomp, TZ, fwd_only, tw);
+    (void)ncomp;
+#define CZ_CALL(L)                                                                                   \
+    {                                                                                                \
+        dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py);                                   \
+        if (fwd_only)                                                                                \
+            hipLaunchKernelGGL((k_convz<T, L, true>), grid, dim3(Shape<L>::BZ), 0, s, G, B, Kt, Lline, nzb, n_in, tw); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_convz<T, L, false>), grid, dim3(Shape<L>::BZ), 0, s, G, B, Kt, Lline, nzb, n_in, tw); \
+    }
+    MMX_LG_SWITCH(G.lgPz, CZ_CALL)
+#undef CZ_CALL
 }
 
 template <class T>
-int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> &P, const Epi<T> &E, const T *m,
-              const cplx<T> *tw, hipStream_t s)
+int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
+              const T *m, const cplx<T> *tw, hipStream_t s)
 {
-    int rpb = Cap<T>::E / (3 * (G.H + 1));
-    if (rpb < 1) rpb = 1;
-    int rows = G.ny * G.nz;
-    int g = (rows + rpb - 1) / rpb;
-    hipLaunchKernelGGL(k_p5<T>, dim3(g), dim3(BLOCK), 0, s, G, A, P, E, m, rpb, tw);
+    const int rows = G.ny * G.nz;
+    int g = 0;
+#define P5_CALL(H)                                                                                   \
+    {                                                                                                \
+        g = (rows + Shape<H>::RX - 1) / Shape<H>::RX;                                                \
+        hipLaunchKernelGGL((k_p5<T, H>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw);  \
+    }
+    MMX_LG_SWITCH(G.lgH, P5_CALL)
+#undef P5_CALL
     return g;
+}
+
+int p5_blocks(const DemagGeom &G)
+{
+    int rx = 1;
+#define RX_CALL(H) rx = Shape<H>::RX;
+    MMX_LG_SWITCH(G.lgH, RX_CALL)
+#undef RX_CALL
+    return (G.ny * G.nz + rx - 1) / rx;
 }
 
 void launch_tensor_pad(const DemagGeom &G, const double *rt, int comp, double *pad, hipStream_t s)
@@ -620,15 +611,15 @@ int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s)
     template void launch_soa_to_aos<T>(const T *, double *, int64_t, hipStream_t);                           \
     template void launch_cast<T>(const double *, T *, int64_t, hipStream_t);                                 \
     template void launch_to_double<T>(const T *, double *, int64_t, hipStream_t);                            \
-    template int launch_local<T>(const FieldParams<T> &, const Epi<T> &, const T *, hipStream_t);            \
+    template int launch_local<T>(int64_t, const FieldParams<T> *, const Dyn &, const Epi<T> &, const T *, hipStream_t); \
     template void launch_p1<T>(const DemagGeom &, const T *, int64_t, const T *, int, int, int, int, cplx<T> *, \
                                const cplx<T> *, hipStream_t);                                                \
     template void launch_coly<T>(const DemagGeom &, const cplx<T> *, cplx<T> *, int, int, int, int, int, int,  \
                                  int, const cplx<T> *, hipStream_t);                                         \
     template void launch_convz<T>(const DemagGeom &, cplx<T> *, const T *, const T *, int, int, int, int,      \
                                   const cplx<T> *, hipStream_t);                                                              \
-    template int launch_p5<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> &, const Epi<T> &,     \
-                              const T *, const cplx<T> *, hipStream_t);                                      \
+    template int launch_p5<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,        \
+                              const Epi<T> &, const T *, const cplx<T> *, hipStream_t);                      \
     template void launch_tensor_extract<T>(const DemagGeom &, const cplx<double> *, int, double, T *,         \
                                            hipStream_t);                                                     \
     template int launch_maxdiff<T>(const T *, const T *, int64_t, double *, hipStream_t);          \
