@@ -266,6 +266,29 @@ struct Engine : CtxBase {
         P.seed = 0;
         for (const auto &h : C->terms)
             if (h.kind == K_THERMAL) P.seed = h.seed;
+        // slot form: every kind of term at most once
+        for (int q = 0; q < 8; q++) P.slot[q] = -1;
+        P.fast = 1;
+        for (int i = 0; i < P.nterms; i++) {
+            int sk;
+            switch (C->terms[i].kind) {
+            case K_EXCH: case K_EXCH_S: sk = SL_EXCH; break;
+            case K_DMI_BULK: case K_DMI_BULK_S: sk = SL_DMIB; break;
+            case K_DMI_INT: sk = SL_DMII; break;
+            case K_ANIS: case K_ANIS_S: sk = SL_ANIS; break;
+            case K_CUBIC: sk = SL_CUBIC; break;
+            case K_ZEEMAN: case K_ZEEMAN_U: sk = SL_ZEE; break;
+            case K_DEMAG: sk = SL_DEMAG; break;
+            default: sk = SL_THERM; break;
+            }
+            if (P.slot[sk] >= 0) P.fast = 0;
+            P.slot[sk] = i;
+            P.ord[i] = sk;
+        }
+        // -gamma/(1+alpha^2) rounded as the kernel computes it (llg.jl:18)
+        volatile T al = (T)C->alpha;
+        T den = T(1) + al * al;
+        P.llg_a = -(T)C->gamma / den;
         return P;
     }
 
@@ -331,7 +354,7 @@ struct Engine : CtxBase {
         int nb;
         if (!demag) {
             mark(MMX_PASS_LOCAL, true);
-            nb = launch_local<T>(N, P, d, E, m, C->stream);
+            nb = launch_local<T>(N, P, d, E, m, Plast.fast, C->stream);
             mark(MMX_PASS_LOCAL, false);
         } else {
             const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -348,7 +371,7 @@ struct Engine : CtxBase {
             launch_coly<T>(G, Bbuf.p, Abuf.p, G.Py, G.Py, ny, ny, nz, 3, 1, tw.p, C->stream);
             mark(MMX_PASS_P4, false);
             mark(MMX_PASS_P5, true);
-            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, tw.p, C->stream);
+            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, Plast.fast, tw.p, C->stream);
             mark(MMX_PASS_P5, false);
         }
         HIPCHECK(hipGetLastError());
@@ -990,6 +1013,12 @@ int mmx_add_dmi(mmx_ctx *c, int type, const double *Dx, const double *Dy, const 
             if (!Dx) {
                 h.uniform = 1;
                 h.s[3] = (double)(TT)Dx0;
+                // harmonic mean of two equal D, rounded as the kernel would (kernels.jl:452)
+                volatile TT D = (TT)Dx0;
+                TT num = TT(2) * D;
+                num = num * D;
+                TT den = D + D;
+                h.s[4] = (double)(TT)(num / den);
                 return add_term(c, e, h, nullptr, nullptr, nullptr);
             }
             const TT *a = e.new_array(Dx, c->N, false);

@@ -53,7 +53,16 @@ struct FieldParams {
     T alpha, gamma;
     int precession;
     uint64_t seed;  // thermal noise stream
+    // Slot form used by the specialised kernels when every kind of term
+    // appears at most once: slot[k] = term index of kind-slot k (or -1), and
+    // ord[q] = kind-slot of the q-th term in insertion order.
+    int fast;
+    int slot[8];
+    int ord[MAX_TERMS];
+    T llg_a;  // -gamma / (1 + alpha^2), src/llg/llg.jl:18
 };
+
+enum SlotKind : int { SL_EXCH = 0, SL_DMIB, SL_DMII, SL_ANIS, SL_CUBIC, SL_ZEE, SL_DEMAG, SL_THERM };
 
 // Per-launch values (by value; the parameter block above lives in device
 // memory and is read with wave-uniform scalar loads).

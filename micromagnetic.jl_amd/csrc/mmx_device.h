@@ -57,6 +57,14 @@ __device__ __forceinline__ double philox_normal(uint64_t seed, uint64_t step, ui
     return (idx & 1) ? b : a;
 }
 
+// NaN-propagating max (a NaN anywhere makes the result NaN, like Julia's maximum)
+__device__ __forceinline__ double pmax(double a, double b)
+{
+    if (a != a) return a;
+    if (b != b) return b;
+    return b > a ? b : a;
+}
+
 // ---- neighbours: indexpbc of src/micro/mesh.jl:31-43 ---------------------
 // order (-x, +x, -y, +y, -z, +z); -1 when there is no neighbour.
 template <class T>
@@ -326,6 +334,382 @@ __device__ __forceinline__ void cell_field(const FieldParams<T> &P, const Dyn &d
     }
 }
 
+// Field and energy of a term that needs only this cell (Zeeman, anisotropies,
+// demag, thermal); returns false for the neighbour terms.
+template <class T>
+__device__ __forceinline__ bool local_term(const TermDev<T> &D, const FieldParams<T> &P, const Dyn &dyn, int64_t I,
+                                           T mx, T my, T mz, T Ms, T hdx, T hdy, T hdz, T &hx, T &hy, T &hz,
+                                           double &e)
+{
+#pragma clang fp contract(off)
+    const T V = P.volume;
+    const int64_t N = P.N;
+    hx = hy = hz = T(0);
+    e = 0;
+    switch (D.kind) {
+    case K_ZEEMAN:
+    case K_ZEEMAN_U: {  // static field; energy src/micro/kernels.jl:14-20
+        if (D.kind == K_ZEEMAN) {
+            hx = D.a[I];
+            hy = D.b[I];
+            hz = D.c[I];
+        } else {
+            hx = D.s[0];
+            hy = D.s[1];
+            hz = D.s[2];
+        }
+        T mh = mx * hx + my * hy + mz * hz;
+        e = (double)(-V * Ms * mh);
+    } break;
+    case K_ANIS:
+    case K_ANIS_S: {  // anisotropy_kernel! / spatial: kernels.jl:54-105
+        if (Ms == T(0)) break;
+        T ax, ay, az;
+        if (D.kind == K_ANIS) {
+            ax = D.s[0];
+            ay = D.s[1];
+            az = D.s[2];
+        } else {
+            ax = D.b[I];
+            ay = D.b[N + I];
+            az = D.b[2 * N + I];
+        }
+        T Ku = D.uniform ? D.s[3] : D.a[I];
+        T Ms_inv = T(2.0) / Ms;
+        T sa = mx * ax + my * ay + mz * az;
+        hx = Ku * sa * ax * Ms_inv;
+        hy = Ku * sa * ay * Ms_inv;
+        hz = Ku * sa * az * Ms_inv;
+        e = (double)(Ku * (T(1.0) - sa * sa) * V);
+    } break;
+    case K_CUBIC: {  // cubic_anisotropy_kernel!: kernels.jl:110-137
+        if (Ms == T(0)) break;
+        T Kc = D.uniform ? D.s[9] : D.a[I];
+        T Ms_inv = T(4.0) * Kc / Ms;
+        T mxp = D.s[0] * mx + D.s[1] * my + D.s[2] * mz;
+        T myp = D.s[3] * mx + D.s[4] * my + D.s[5] * mz;
+        T mzp = D.s[6] * mx + D.s[7] * my + D.s[8] * mz;
+        T mxp3 = mxp * mxp * mxp;
+        T myp3 = myp * myp * myp;
+        T mzp3 = mzp * mzp * mzp;
+        hx = Ms_inv * (mxp3 * D.s[0] + myp3 * D.s[3] + mzp3 * D.s[6]);
+        hy = Ms_inv * (mxp3 * D.s[1] + myp3 * D.s[4] + mzp3 * D.s[7]);
+        hz = Ms_inv * (mxp3 * D.s[2] + myp3 * D.s[5] + mzp3 * D.s[8]);
+        e = (double)(-Kc * (mxp * mxp3 + myp * myp3 + mzp * mzp3) * V);
+    } break;
+    case K_DEMAG: {  // collect_h_kernel!: src/micro/demag.jl:435-449
+        hx = hdx;
+        hy = hdy;
+        hz = hdz;
+        T mh = mx * hx + my * hy + mz * hz;
+        e = (double)(T(-0.5) * Ms * V * mh);
+    } break;
+    case K_THERMAL: {  // stochastic_field_kernel!: kernels.jl:469-494
+        if (!(Ms > T(0))) break;
+        T Tl = (D.uniform ? D.s[1] : D.a[I]) + D.s[0];
+        T scale = (T)sqrt((T)dyn.thermal_factor * Tl / Ms);
+        uint64_t base = 3 * (uint64_t)I;
+        hx = (T)philox_normal(P.seed, dyn.noise_step, base) * scale;
+        hy = (T)philox_normal(P.seed, dyn.noise_step, base + 1) * scale;
+        hz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
+        e = (double)(-Ms * V * (mx * hx + my * hy + mz * hz));
+    } break;
+    default:
+        return false;
+    }
+    return true;
+}
+
+// Slot form of cell_field for meshes whose terms are at most one of each kind:
+// each neighbour is loaded once and contributes to the exchange and DMI slots
+// in the reference's neighbour order (-x,+x,-y,+y,-z,+z) per term; every
+// slot's field is then added in insertion order, exactly as cell_field does.
+// Returns m of the cell in mx,my,mz.
+template <class T>
+__device__ __forceinline__ void cell_field_slots(const FieldParams<T> &P, const Dyn &dyn, int I, int i, int j, int k,
+                                                 const T *__restrict__ m, T hdx, T hdy, T hdz, T &mx, T &my, T &mz,
+                                                 T &Hx, T &Hy, T &Hz)
+{
+#pragma clang fp contract(off)
+    const int N = (int)P.N;
+    mx = m[I];
+    my = m[N + I];
+    mz = m[2 * N + I];
+    const T Ms = P.mu0Ms[I];
+    const int se = P.slot[SL_EXCH], sb = P.slot[SL_DMIB], si = P.slot[SL_DMII];
+    bool ex_on = false, ex_sp = false;
+    T exA = 0, nab0 = 0, nab1 = 0, nab2 = 0;
+    if (se >= 0) {
+        const TermDev<T> &D = P.t[se];
+        ex_sp = D.kind == K_EXCH_S;
+        nab0 = D.s[0];
+        nab1 = D.s[1];
+        nab2 = D.s[2];
+        if (ex_sp) exA = D.a[I];
+        ex_on = (Ms != T(0)) && (!ex_sp || exA != T(0));
+    }
+    bool bd_on = false, bd_sp = false;
+    T bs0 = 0, bs1 = 0, bs2 = 0, bDx = 0, bDy = 0, bDz = 0;
+    if (sb >= 0) {
+        const TermDev<T> &D = P.t[sb];
+        bd_sp = D.kind == K_DMI_BULK_S;
+        bs0 = D.s[0];
+        bs1 = D.s[1];
+        bs2 = D.s[2];
+        if (bd_sp) {
+            bDx = D.a[I];
+            bDy = D.b[I];
+            bDz = D.c[I];
+        }
+        bd_on = Ms != T(0);
+    }
+    bool id_on = false, id_un = false;
+    T iDI = 0, id0 = 0, id1 = 0, iDe = 0;
+    if (si >= 0) {
+        const TermDev<T> &D = P.t[si];
+        id_un = D.uniform != 0;
+        id0 = D.s[0];
+        id1 = D.s[1];
+        iDe = D.s[4];
+        iDI = id_un ? D.s[3] : D.a[I];
+        id_on = (Ms != T(0)) && (iDI != T(0));
+    }
+    T fex = 0, fey = 0, fez = 0, fbx = 0, fby = 0, fbz = 0, fix = 0, fiy = 0, fiz = 0;
+    if (ex_on || bd_on || id_on) {
+        const int nx = P.nx, ny = P.ny, nz = P.nz, sxy = nx * ny;
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            int id = -1, t;
+            if (q == 0) { t = i - 1; if (P.px && t < 0) t += nx; id = (t >= 0) ? I - i + t : -1; }
+            if (q == 1) { t = i + 1; if (P.px && t >= nx) t -= nx; id = (t < nx) ? I - i + t : -1; }
+            if (q == 2) { t = j - 1; if (P.py && t < 0) t += ny; id = (t >= 0) ? I + (t - j) * nx : -1; }
+            if (q == 3) { t = j + 1; if (P.py && t >= ny) t -= ny; id = (t < ny) ? I + (t - j) * nx : -1; }
+            if (q == 4) { t = k - 1; if (P.pz && t < 0) t += nz; id = (t >= 0) ? I + (t - k) * sxy : -1; }
+            if (q == 5) { t = k + 1; if (P.pz && t >= nz) t -= nz; id = (t < nz) ? I + (t - k) * sxy : -1; }
+            if (id < 0) continue;
+            if (!(P.mu0Ms[id] > T(0))) continue;
+            const T ax = m[id], ay = m[N + id], az = m[2 * N + id];
+            if (ex_on) {  // kernels.jl:269-304 (uniform), :226-266 (spatial)
+                const T nab = (q < 2) ? nab0 : ((q < 4) ? nab1 : nab2);
+                if (!ex_sp) {
+                    fex += nab * (ax - mx);
+                    fey += nab * (ay - my);
+                    fez += nab * (az - mz);
+                } else {
+                    const T A_nb = P.t[se].a[id];
+                    if (A_nb != T(0)) {
+                        const T coeff = (T(2) * exA * A_nb / (exA + A_nb)) * nab;
+                        fex += coeff * (ax - mx);
+                        fey += coeff * (ay - my);
+                        fez += coeff * (az - mz);
+                    }
+                }
+            }
+            if (bd_on) {
+                if (!bd_sp) {  // bulkdmi_kernel!: kernels.jl:306-344
+                    if (q == 0) { fby += bs0 * (-az); fbz += bs0 * ay; }
+                    if (q == 1) { fby += bs0 * az;    fbz += bs0 * (-ay); }
+                    if (q == 2) { fbx += bs1 * az;    fbz += bs1 * (-ax); }
+                    if (q == 3) { fbx += bs1 * (-az); fbz += bs1 * ax; }
+                    if (q == 4) { fbx += bs2 * (-ay); fby += bs2 * ax; }
+                    if (q == 5) { fbx += bs2 * ay;    fby += bs2 * (-ax); }
+                } else {  // spatial_vector_bulkdmi_kernel!: kernels.jl:347-422
+                    const TermDev<T> &D = P.t[sb];
+                    if (q < 2) {
+                        const T Dn = D.a[id];
+                        if (bDx != T(0) && Dn != T(0)) {
+                            const T De = T(2) * bDx * Dn / (bDx + Dn);
+                            const T a = q == 0 ? bs0 : -bs0;
+                            fby += De * (-a * az);
+                            fbz += De * (a * ay);
+                        }
+                    } else if (q < 4) {
+                        const T Dn = D.b[id];
+                        if (bDy != T(0) && Dn != T(0)) {
+                            const T De = T(2) * bDy * Dn / (bDy + Dn);
+                            const T a = q == 2 ? bs1 : -bs1;
+                            fbx += De * (a * az);
+                            fbz += De * (-a * ax);
+                        }
+                    } else {
+                        const T Dn = D.c[id];
+                        if (bDz != T(0) && Dn != T(0)) {
+                            const T De = T(2) * bDz * Dn / (bDz + Dn);
+                            const T a = q == 4 ? bs2 : -bs2;
+                            fbx += De * (-a * ay);
+                            fby += De * (a * ax);
+                        }
+                    }
+                }
+            }
+            if (id_on && q < 4) {  // interfacial_dmi_kernel!: kernels.jl:426-466
+                const T Dn = id_un ? iDI : P.t[si].a[id];
+                if (Dn != T(0)) {
+                    const T De = id_un ? iDe : T(2) * iDI * Dn / (iDI + Dn);
+                    const T c = De * (q < 2 ? id0 : id1);
+                    if (q == 0) { fix += c * az;    fiz += c * (-ax); }
+                    if (q == 1) { fix += c * (-az); fiz += c * ax; }
+                    if (q == 2) { fiy += c * az;    fiz += c * (-ay); }
+                    if (q == 3) { fiy += c * (-az); fiz += c * ay; }
+                }
+            }
+        }
+    }
+    const T Ms_inv = (Ms != T(0)) ? T(1.0) / Ms : T(0);
+    // local slots
+    T ahx = 0, ahy = 0, ahz = 0;  // uniaxial
+    if (P.slot[SL_ANIS] >= 0 && Ms != T(0)) {
+        const TermDev<T> &D = P.t[P.slot[SL_ANIS]];
+        T ax, ay, az;
+        if (D.kind == K_ANIS) {
+            ax = D.s[0];
+            ay = D.s[1];
+            az = D.s[2];
+        } else {
+            ax = D.b[I];
+            ay = D.b[N + I];
+            az = D.b[2 * N + I];
+        }
+        const T Ku = D.uniform ? D.s[3] : D.a[I];
+        const T Ms2 = T(2.0) / Ms;
+        const T sa = mx * ax + my * ay + mz * az;
+        ahx = Ku * sa * ax * Ms2;
+        ahy = Ku * sa * ay * Ms2;
+        ahz = Ku * sa * az * Ms2;
+    }
+    T chx = 0, chy = 0, chz = 0;  // cubic
+    if (P.slot[SL_CUBIC] >= 0 && Ms != T(0)) {
+        const TermDev<T> &D = P.t[P.slot[SL_CUBIC]];
+        const T Kc = D.uniform ? D.s[9] : D.a[I];
+        const T Ms4 = T(4.0) * Kc / Ms;
+        const T mxp = D.s[0] * mx + D.s[1] * my + D.s[2] * mz;
+        const T myp = D.s[3] * mx + D.s[4] * my + D.s[5] * mz;
+        const T mzp = D.s[6] * mx + D.s[7] * my + D.s[8] * mz;
+        const T mxp3 = mxp * mxp * mxp, myp3 = myp * myp * myp, mzp3 = mzp * mzp * mzp;
+        chx = Ms4 * (mxp3 * D.s[0] + myp3 * D.s[3] + mzp3 * D.s[6]);
+        chy = Ms4 * (mxp3 * D.s[1] + myp3 * D.s[4] + mzp3 * D.s[7]);
+        chz = Ms4 * (mxp3 * D.s[2] + myp3 * D.s[5] + mzp3 * D.s[8]);
+    }
+    T zhx = 0, zhy = 0, zhz = 0;  // Zeeman
+    if (P.slot[SL_ZEE] >= 0) {
+        const TermDev<T> &D = P.t[P.slot[SL_ZEE]];
+        if (D.kind == K_ZEEMAN) {
+            zhx = D.a[I];
+            zhy = D.b[I];
+            zhz = D.c[I];
+        } else {
+            zhx = D.s[0];
+            zhy = D.s[1];
+            zhz = D.s[2];
+        }
+    }
+    T thx = 0, thy = 0, thz = 0;  // thermal
+    if (P.slot[SL_THERM] >= 0 && Ms > T(0)) {
+        const TermDev<T> &D = P.t[P.slot[SL_THERM]];
+        const T Tl = (D.uniform ? D.s[1] : D.a[I]) + D.s[0];
+        const T scale = (T)sqrt((T)dyn.thermal_factor * Tl / Ms);
+        const uint64_t base = 3 * (uint64_t)I;
+        thx = (T)philox_normal(P.seed, dyn.noise_step, base) * scale;
+        thy = (T)philox_normal(P.seed, dyn.noise_step, base + 1) * scale;
+        thz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
+    }
+    Hx = Hy = Hz = T(0);
+#pragma unroll
+    for (int q = 0; q < MAX_TERMS; q++) {
+        if (q >= P.nterms) break;
+        const int sk = P.ord[q];
+        T hx, hy, hz;
+        if (sk == SL_EXCH) { hx = ex_on ? fex * Ms_inv : T(0); hy = ex_on ? fey * Ms_inv : T(0); hz = ex_on ? fez * Ms_inv : T(0); }
+        else if (sk == SL_DMIB) { hx = bd_on ? fbx * Ms_inv : T(0); hy = bd_on ? fby * Ms_inv : T(0); hz = bd_on ? fbz * Ms_inv : T(0); }
+        else if (sk == SL_DMII) { hx = id_on ? fix * Ms_inv : T(0); hy = id_on ? fiy * Ms_inv : T(0); hz = id_on ? fiz * Ms_inv : T(0); }
+        else if (sk == SL_ANIS) { hx = ahx; hy = ahy; hz = ahz; }
+        else if (sk == SL_CUBIC) { hx = chx; hy = chy; hz = chz; }
+        else if (sk == SL_ZEE) { hx = zhx; hy = zhy; hz = zhz; }
+        else if (sk == SL_DEMAG) { hx = hdx; hy = hdy; hz = hdz; }
+        else { hx = thx; hy = thy; hz = thz; }
+        Hx = Hx + hx;
+        Hy = Hy + hy;
+        Hz = Hz + hz;
+    }
+}
+
+// Epilogue of the specialised kernels: MODE is EPI_STAGE, EPI_RK4_FINAL or EPI_ERR.
+template <class T, int MODE>
+__device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I, int i,
+                                             int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz, double &red)
+{
+#pragma clang fp contract(off)
+    const int N = (int)P.N;
+    T mx, my, mz, Hx, Hy, Hz;
+    cell_field_slots(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz);
+    // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
+    T kk[3];
+    if (P.pins && P.pins[I]) {
+        kk[0] = kk[1] = kk[2] = T(0);
+    } else {
+        const T alpha = P.alpha, a = P.llg_a;
+        const T mm = mx * mx + my * my + mz * mz;
+        const T mh = mx * Hx + my * Hy + mz * Hz;
+        const T h1 = mm * Hx - mh * mx, h2 = mm * Hy - mh * my, h3 = mm * Hz - mh * mz;
+        T f1 = 0, f2 = 0, f3 = 0;
+        if (P.precession) {
+            f1 = -mz * h2 + my * h3;
+            f2 = mz * h1 - mx * h3;
+            f3 = -my * h1 + mx * h2;
+        }
+        kk[0] = a * (f1 - h1 * alpha);
+        kk[1] = a * (f2 - h2 * alpha);
+        kk[2] = a * (f3 - h3 * alpha);
+    }
+    if (E.kstore) {
+        E.kstore[I] = kk[0];
+        E.kstore[N + I] = kk[1];
+        E.kstore[2 * N + I] = kk[2];
+    }
+    if constexpr (MODE == EPI_ERR) {
+        double mval = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int o = c * N + I;
+            double err = 0;
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (q < E.np) err = (q == 0) ? E.cp[0] * (double)E.kp[0][o] : err + E.cp[q] * (double)E.kp[q][o];
+            err = err + E.cself * (double)kk[c];
+            mval = pmax(mval, fabs(err / (1.0 + fabs((double)m[o]))));
+        }
+        red = pmax(red, mval);
+    } else {
+        T y[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int o = c * N + I;
+            if constexpr (MODE == EPI_RK4_FINAL) {
+                const T s = E.kp[0][o] + T(2) * E.kp[1][o] + T(2) * E.kp[2][o] + kk[c];
+                y[c] = E.y0[o] + (T)E.cself * s;
+            } else {
+                T s = E.y0[o];
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+                    if (q < E.np) s = s + (T)E.cp[q] * E.kp[q][o];
+                y[c] = s + (T)E.cself * kk[c];
+            }
+        }
+        if (E.normalize) {
+            const T m2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+            if (m2 > T(0)) {
+                const T len = T(1) / sqrt(m2);
+                y[0] *= len;
+                y[1] *= len;
+                y[2] *= len;
+            }
+        }
+        E.yout[I] = y[0];
+        E.yout[N + I] = y[1];
+        E.yout[2 * N + I] = y[2];
+    }
+}
+
 // llg_rhs_kernel!: src/llg/llg.jl:6-40
 template <class T>
 __device__ __forceinline__ void llg_rhs(const FieldParams<T> &P, int64_t I, T mx, T my, T mz, T hx,
@@ -352,14 +736,6 @@ __device__ __forceinline__ void llg_rhs(const FieldParams<T> &P, int64_t I, T mx
     dx = a * (f1 - h1 * alpha);
     dy = a * (f2 - h2 * alpha);
     dz = a * (f3 - h3 * alpha);
-}
-
-// NaN-propagating max (a NaN anywhere makes the result NaN, like Julia's maximum)
-__device__ __forceinline__ double pmax(double a, double b)
-{
-    if (a != a) return a;
-    if (b != b) return b;
-    return b > a ? b : a;
 }
 
 // The fused last step of one RHS evaluation for cell I.  `red` accumulates the
@@ -460,27 +836,35 @@ __device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Dyn &
 }
 
 // Block-wide reduction of `nval` doubles per thread into
-// partial[blockIdx*stride + v]; works for any block size up to 1024.
+// partial[blockIdx*stride + v]: wave reduction with shuffles, then one value
+// per wave through a small shared array (any block size up to 1024).
+__device__ __forceinline__ double wave_reduce(double v, bool is_max)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = is_max ? pmax(v, o) : v + o;
+    }
+    return v;
+}
+
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double *red, int nval, bool is_max, double *partial,
                                                    int stride)
 {
-    __shared__ double sred[1024];
+    __shared__ double sw[16];
     const int nt = blockDim.x * blockDim.y;
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    int p2 = 1;
-    while (p2 < nt) p2 <<= 1;
+    const int nw = (nt + 63) / 64;
     for (int v = 0; v < nval; v++) {
-        sred[tid] = red[v];
+        double x = wave_reduce(red[v], is_max);
+        if ((tid & 63) == 0) sw[tid >> 6] = x;
         __syncthreads();
-        for (int s = p2 / 2; s > 0; s >>= 1) {
-            if (tid < s && tid + s < nt) {
-                double a = sred[tid], b = sred[tid + s];
-                sred[tid] = is_max ? pmax(a, b) : a + b;
-            }
-            __syncthreads();
+        if (tid == 0) {
+            double acc = sw[0];
+            for (int w = 1; w < nw; w++) acc = is_max ? pmax(acc, sw[w]) : acc + sw[w];
+            partial[(size_t)blockIdx.x * stride + v] = acc;
         }
-        if (tid == 0) partial[(size_t)blockIdx.x * stride + v] = sred[0];
         __syncthreads();
     }
 }

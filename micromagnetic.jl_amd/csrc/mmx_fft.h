@@ -1,10 +1,10 @@
 // mmx_fft.h -- register-resident Stockham FFT for power-of-two lengths.
 //
 // One sequence of length L is spread over TPS(L) = L / EPT(L) threads; thread
-// j in [0, TPS) holds EPT(L) = min(L, 8) elements v[m] = x[j + m*TPS] on entry
+// j in [0, TPS) holds EPT(L) = min(L, E) elements (E = 8 or 4) v[m] = x[j + m*TPS] on entry
 // and X[j + m*TPS] on exit, so a column can be loaded from and stored to
-// global memory in the same pattern.  Stages are radix 8 (a final radix 4 or 2
-// when log2 L is not a multiple of 3); values move between stages through an
+// global memory in the same pattern.  Stages are radix E (a final smaller
+// radix when log2 L is not a multiple of log2 E); values move between stages through an
 // LDS exchange buffer of L elements per sequence.  Twiddles come from an FP64-
 // accurate table tw[t] = exp(-2 pi i t / 2^lgLmax) indexed at t << (lgLmax - lgL).
 #pragma once
@@ -12,9 +12,9 @@
 
 namespace mmx {
 
-template <int L>
+template <int L, int E = 8>
 struct FftShape {
-    static constexpr int EPT = L >= 8 ? 8 : L;
+    static constexpr int EPT = L >= E ? E : L;
     static constexpr int TPS = L / EPT;
     static constexpr int LG = L <= 1 ? 0 : (L <= 2 ? 1 : (L <= 4 ? 2 : (L <= 8 ? 3 : (L <= 16 ? 4 : (L <= 32 ? 5 : (L <= 64 ? 6 : (L <= 128 ? 7 : (L <= 256 ? 8 : (L <= 512 ? 9 : (L <= 1024 ? 10 : (L <= 2048 ? 11 : 12)))))))))));
 };
@@ -87,11 +87,11 @@ __device__ __forceinline__ void dftR(cplx<T> *a)
 // (q < EPT/Rs) reads v[q + r*EPT/Rs] = x[jb + r*L/Rs], applies W_{Ns Rs}^{r (jb mod Ns)},
 // and writes y[(jb/Ns) Ns Rs + jb mod Ns + r Ns].  The last stage (Ns Rs = L)
 // leaves its outputs in v at the input positions.
-template <class T, int L, int Rs, int Ns, bool INV>
+template <class T, int L, int E, int Rs, int Ns, bool INV>
 __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw,
                                           int twsh)
 {
-    constexpr int EPT = FftShape<L>::EPT, TPS = FftShape<L>::TPS;
+    constexpr int EPT = FftShape<L, E>::EPT, TPS = FftShape<L, E>::TPS;
     constexpr int NQ = EPT / Rs;
     constexpr bool LAST = (Ns * Rs == L);
     constexpr int TWSTEP = L / (Ns * Rs);
@@ -132,24 +132,25 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, const
     }
 }
 
-template <class T, int L, int Ns, bool INV>
+template <class T, int L, int E, int Ns, bool INV>
 __device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int twsh)
 {
     if constexpr (Ns < L) {
         constexpr int rem = L / Ns;
-        constexpr int Rs = rem >= 8 ? 8 : rem;
-        fft_stage<T, L, Rs, Ns, INV>(v, j, lds, tw, twsh);
-        fft_stages<T, L, Ns * Rs, INV>(v, j, lds, tw, twsh);
+        constexpr int EPT = FftShape<L, E>::EPT;
+        constexpr int Rs = rem >= EPT ? EPT : rem;
+        fft_stage<T, L, E, Rs, Ns, INV>(v, j, lds, tw, twsh);
+        fft_stages<T, L, E, Ns * Rs, INV>(v, j, lds, tw, twsh);
     }
 }
 
 // Transform of one sequence.  Threads of the same block must all call this
 // together (the exchanges use __syncthreads); `lds` is this sequence's buffer
 // of L elements.  `active` threads hold data; inactive ones only join barriers.
-template <class T, int L, bool INV>
+template <class T, int L, bool INV, int E = 8>
 __device__ __forceinline__ void fft_reg(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int lgLmax)
 {
-    if constexpr (L > 1) fft_stages<T, L, 1, INV>(v, j, lds, tw, lgLmax - FftShape<L>::LG);
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV>(v, j, lds, tw, lgLmax - FftShape<L, E>::LG);
 }
 
 }  // namespace mmx

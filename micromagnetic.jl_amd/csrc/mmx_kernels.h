@@ -17,7 +17,8 @@ void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s);
 // local field (no demag) + epilogue over all cells; returns number of blocks
 // used (the number of partial-reduction slots written).
 template <class T>
-int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, hipStream_t s);
+int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, int fast,
+                 hipStream_t s);
 
 // demag passes
 template <class T>
@@ -31,7 +32,7 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
                   int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s);
 template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
-              const T *m, const cplx<T> *tw, hipStream_t s);
+              const T *m, int fast, const cplx<T> *tw, hipStream_t s);
 
 // tensor set-up (FP64)
 void launch_tensor_pad(const DemagGeom &G, const double *rt, int comp, double *pad, hipStream_t s);

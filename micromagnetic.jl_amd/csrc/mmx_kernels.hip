@@ -25,15 +25,19 @@ struct Shape : FftShape<L> {
     // y columns: TY consecutive kx columns per block (>= 8 for whole cache lines)
     static constexpr int TY = (256 / TPS) >= 8 ? (256 / TPS) : ((1024 / TPS) < 8 ? (1024 / TPS) : 8);
     static constexpr int BY = TY * TPS;
-    // z columns: TZ columns, all three components in one thread
-    static constexpr int TZ = (256 / TPS) > 64 ? 64 : ((256 / TPS) < 1 ? 1 : (256 / TPS));
-    static constexpr int BZ = TZ * TPS;
+    // z columns: TZ columns, all three components in one thread, 4 elements each
+    static constexpr int TPSZ = FftShape<L, 4>::TPS;
+    static constexpr int TZ = (256 / TPSZ) > 64 ? 64 : ((256 / TPSZ) < 1 ? 1 : (256 / TPSZ));
+    static constexpr int BZ = TZ * TPSZ;
     // x rows: SX sequences per block
     static constexpr int SX = (256 / TPS) < 1 ? 1 : (256 / TPS);
     static constexpr int BX = SX * TPS;
-    // last pass: RX rows x 3 components per block
-    static constexpr int RX = (256 / (3 * TPS)) < 1 ? 1 : (256 / (3 * TPS));
+    // last pass: RX rows x 3 components per block; 3*RX*TPS is a whole number
+    // of 64-wide waves (the block reductions shuffle across full waves)
+    static constexpr int GX = TPS >= 64 ? 1 : 64 / TPS;
+    static constexpr int RX = (256 / (3 * TPS)) / GX * GX < GX ? GX : (256 / (3 * TPS)) / GX * GX;
     static constexpr int B5 = 3 * RX * TPS;
+    static_assert(B5 % 64 == 0, "last-pass blocks must be whole waves");
 };
 
 // ---------------------------------------------------------------------------
@@ -146,7 +150,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
                                                          int nzb, int n_in, const cplx<T> *__restrict__ tw)
 {
     using S = Shape<L>;
-    constexpr int EPT = S::EPT, TPS = S::TPS, TZ = S::TZ;
+    constexpr int EPT = FftShape<L, 4>::EPT, TPS = FftShape<L, 4>::TPS, TZ = S::TZ;
     constexpr int LS = L + 1;
     constexpr int NC = FWD_ONLY ? 1 : 3;
     __shared__ cplx<T> lds[TZ * LS];
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
         }
     cplx<T> *buf = lds + t * LS;
 #pragma unroll
-    for (int c = 0; c < NC; c++) fft_reg<T, L, false>(v[c], j, buf, tw, G.lgLmax);
+    for (int c = 0; c < NC; c++) fft_reg<T, L, false, 4>(v[c], j, buf, tw, G.lgLmax);
     if constexpr (FWD_ONLY) {
         if (!ok) return;
 #pragma unroll
@@ -197,7 +201,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
             v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
         }
 #pragma unroll
-        for (int c = 0; c < NC; c++) fft_reg<T, L, true>(v[c], j, buf, tw, G.lgLmax);
+        for (int c = 0; c < NC; c++) fft_reg<T, L, true, 4>(v[c], j, buf, tw, G.lgLmax);
         if (!ok) return;
         const int nz = G.nz;
 #pragma unroll
@@ -215,7 +219,22 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 // fused epilogue for every cell of those rows.  A element (c, row, kx) at
 // (c*ny*nz + row)*KS + kx.
 // ---------------------------------------------------------------------------
-template <class T, int H>
+// V selects the epilogue: 0 = any mode with the generic field (cell_field),
+// or the slot-form specialisations 1 = EPI_STAGE, 2 = EPI_RK4_FINAL, 3 = EPI_ERR.
+template <int V>
+struct VMode {
+    static constexpr int mode = V == 1 ? EPI_STAGE : (V == 2 ? EPI_RK4_FINAL : EPI_ERR);
+};
+
+template <class T, int V>
+__device__ __forceinline__ void epilogue(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I, int i,
+                                         int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
+{
+    if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
+    else finish_slots<T, VMode<V>::mode>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
+}
+
+template <class T, int H, int V>
 __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
                                                       const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
                                                       const T *__restrict__ m, const cplx<T> *__restrict__ tw)
@@ -267,7 +286,7 @@ __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> 
         const T hx = (x & 1) ? h0.y : h0.x;
         const T hy = (x & 1) ? h1.y : h1.x;
         const T hz = (x & 1) ? h2.y : h2.x;
-        finish_cell(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
+        epilogue<T, V>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
     }
     if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
@@ -275,7 +294,7 @@ __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> 
 // ---------------------------------------------------------------------------
 // Local-field kernel (meshes without demag): one thread per cell.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int V>
 __global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
                                                  const T *__restrict__ m)
 {
@@ -287,7 +306,7 @@ __global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restric
         int64_t q = I / nx;
         int j = (int)(q % ny);
         int k = (int)(q / ny);
-        finish_cell(P, dyn, E, I, i, j, k, m, T(0), T(0), T(0), red);
+        epilogue<T, V>(P, dyn, E, I, i, j, k, m, T(0), T(0), T(0), red);
     }
     if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
@@ -434,6 +453,16 @@ __global__ void k_maxabs(const T *__restrict__ v, int64_t n, double *__restrict_
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// epilogue variant for a mode: the slot-form kernels take the integrator stages
+static int variant(int mode, int fast)
+{
+    if (!fast) return 0;
+    if (mode == EPI_STAGE) return 1;
+    if (mode == EPI_RK4_FINAL) return 2;
+    if (mode == EPI_ERR) return 3;
+    return 0;
+}
+
 static int grid_for(int64_t n, int cap = 8192)
 {
     int64_t g = (n + BLOCK - 1) / BLOCK;
@@ -465,10 +494,15 @@ void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s)
 }
 
 template <class T>
-int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, hipStream_t s)
+int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, int fast,
+                 hipStream_t s)
 {
     int g = grid_for(N, max_local_blocks());
-    hipLaunchKernelGGL(k_local<T>, dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    const int v = variant(E.mode, fast);
+    if (v == 1) hipLaunchKernelGGL((k_local<T, 1>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    else if (v == 2) hipLaunchKernelGGL((k_local<T, 2>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    else if (v == 3) hipLaunchKernelGGL((k_local<T, 3>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    else hipLaunchKernelGGL((k_local<T, 0>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     return g;
 }
 
@@ -541,14 +575,22 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
 
 template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
-              const T *m, const cplx<T> *tw, hipStream_t s)
+              const T *m, int fast, const cplx<T> *tw, hipStream_t s)
 {
     const int rows = G.ny * G.nz;
     int g = 0;
 #define P5_CALL(H)                                                                                   \
     {                                                                                                \
         g = (rows + Shape<H>::RX - 1) / Shape<H>::RX;                                                \
-        hipLaunchKernelGGL((k_p5<T, H>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw);  \
+        const int v = variant(E.mode, fast);                                                         \
+        if (v == 1)                                                                                  \
+            hipLaunchKernelGGL((k_p5<T, H, 1>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 2)                                                                             \
+            hipLaunchKernelGGL((k_p5<T, H, 2>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 3)                                                                             \
+            hipLaunchKernelGGL((k_p5<T, H, 3>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_p5<T, H, 0>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
     }
     MMX_LG_SWITCH(G.lgH, P5_CALL)
 #undef P5_CALL
@@ -611,7 +653,8 @@ int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s)
     template void launch_soa_to_aos<T>(const T *, double *, int64_t, hipStream_t);                           \
     template void launch_cast<T>(const double *, T *, int64_t, hipStream_t);                                 \
     template void launch_to_double<T>(const T *, double *, int64_t, hipStream_t);                            \
-    template int launch_local<T>(int64_t, const FieldParams<T> *, const Dyn &, const Epi<T> &, const T *, hipStream_t); \
+    template int launch_local<T>(int64_t, const FieldParams<T> *, const Dyn &, const Epi<T> &, const T *, int,  \
+                                 hipStream_t);                                                               \
     template void launch_p1<T>(const DemagGeom &, const T *, int64_t, const T *, int, int, int, int, cplx<T> *, \
                                const cplx<T> *, hipStream_t);                                                \
     template void launch_coly<T>(const DemagGeom &, const cplx<T> *, cplx<T> *, int, int, int, int, int, int,  \
@@ -619,7 +662,7 @@ int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s)
     template void launch_convz<T>(const DemagGeom &, cplx<T> *, const T *, const T *, int, int, int, int,      \
                                   const cplx<T> *, hipStream_t);                                                              \
     template int launch_p5<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,        \
-                              const Epi<T> &, const T *, const cplx<T> *, hipStream_t);                      \
+                              const Epi<T> &, const T *, int, const cplx<T> *, hipStream_t);                 \
     template void launch_tensor_extract<T>(const DemagGeom &, const cplx<double> *, int, double, T *,         \
                                            hipStream_t);                                                     \
     template int launch_maxdiff<T>(const T *, const T *, int64_t, double *, hipStream_t);          \
