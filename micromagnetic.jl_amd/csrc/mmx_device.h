@@ -486,9 +486,13 @@ __device__ __forceinline__ void cell_field_slots(const FieldParams<T> &P, const 
             if (q == 3) { t = j + 1; if (P.py && t >= ny) t -= ny; id = (t < ny) ? I + (t - j) * nx : -1; }
             if (q == 4) { t = k - 1; if (P.pz && t < 0) t += nz; id = (t >= 0) ? I + (t - k) * sxy : -1; }
             if (q == 5) { t = k + 1; if (P.pz && t >= nz) t -= nz; id = (t < nz) ? I + (t - k) * sxy : -1; }
-            if (id < 0) continue;
-            if (!(P.mu0Ms[id] > T(0))) continue;
-            const T ax = m[id], ay = m[N + id], az = m[2 * N + id];
+            // all loads of this neighbour are issued together from a clamped
+            // index; whether it counts is decided afterwards (no load waits
+            // on another load's value)
+            const int idc = id >= 0 ? id : I;
+            const T nMs = P.mu0Ms[idc];
+            const T ax = m[idc], ay = m[N + idc], az = m[2 * N + idc];
+            if (id < 0 || !(nMs > T(0))) continue;
             if (ex_on) {  // kernels.jl:269-304 (uniform), :226-266 (spatial)
                 const T nab = (q < 2) ? nab0 : ((q < 4) ? nab1 : nab2);
                 if (!ex_sp) {
@@ -640,6 +644,19 @@ __device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn 
 {
 #pragma clang fp contract(off)
     const int N = (int)P.N;
+    // operands of the stage combination, loaded before the field so their
+    // latency overlaps the neighbour loads
+    T y0v[3] = {0, 0, 0}, kpv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if constexpr (MODE != EPI_ERR) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) y0v[c] = E.y0[c * N + I];
+    }
+    if constexpr (MODE == EPI_RK4_FINAL) {
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) kpv[q][c] = E.kp[q][c * N + I];
+    }
     T mx, my, mz, Hx, Hy, Hz;
     cell_field_slots(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz);
     // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
@@ -685,10 +702,10 @@ __device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn 
         for (int c = 0; c < 3; c++) {
             const int o = c * N + I;
             if constexpr (MODE == EPI_RK4_FINAL) {
-                const T s = E.kp[0][o] + T(2) * E.kp[1][o] + T(2) * E.kp[2][o] + kk[c];
-                y[c] = E.y0[o] + (T)E.cself * s;
+                const T s = kpv[0][c] + T(2) * kpv[1][c] + T(2) * kpv[2][c] + kk[c];
+                y[c] = y0v[c] + (T)E.cself * s;
             } else {
-                T s = E.y0[o];
+                T s = y0v[c];
 #pragma unroll
                 for (int q = 0; q < 6; q++)
                     if (q < E.np) s = s + (T)E.cp[q] * E.kp[q][o];

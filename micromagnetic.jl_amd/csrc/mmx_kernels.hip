@@ -25,10 +25,11 @@ struct Shape : FftShape<L> {
     // y columns: TY consecutive kx columns per block (>= 8 for whole cache lines)
     static constexpr int TY = (256 / TPS) >= 8 ? (256 / TPS) : ((1024 / TPS) < 8 ? (1024 / TPS) : 8);
     static constexpr int BY = TY * TPS;
-    // z columns: TZ columns, all three components in one thread, 4 elements each
+    // z columns: TZ columns at ky and TZ at Py-ky (which share the folded
+    // tensor), all three components in one thread, 4 elements each
     static constexpr int TPSZ = FftShape<L, 4>::TPS;
-    static constexpr int TZ = (256 / TPSZ) > 64 ? 64 : ((256 / TPSZ) < 1 ? 1 : (256 / TPSZ));
-    static constexpr int BZ = TZ * TPSZ;
+    static constexpr int TZ = (128 / TPSZ) > 64 ? 64 : ((128 / TPSZ) < 1 ? 1 : (128 / TPSZ));
+    static constexpr int BZ = 2 * TZ * TPSZ;
     // x rows: SX sequences per block
     static constexpr int SX = (256 / TPS) < 1 ? 1 : (256 / TPS);
     static constexpr int BX = SX * TPS;
@@ -43,7 +44,7 @@ struct Shape : FftShape<L> {
 // ---------------------------------------------------------------------------
 // P1: rows of n_in reals (zero-padded to Px = 2H) -> Kx = H+1 complex values
 // per row, via the half-length complex transform of z[j] = x[2j] + i x[2j+1].
-// Sequence s = c*nrows + row; row r of component c starts at in[c*cstride +
+// Sequence s = row*ncomp + c; row r of component c starts at in[c*cstride +
 // r*n_in]; inputs are multiplied by scale[r*n_in + x] when scale != null
 // (mu0 Ms, demag.jl:419-427).  Output A[(c*nrows + row)*KS + k].
 // ---------------------------------------------------------------------------
@@ -59,8 +60,10 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
     const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
     const int64_t seq = (int64_t)blockIdx.x * SX + ls;
     const bool active = seq < (int64_t)ncomp * nrows;
-    const int c = active ? (int)(seq / nrows) : 0;
-    const int64_t row = active ? seq % nrows : 0;
+    // the components of one row are neighbouring sequences, so the row's
+    // scale values (mu0 Ms) are fetched once and then hit the cache
+    const int c = active ? (int)(seq % ncomp) : 0;
+    const int64_t row = active ? seq / ncomp : 0;
     const int64_t rb = row * n_in;
     const T *src = in + c * cstride + rb;
     cplx<T> v[EPT];
@@ -153,15 +156,19 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
     constexpr int EPT = FftShape<L, 4>::EPT, TPS = FftShape<L, 4>::TPS, TZ = S::TZ;
     constexpr int LS = L + 1;
     constexpr int NC = FWD_ONLY ? 1 : 3;
-    __shared__ cplx<T> lds[TZ * LS];
-    const int t = threadIdx.x % TZ, j = threadIdx.x / TZ;
+    __shared__ cplx<T> lds[2 * TZ * LS];
+    const int half = threadIdx.x / (TZ * TPS);
+    const int tl = threadIdx.x % (TZ * TPS);
+    const int t = tl % TZ, j = tl / TZ;
     const int kx = blockIdx.x * TZ + t;
-    const int ky = blockIdx.y;
     const int KS = G.KS, Py = G.Py;
-    const bool ok = kx < G.Kx;
+    const int ky0 = blockIdx.y;
+    // half 1 takes the mirror Py-ky0 when it is a distinct row (0 < ky0 < Py/2)
+    const int ky = half ? Py - ky0 : ky0;
+    const bool ok = kx < G.Kx && (half == 0 || (ky0 >= 1 && ky0 < Py / 2));
     const int64_t cs_b = (int64_t)nzb * Py * KS;  // component stride in B
     const int64_t zs_b = (int64_t)Py * KS;        // z stride in B
-    cplx<T> *col = B + (int64_t)ky * KS + kx;
+    cplx<T> *col = B + (int64_t)(ok ? ky : 0) * KS + (ok ? kx : 0);
     cplx<T> v[NC][EPT];
 #pragma unroll
     for (int c = 0; c < NC; c++)
@@ -170,7 +177,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
             const int z = j + m * TPS;
             v[c][m] = (ok && z < n_in) ? col[c * cs_b + z * zs_b] : cplx<T>{0, 0};
         }
-    cplx<T> *buf = lds + t * LS;
+    cplx<T> *buf = lds + (half * TZ + t) * LS;
 #pragma unroll
     for (int c = 0; c < NC; c++) fft_reg<T, L, false, 4>(v[c], j, buf, tw, G.lgLmax);
     if constexpr (FWD_ONLY) {
@@ -243,14 +250,27 @@ __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> 
     constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
     constexpr int HS = H + 1;
     __shared__ cplx<T> lds[3 * RX * HS];
-    const int nx = G.nx, ny = G.ny;
-    const int nrows_tot = G.ny * G.nz;
-    const int row0 = blockIdx.x * RX;
-    const int nrows = min(RX, nrows_tot - row0);
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int nrows_tot = ny * nz;
+    // A block takes RX rows (x lines) at one y and consecutive z.  Groups are
+    // ordered y-major, z-minor, and each XCD (launch indices go round-robin
+    // over the 8 XCDs) gets one contiguous run of groups: the z+-1 rows a cell
+    // reads were just fetched by this or the previous block, the y+-1 rows a
+    // few blocks earlier, so both come from that XCD's L2.
+    const int gz = (nz + RX - 1) / RX;
+    const int ngroups = ny * gz;
+    const int per = (ngroups + 7) / 8;
+    const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (lb >= ngroups) {
+        if (threadIdx.x == 0 && (E.mode == EPI_ERR || E.mode == EPI_ENERGY)) E.partial[blockIdx.x] = 0.0;
+        return;
+    }
+    const int y0 = lb / gz, z0 = (lb % gz) * RX;
+    const int nrows = min(RX, nz - z0);
     const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
     const int c = ls / RX, r = ls % RX;
     const bool active = r < nrows;
-    const cplx<T> *row = A + ((int64_t)c * nrows_tot + row0 + (active ? r : 0)) * G.KS;
+    const cplx<T> *row = A + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
     const int sh = G.lgLmax - (FftShape<H>::LG + 1);
     cplx<T> v[EPT];
 #pragma unroll
@@ -278,8 +298,8 @@ __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> 
     double red = 0;
     for (int idx = threadIdx.x; idx < nrows * nx; idx += blockDim.x) {
         const int rr = idx / nx, x = idx - rr * nx;
-        const int rw = row0 + rr;
-        const int y = rw % ny, z = rw / ny;
+        const int y = y0, z = z0 + rr;
+        const int rw = z * ny + y;
         const cplx<T> h0 = lds[(0 * RX + rr) * HS + (x >> 1)];
         const cplx<T> h1 = lds[(1 * RX + rr) * HS + (x >> 1)];
         const cplx<T> h2 = lds[(2 * RX + rr) * HS + (x >> 1)];
@@ -563,7 +583,7 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
     (void)ncomp;
 #define CZ_CALL(L)                                                                                   \
     {                                                                                                \
-        dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py);                                   \
+        dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py / 2 + 1);                           \
         if (fwd_only)                                                                                \
             hipLaunchKernelGGL((k_convz<T, L, true>), grid, dim3(Shape<L>::BZ), 0, s, G, B, Kt, Lline, nzb, n_in, tw); \
         else                                                                                         \
@@ -577,11 +597,10 @@ template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
               const T *m, int fast, const cplx<T> *tw, hipStream_t s)
 {
-    const int rows = G.ny * G.nz;
     int g = 0;
 #define P5_CALL(H)                                                                                   \
     {                                                                                                \
-        g = (rows + Shape<H>::RX - 1) / Shape<H>::RX;                                                \
+        g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;                         \
         const int v = variant(E.mode, fast);                                                         \
         if (v == 1)                                                                                  \
             hipLaunchKernelGGL((k_p5<T, H, 1>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
@@ -603,7 +622,7 @@ int p5_blocks(const DemagGeom &G)
 #define RX_CALL(H) rx = Shape<H>::RX;
     MMX_LG_SWITCH(G.lgH, RX_CALL)
 #undef RX_CALL
-    return (G.ny * G.nz + rx - 1) / rx;
+    return (G.ny * ((G.nz + rx - 1) / rx) + 7) / 8 * 8;
 }
 
 void launch_tensor_pad(const DemagGeom &G, const double *rt, int comp, double *pad, hipStream_t s)

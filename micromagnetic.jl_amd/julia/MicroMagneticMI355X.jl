@@ -1,0 +1,229 @@
+# MicroMagneticMI355X.jl -- routes MicroMagnetic.jl's finite-difference hot path
+# (effective field, LLG right-hand side, RK4 / Heun / Dormand-Prince / BS23
+# stepping) through libmmx, the HIP library for MI355X (gfx950), via ccall.
+#
+# Load it after MicroMagnetic:
+#
+#     using MicroMagnetic
+#     include("MicroMagneticMI355X.jl"); using .MicroMagneticMI355X
+#     MicroMagneticMI355X.attach!(sim)        # after the add_* calls
+#
+# From then on the reference's own entry points -- effective_field(sim, spin, t),
+# llg_call_back, advance_step(sim, integrator), run_until, run_sim, relax -- run
+# on the GPU; sim.spin is kept in sync at the boundary.  The C ABI it calls is
+# declared in include/mmx.h; the mapping of each reference method to an entry
+# point is listed in INTEGRATION.md.
+module MicroMagneticMI355X
+
+using MicroMagnetic
+import MicroMagnetic: effective_field, advance_step, llg_call_back
+
+const libmmx = get(ENV, "MMX_LIBRARY", joinpath(@__DIR__, "..", "libmmx.so"))
+
+struct MeshDesc
+    nx::Int32
+    ny::Int32
+    nz::Int32
+    pbc::Int32
+    dx::Float64
+    dy::Float64
+    dz::Float64
+    precision::Int32
+    device::Int32
+end
+
+const contexts = IdDict{Any,Ptr{Cvoid}}()
+
+function check(rc::Integer)
+    if rc < 0
+        msg = unsafe_string(ccall((:mmx_last_error, libmmx), Cstring, ()))
+        rc == -3 && (@warn msg; return false)   # NaN error norm (adaptive_rk.jl:112-117)
+        rc == -4 && (@warn msg; return false)   # too many tries (adaptive_rk.jl:119-122)
+        error("libmmx: " * msg)
+    end
+    return true
+end
+
+host(a) = Array{Float64}(Array(a))
+
+"""
+    attach!(sim; device=0)
+
+Create the GPU context for `sim` (an FD `MicroSim`) and copy its mesh, Ms, pins,
+m and interaction terms into it, in insertion order.
+"""
+function attach!(sim::MicroMagnetic.MicroSim; device::Integer=0)
+    mesh = sim.mesh
+    pbc = Int32(mesh.xperiodic) | (Int32(mesh.yperiodic) << 1) | (Int32(mesh.zperiodic) << 2)
+    prec = MicroMagnetic.Float[] == Float32 ? Int32(32) : Int32(64)
+    desc = Ref(MeshDesc(mesh.nx, mesh.ny, mesh.nz, pbc, mesh.dx, mesh.dy, mesh.dz, prec, device))
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:mmx_create, libmmx), Cint, (Ref{Ptr{Cvoid}}, Ref{MeshDesc}), ctx, desc))
+    h = ctx[]
+    contexts[sim] = h
+    Ms = host(sim.mu0_Ms) ./ MicroMagnetic.mu_0
+    check(ccall((:mmx_set_Ms, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}), h, Ms))
+    pins = UInt8.(Array(sim.pins))
+    check(ccall((:mmx_set_pinning, libmmx), Cint, (Ptr{Cvoid}, Ptr{UInt8}), h, pins))
+    check(ccall((:mmx_init_m0, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), h, host(sim.spin), 0))
+    for term in sim.interactions
+        add_term!(h, term)
+    end
+    drv = sim.driver
+    check(ccall((:mmx_set_llg, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint), h,
+                Float64(drv.alpha), Float64(drv.gamma), Cint(drv.precession)))
+    set_integrator!(sim)
+    return h
+end
+
+function add_term!(h, t::MicroMagnetic.UniformExchange)
+    ccall((:mmx_add_exch, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, C_NULL,
+          t.Ax, t.Ay, t.Az) |> check
+end
+function add_term!(h, t::MicroMagnetic.SpatialExchange)
+    ccall((:mmx_add_exch, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, host(t.A),
+          0, 0, 0) |> check
+end
+function add_term!(h, t::MicroMagnetic.BulkDMI)
+    ccall((:mmx_add_dmi, libmmx), Cint,
+          (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 0, C_NULL,
+          C_NULL, C_NULL, t.Dx, t.Dy, t.Dz) |> check
+end
+function add_term!(h, t::MicroMagnetic.SpatialVectorBulkDMI)
+    ccall((:mmx_add_dmi, libmmx), Cint,
+          (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 0,
+          host(t.Dx), host(t.Dy), host(t.Dz), 0, 0, 0) |> check
+end
+function add_term!(h, t::MicroMagnetic.InterfacialDMI)
+    ccall((:mmx_add_dmi, libmmx), Cint,
+          (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 1,
+          host(t.D), C_NULL, C_NULL, 0, 0, 0) |> check
+end
+function add_term!(h, t::MicroMagnetic.Anisotropy)
+    ccall((:mmx_add_anis, libmmx), Cint,
+          (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, host(t.Ku), 0, C_NULL,
+          t.axis[1], t.axis[2], t.axis[3]) |> check
+end
+function add_term!(h, t::MicroMagnetic.SpatialAnisotropy)
+    ccall((:mmx_add_anis, libmmx), Cint,
+          (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, host(t.Ku), 0,
+          host(t.axes), 0, 0, 0) |> check
+end
+function add_term!(h, t::MicroMagnetic.CubicAnisotropy)
+    # the reference stores axis3 = axis1 x axis2 of the inputs; libmmx rebuilds it the same way
+    a1 = collect(Float64, t.axis1)
+    a2 = collect(Float64, t.axis2)
+    ccall((:mmx_add_cubic_anis, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}, Ptr{Float64}),
+          h, host(t.Kc), 0, a1, a2) |> check
+end
+function add_term!(h, t::MicroMagnetic.Zeeman)
+    ccall((:mmx_add_zeeman, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h,
+          host(t.field), 0, 0, 0) |> check
+end
+function add_term!(h, t::MicroMagnetic.Demag)
+    error("attach! needs the macro-PBC image counts of the demag term: call attach_demag!(sim, Nx, Ny, Nz)")
+end
+add_term!(h, t) = error("term $(typeof(t)) is not on the MI355X path")
+
+"""
+    set_integrator!(sim)
+
+Mirror create_driver (src/driver.jl:126-145): RungeKutta/Heun with their step,
+DormandPrince/BS23 with their tolerance.
+"""
+function set_integrator!(sim)
+    h = contexts[sim]
+    I = sim.driver.integrator
+    kind, step, tol = if I isa MicroMagnetic.RungeKutta
+        0, I.step, 0.0
+    elseif I isa MicroMagnetic.ModifiedEuler
+        1, I.step, 0.0
+    elseif I isa MicroMagnetic.AdaptiveRK && I.method isa MicroMagnetic.DOPRI54Method
+        2, 0.0, I.tol
+    elseif I isa MicroMagnetic.AdaptiveRK && I.method isa MicroMagnetic.BS23Method
+        3, 0.0, I.tol
+    else
+        error("integrator $(typeof(I)) is not on the MI355X path")
+    end
+    check(ccall((:mmx_set_integrator, libmmx), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cdouble), h, kind, step, tol))
+end
+
+function pull_spin!(sim)
+    m = Vector{Float64}(undef, 3 * sim.n_total)
+    check(ccall((:mmx_get_m, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}), contexts[sim], m))
+    copyto!(sim.spin, m)
+end
+
+function push_spin!(sim, spin)
+    check(ccall((:mmx_init_m0, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), contexts[sim], host(spin), 0))
+end
+
+# effective_field(sim, spin, t): src/micro/field.jl:496-505
+function effective_field(sim::MicroMagnetic.MicroSim, spin, t::Float64=0.0)
+    haskey(contexts, sim) || return invoke(effective_field, Tuple{MicroMagnetic.AbstractSim,Any,Float64}, sim, spin, t)
+    push_spin!(sim, spin)
+    H = Vector{Float64}(undef, 3 * sim.n_total)
+    check(ccall((:mmx_effective_field, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Ptr{Float64}), contexts[sim], t, H))
+    copyto!(sim.field, H)
+    return nothing
+end
+
+# llg_call_back(sim, dm_dt, spin, t): src/llg/llg.jl:45-58
+function llg_call_back(sim::MicroMagnetic.MicroSim, dm_dt::AbstractArray{T,1}, spin::AbstractArray{T,1},
+                       t::Float64) where {T<:AbstractFloat}
+    haskey(contexts, sim) || return invoke(llg_call_back, Tuple{MicroMagnetic.AbstractSim,AbstractArray{T,1},
+                                                                AbstractArray{T,1},Float64}, sim, dm_dt, spin, t)
+    out = Vector{Float64}(undef, 3 * sim.n_total)
+    check(ccall((:mmx_llg_rhs, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Ptr{Float64}, Ptr{Float64}), contexts[sim], t,
+                host(spin), out))
+    copyto!(dm_dt, out)
+    return nothing
+end
+
+# advance_step for the fixed-step and adaptive integrators (rk.jl:23, heun.jl:28, adaptive_rk.jl:93)
+for IT in (:RungeKutta, :ModifiedEuler, :AdaptiveRK)
+    @eval function advance_step(sim::MicroMagnetic.MicroSim, integrator::MicroMagnetic.$IT)
+        haskey(contexts, sim) || return invoke(advance_step, Tuple{MicroMagnetic.AbstractSim,MicroMagnetic.$IT},
+                                               sim, integrator)
+        ok = check(ccall((:mmx_advance_step, libmmx), Cint, (Ptr{Cvoid}, Int64), contexts[sim], 1))
+        t = Ref{Float64}(0.0)
+        step = Ref{Float64}(0.0)
+        step_next = Ref{Float64}(0.0)
+        nsteps = Ref{Int64}(0)
+        nfe = Ref{Int64}(0)
+        ccall((:mmx_get_state, libmmx), Cint,
+              (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}, Ref{Float64}, Ref{Int64}, Ref{Int64}), contexts[sim], t,
+              step, step_next, nsteps, nfe)
+        integrator.t = t[]
+        integrator.nsteps = nsteps[]
+        if integrator isa MicroMagnetic.AdaptiveRK
+            integrator.step = step[]
+            integrator.step_next = step_next[]
+            integrator.nfevals = nfe[]
+        end
+        copyto!(sim.prespin, sim.spin)
+        pull_spin!(sim)
+        integrator isa MicroMagnetic.AdaptiveRK && copyto!(view(integrator.y_current, 1:(3 * sim.n_total)), sim.spin)
+        return ok
+    end
+end
+
+"""
+    attach_demag!(sim, Nx, Ny, Nz)
+
+The reference's Demag struct does not keep its image counts; pass the values
+given to add_demag(sim; Nx, Ny, Nz) here (in insertion order with the other
+terms: call attach! with the demag term removed, then this, or build the
+context term by term).
+"""
+function attach_demag!(sim, Nx::Integer=0, Ny::Integer=0, Nz::Integer=0)
+    check(ccall((:mmx_add_demag, libmmx), Cint, (Ptr{Cvoid}, Cint, Cint, Cint), contexts[sim], Nx, Ny, Nz))
+end
+
+function detach!(sim)
+    h = pop!(contexts, sim, C_NULL)
+    h == C_NULL || ccall((:mmx_destroy, libmmx), Cint, (Ptr{Cvoid},), h)
+    return nothing
+end
+
+end # module

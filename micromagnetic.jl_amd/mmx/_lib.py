@@ -9,7 +9,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(PKG_ROOT, "libmmx.so")
+LIB_PATH = os.environ.get("MMX_LIBRARY") or os.path.join(PKG_ROOT, "libmmx.so")
 REPO_ROOT = os.path.dirname(PKG_ROOT)
 HEADER = os.path.join(REPO_ROOT, "include", "mmx.h")
 
