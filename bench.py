@@ -137,6 +137,176 @@ def cpu_baseline(nx, ny, nz, m0, images, budget_s=30.0):
                       f"demag set-up {t_setup:.1f} s not timed), FP64, OpenMP"}
 
 
+def grf(rng, nx, ny, dx, corr, mean, rel):
+    """Seeded Gaussian random field on an nx x ny grid: white noise filtered with
+    a Gaussian of correlation length `corr`, scaled to unit variance, then
+    mean * (1 + rel * g)."""
+    g = rng.standard_normal((ny, nx))
+    kx = np.fft.fftfreq(nx, d=dx) * 2 * np.pi
+    ky = np.fft.fftfreq(ny, d=dx) * 2 * np.pi
+    K2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    g = np.real(np.fft.ifft2(np.fft.fft2(g) * np.exp(-K2 * corr * corr / 4)))
+    g = (g - g.mean()) / g.std()
+    return mean * (1 + rel * g)
+
+
+def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
+    """Batching form: stochastic-LLG ensemble members of uneven size (Heun,
+    300 K, alpha 0.1, seeded Philox noise) advanced by one launch per stage over
+    all members, against the same members advanced one Sim at a time."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for q in range(n_members):
+        nx, ny = (int(v) for v in rng.integers(32, 193, 2))
+        nz = int(rng.integers(4, 17))
+        specs.append(dict(nx=nx, ny=ny, nz=nz, Ms=rng.uniform(4e5, 1.2e6), A=rng.uniform(5e-12, 2e-11),
+                          D=rng.uniform(0, 4e-3), Ku=rng.uniform(0, 1e6), seed=int(rng.integers(1, 1 << 30))))
+
+    def make():
+        sims = []
+        for sp in specs:
+            mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=sp["nx"], ny=sp["ny"], nz=sp["nz"], pbc="xy")
+            s = mmx.Sim(mesh, integrator="Heun", precision=precision)
+            mmx.set_Ms(s, sp["Ms"])
+            mmx.init_m0(s, (0, 0, 1))
+            mmx.add_exch(s, sp["A"])
+            mmx.add_dmi(s, sp["D"], type="interfacial")
+            mmx.add_anis(s, sp["Ku"], axis=(0, 0, 1))
+            mmx.add_thermal_noise(s, 300.0, seed=sp["seed"])
+            mmx.set_driver(s, integrator="Heun", step=1e-14)
+            sims.append(s)
+        return sims
+
+    cells = sum(sp["nx"] * sp["ny"] * sp["nz"] for sp in specs)
+    L = mmx._lib.load()
+    sims = make()
+    b = mmx.Batch(sims)
+    b.advance(warmup)
+    t0 = time.perf_counter()
+    b.advance(steps)
+    tb = time.perf_counter() - t0
+    cpt, grid, items, tune = b.info()
+    batched_spins = [s.spin for s in sims[:4]]
+    del b, sims
+    sims = make()
+    for s in sims:
+        mmx.run_step(s, n=warmup)
+    for s in sims:
+        mmx._lib.check(L.mmx_synchronize(s._h))
+    t0 = time.perf_counter()
+    for s in sims:
+        mmx._lib.check(L.mmx_advance_step(s._h, steps))  # synchronous per member
+    tp = time.perf_counter() - t0
+    same = all(np.array_equal(a, s.spin) for a, s in zip(batched_spins, sims[:4]))
+    del sims
+    ev = cells * 2 * steps
+    return {"workload": f"{n_members} sLLG members, meshes 32..192 x 32..192 x 4..16 (seed {seed}), "
+                        f"{cells} cells, Heun dt 1e-14, 300 K",
+            "value": round(ev / tb / 1e6, 1), "unit": "Mcell-evals/s",
+            "plain_value": round(ev / tp / 1e6, 1), "speedup_vs_plain": round(tp / tb, 3),
+            "identical_to_plain": same, "tuned": {"cells_per_thread": cpt, "grid": grid, "work_items": items}}
+
+
+def bench_adjoint(mmx, precision, nsteps=2000, every=50, n_fd=8, seed=5005):
+    """Differentiation form on the 128x128x8 film: Ku(x,y) and D(x,y) Gaussian
+    random fields (means 6e5 J/m^3 and 2.5e-3 J/m^2, 20 % std, 10 nm
+    correlation), exchange + interfacial DMI + anisotropy + demag, RK4 dt 5e-14
+    for nsteps with a checkpoint every `every` steps; loss = mean over the
+    centred 32x32 window (all layers) of (m_z(T) - target)^2.  Gradients with
+    respect to Ku, D and m0 from the adjoint, checked against central finite
+    differences of the same GPU forward path on n_fd cells per quantity."""
+    nx, ny, nz, d = 128, 128, 8, 2e-9
+    N = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    Ku2 = grf(rng, nx, ny, d, 10e-9, 6e5, 0.2)
+    D2 = grf(np.random.default_rng(seed + 1), nx, ny, d, 10e-9, 2.5e-3, 0.2)
+    Ku = np.tile(Ku2.reshape(-1), nz)
+    D = np.tile(D2.reshape(-1), nz)
+    target = np.random.default_rng(seed + 2).uniform(-1, 1, N)
+    win = np.zeros((nz, ny, nx), dtype=bool)
+    win[:, ny // 2 - 16:ny // 2 + 16, nx // 2 - 16:nx // 2 + 16] = True
+    win = win.reshape(-1)
+    r = np.random.default_rng(seed + 4)
+    m0 = np.array([0.1, 0.1, 1.0]) + 0.3 * r.standard_normal((N, 3))
+    m0 = (m0 / np.linalg.norm(m0, axis=1, keepdims=True)).reshape(-1)
+
+    mesh = mmx.FDMesh(dx=d, dy=d, dz=d, nx=nx, ny=ny, nz=nz)
+    sim = mmx.Sim(mesh, integrator="RungeKutta", precision=precision)
+    mmx.set_Ms(sim, 5.8e5)
+    mmx.init_m0(sim, m0, norm=False)
+    mmx.add_exch(sim, 1.5e-11)
+    tD = mmx.add_dmi(sim, D, type="interfacial", name="D")
+    tK = mmx.add_anis(sim, Ku, axis=(0, 0, 1), name="Ku")
+    mmx.add_demag(sim)
+    mmx.set_driver(sim, integrator="RungeKutta", step=5e-14)
+    L = mmx._lib.load()
+
+    def loss(mT):
+        mz = mT.reshape(-1, 3)[:, 2]
+        return float(np.sum((mz[win] - target[win]) ** 2) / win.sum())
+
+    def reset(m=m0, Ku_=Ku, D_=D):
+        mmx.init_m0(sim, m, norm=False)
+        mmx.set_term_param(sim, tK, Ku_)
+        mmx.set_term_param(sim, tD, D_)
+        mmx.set_driver(sim, integrator="RungeKutta", step=5e-14)
+
+    # plain forward for reference
+    reset()
+    mmx._lib.check(L.mmx_synchronize(sim._h))
+    t0 = time.perf_counter()
+    mmx._lib.check(L.mmx_advance_step(sim._h, nsteps))
+    t_plain = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    mT, pullback = mmx.rk4_trajectory_rrule(sim, nsteps, every)
+    t_fwd = time.perf_counter() - t0
+    lam = np.zeros(3 * N)
+    lam.reshape(-1, 3)[win, 2] = 2 * (mT.reshape(-1, 3)[win, 2] - target[win]) / win.sum()
+    t0 = time.perf_counter()
+    g = pullback(lam)
+    t_bwd = time.perf_counter() - t0
+    out = {"workload": f"128x128x8 film, 2 nm, exch+iDMI(x,y)+Ku(x,y)+demag, RK4 dt 5e-14, {nsteps} steps, "
+                       f"checkpoint every {every}",
+           "loss": loss(mT), "forward_record_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4),
+           "plain_forward_s": round(t_plain, 4),
+           "value": round(N * 4 * nsteps / (t_fwd + t_bwd) / 1e6, 1), "unit": "Mcell-evals/s of the differentiated trajectory",
+           "plain_value": round(N * 4 * nsteps / t_plain / 1e6, 1)}
+    if n_fd:
+        pr = np.random.default_rng(seed + 3)
+        errs = {}
+        for key in ("Ku", "D", "m0"):
+            scale = np.abs(g[key]).max()
+            worst = 0.0
+            for i in pr.choice(3 * N if key == "m0" else N, n_fd, replace=False):
+                if key == "m0":
+                    eps = 1e-6
+                    p_, q_ = m0.copy(), m0.copy()
+                    p_[i] += eps
+                    q_[i] -= eps
+                    args = [dict(m=p_), dict(m=q_)]
+                else:
+                    base = Ku if key == "Ku" else D
+                    eps = 1e-4 * abs(base[i])
+                    p_, q_ = base.copy(), base.copy()
+                    p_[i] += eps
+                    q_[i] -= eps
+                    kw = "Ku_" if key == "Ku" else "D_"
+                    args = [{kw: p_}, {kw: q_}]
+                vals = []
+                for a in args:
+                    reset(**a)
+                    mmx._lib.check(L.mmx_advance_step(sim._h, nsteps))
+                    vals.append(loss(sim.spin))
+                fd = (vals[0] - vals[1]) / (2 * eps)
+                worst = max(worst, abs(fd - g[key][i]) / scale)
+            errs[key] = worst
+        out["fd_cells_per_quantity"] = n_fd
+        out["fd_max_rel_err"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+    out["grads"] = g
+    return out
+
+
 def read_traffic(kernel_pass):
     """HBM bytes per launch of a pass from the committed PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
@@ -158,6 +328,8 @@ def main():
     ap.add_argument("--images", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batching and differentiation forms")
+    ap.add_argument("--fd-cells", type=int, default=8, help="finite-difference probes per gradient quantity")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -232,6 +404,25 @@ def main():
         log(f"[bench] FP32: {v32:.1f} Mcell-evals/s")
         del sim32
 
+    forms = None
+    if not args.no_forms:
+        log("[bench] batching form (FP64, FP32)")
+        b64 = bench_batch(mmx, mmx.F64, steps=100, warmup=5)
+        log(f"[bench] batch FP64: {b64['value']} Mcell-evals/s vs plain {b64['plain_value']}")
+        b32 = bench_batch(mmx, mmx.F32, steps=100, warmup=5)
+        log(f"[bench] batch FP32: {b32['value']} Mcell-evals/s vs plain {b32['plain_value']}")
+        log("[bench] differentiation form (FP64 with finite-difference probes, FP32)")
+        a64 = bench_adjoint(mmx, mmx.F64, n_fd=args.fd_cells)
+        log(f"[bench] adjoint FP64: {a64['value']} Mcell-evals/s (plain {a64['plain_value']}), "
+            f"fd error {a64.get('fd_max_rel_err')}")
+        a32 = bench_adjoint(mmx, mmx.F32, n_fd=0)
+        a32["grad_rel_err_vs_fp64"] = {k: float(f"{np.abs(a32['grads'][k] - a64['grads'][k]).max() / np.abs(a64['grads'][k]).max():.3e}")
+                                       for k in ("Ku", "D", "m0")}
+        log(f"[bench] adjoint FP32: {a32['value']} Mcell-evals/s, error vs FP64 {a32['grad_rel_err_vs_fp64']}")
+        for a in (a64, a32):
+            del a["grads"]
+        forms = {"batched": {"fp64": b64, "fp32": b32}, "adjoint": {"fp64": a64, "fp32": a32}}
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU oracle baseline")
@@ -261,6 +452,7 @@ def main():
         "passes": pass_report,
         "roofline": roofline,
         "fp32": fp32,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

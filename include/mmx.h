@@ -119,6 +119,70 @@ int mmx_relax(mmx_ctx *ctx, int64_t max_steps, double stopping_dmdt, int64_t *st
 int mmx_get_state(mmx_ctx *ctx, double *t, double *step, double *step_next, int64_t *nsteps,
                   int64_t *nfevals);
 
+/* Replace the per-cell scalar parameter of a term (N values): A of a spatial
+ * exchange, D of an interfacial DMI, Ku of an anisotropy, Kc of a cubic
+ * anisotropy, T of a thermal term.  A uniform term becomes spatial.  The
+ * reference changes these by rebuilding the term (add_anis(sim, Ku) again,
+ * src/micro/add_field.jl:430-465); this keeps the demag set-up. */
+int mmx_set_term_param(mmx_ctx *ctx, int term, const double *values);
+
+/* Differentiation form.  The reference's only derivative of this path is the
+ * Enzyme forward-mode Jacobian of gamma m x H_eff (ext/EnzymeExt.jl:6-59);
+ * these entry points give reverse-mode gradients of a loss on the state after
+ * an RK4 trajectory instead.
+ * mmx_adjoint_record: advance nsteps RK4 steps as mmx_advance_step does,
+ * keeping a checkpoint of the state every `every` steps (RK4 only). */
+int mmx_adjoint_record(mmx_ctx *ctx, int64_t nsteps, int64_t every);
+/* Pull the cotangent dL/dm(T) (3N AoS) of the recorded trajectory's final
+ * state back to dL/dm(0) (grad_m0, 3N AoS, nullable), recomputing each segment
+ * from its checkpoint; also forms the per-cell parameter gradients below. */
+int mmx_adjoint_backward(mmx_ctx *ctx, const double *lambda_T, double *grad_m0);
+/* dL/dp per cell (N values) of the last backward pass for the term's scalar
+ * parameter field: Ku (anisotropy), Kc (cubic), D (interfacial DMI), A
+ * (spatial exchange). */
+int mmx_adjoint_param_grad(mmx_ctx *ctx, int term, double *grad);
+
+/* Batching form.  Many independent systems (thermal-ensemble members,
+ * parameter sweeps) of any mesh sizes advanced together: every stage of a
+ * step is one launch over all members, and two steps are replayed from a HIP
+ * graph.  The reference runs such systems one Sim at a time (one
+ * run_sim / advance_step per Sim, src/sim.jl:813-857); each member keeps its
+ * own terms, parameters, state and counters, exactly as if advanced alone.
+ * Members: same precision and device, no demag term, all RungeKutta or all
+ * Heun with a step set.  The members stay usable on their own. */
+typedef struct mmx_batch mmx_batch;
+int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n);
+int mmx_batch_destroy(mmx_batch *batch);
+/* advance_step(sim) nsteps times for every member */
+int mmx_batch_advance(mmx_batch *batch, int64_t nsteps);
+/* the launch shape the set-up picked by timing (cells per thread and grid of
+ * the stage launch, number of work items) and the candidates' times in ms */
+int mmx_batch_info(mmx_batch *batch, int32_t *cells_per_thread, int32_t *grid, int32_t *work_items,
+                   float *tune_ms, int32_t n_tune);
+
+/* Nudged elastic band with the LLG driver: NEB(sim, init_images,
+ * frames_between_images; spring_constant, clib_image), src/neb/neb.jl:84-153.
+ * images: (n_free + 2) x 3N AoS -- the fixed left image, the n_free free
+ * images (already interpolated, e.g. by slerp as init_images does) and the
+ * fixed right image.  The field and energy of each image come from `sim` (its
+ * terms, demag included); sim must outlive the band.  Defaults as the
+ * reference sets them: BS23, tol 1e-5, alpha 0.2, gamma 2.21e5, precession
+ * off.  climbing_image is 1-based among the free images, -1 for none. */
+typedef struct mmx_neb mmx_neb;
+int mmx_neb_create(mmx_neb **out, mmx_ctx *sim, const double *images, int n_free, double spring_constant,
+                   int climbing_image);
+int mmx_neb_destroy(mmx_neb *neb);
+/* neb.driver.alpha / gamma / precession / tol */
+int mmx_neb_set_driver(mmx_neb *neb, double alpha, double gamma, int precession, double tol);
+/* advance_step(neb, integrator) n times (adaptive_rk.jl:93-148) */
+int mmx_neb_advance(mmx_neb *neb, int64_t nsteps);
+/* relax(neb; max_steps, stopping_dmdt), src/neb/neb.jl:231-296 (no file output) */
+int mmx_neb_relax(mmx_neb *neb, int64_t max_steps, double stopping_dmdt, int64_t *steps, double *max_dmdt);
+/* free images (n_free x 3N AoS), energies E_total of all n_free+2 images (J),
+ * the n_free+1 geodesic distances, and {t, step, step_next, nsteps, nfevals};
+ * each output nullable */
+int mmx_neb_get(mmx_neb *neb, double *images, double *energies, double *distances, double *state);
+
 /* diagnostics: the compact FP64 real-space tensor of a demag term (6N values,
  * xx,yy,zz,xy,xz,yz), the padded FFT geometry and per-pass kernel timers */
 int mmx_demag_tensor(mmx_ctx *ctx, int term, double *out);

@@ -118,6 +118,19 @@ struct mmx_ctx {
     bool profile = false;
 };
 
+struct mmx_neb {
+    mmx_ctx *sim = nullptr;
+    std::unique_ptr<CtxBase> impl;
+};
+
+struct mmx_batch {
+    std::vector<mmx_ctx *> members;
+    int precision = MMX_F64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<CtxBase> impl;
+};
+
 namespace {
 
 template <class T>
@@ -128,6 +141,7 @@ struct Engine : CtxBase {
     DevBuf<T> mu0Ms;
     DevBuf<uint8_t> pins;
     bool have_pins = false;
+    DevBuf<uint8_t> nbm;  // neighbour / pin mask, refreshed when Ms or the pins change
     // state: current spin, three stage buffers, seven k buffers (SoA 3N each)
     DevBuf<T> ybuf[4];
     T *y = nullptr;
@@ -172,6 +186,16 @@ struct Engine : CtxBase {
         partial.alloc((size_t)max_local_blocks() + 64);
         red.alloc(64);
         dscratch.alloc(3 * N);
+        nbm.alloc(N);
+        update_mask();
+    }
+
+    void update_mask()
+    {
+        const FieldParams<T> *P = params_dev();
+        launch_nbmask<T>(N, P, nbm.p, C->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(C->stream));
     }
     ~Engine() override
     {
@@ -249,6 +273,7 @@ struct Engine : CtxBase {
         P.volume = (T)C->volume;
         P.mu0Ms = mu0Ms.p;
         P.pins = have_pins ? pins.p : nullptr;
+        P.nbm = nbm.p;
         P.nterms = (int)C->terms.size();
         for (int i = 0; i < P.nterms; i++) {
             const HostTerm &h = C->terms[i];
@@ -347,14 +372,15 @@ struct Engine : CtxBase {
     }
 
     // ---- one RHS evaluation with a fused epilogue; returns partial blocks --
-    int eval(const T *m, const Epi<T> &E)
+    int eval(const T *m, const Epi<T> &E, bool generic = false)
     {
         const FieldParams<T> *P = params_dev();
         const Dyn d = dyn();
+        const int fast = generic ? 0 : Plast.fast;
         int nb;
         if (!demag) {
             mark(MMX_PASS_LOCAL, true);
-            nb = launch_local<T>(N, P, d, E, m, Plast.fast, C->stream);
+            nb = launch_local<T>(N, P, d, E, m, fast, C->stream);
             mark(MMX_PASS_LOCAL, false);
         } else {
             const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -371,7 +397,7 @@ struct Engine : CtxBase {
             launch_coly<T>(G, Bbuf.p, Abuf.p, G.Py, G.Py, ny, ny, nz, 3, 1, tw.p, C->stream);
             mark(MMX_PASS_P4, false);
             mark(MMX_PASS_P5, true);
-            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, Plast.fast, tw.p, C->stream);
+            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, fast, tw.p, C->stream);
             mark(MMX_PASS_P5, false);
         }
         HIPCHECK(hipGetLastError());
@@ -521,18 +547,18 @@ struct Engine : CtxBase {
         other = t;
     }
 
-    // RungeKutta.advance_step: src/integrator/rk.jl:23-52
-    void rk4_step()
+    // RungeKutta.advance_step: src/integrator/rk.jl:23-52.  rk4_into writes the
+    // new state of y0 to out (out may be stg[1] but not stg[0] or y0).
+    void rk4_into(const T *y0, T *out, double h)
     {
         ensure_k();
-        const double h = C->step;
         T *a = stg[0], *b = stg[1];
         Epi<T> E = epi(EPI_STAGE);
-        E.y0 = y;
+        E.y0 = y0;
         E.kstore = k[0];
         E.cself = 0.5 * h;
         E.yout = a;
-        eval(y, E);
+        eval(y0, E);
         E.kstore = k[1];
         E.yout = b;
         eval(a, E);
@@ -541,16 +567,22 @@ struct Engine : CtxBase {
         E.yout = a;
         eval(b, E);
         Epi<T> F = epi(EPI_RK4_FINAL);
-        F.y0 = y;
+        F.y0 = y0;
         F.kp[0] = k[0];
         F.kp[1] = k[1];
         F.kp[2] = k[2];
         F.np = 3;
         F.cself = 1.0 / 6 * h;
-        F.yout = b;
+        F.yout = out;
         F.normalize = 1;
         eval(a, F);
-        swap_y(stg[1]);  // new state in b; the previous state stays in stg[1] (prespin)
+    }
+
+    void rk4_step()
+    {
+        const double h = C->step;
+        rk4_into(y, stg[1], h);
+        swap_y(stg[1]);  // new state in stg[1]'s buffer; the previous state stays in stg[1] (prespin)
         C->nfevals += 4;
         C->nsteps += 1;
         C->t = C->nsteps * h;
@@ -799,6 +831,149 @@ struct Engine : CtxBase {
         return MMX_OK;
     }
 
+
+    // ---- differentiation form: RK4 trajectory pull-back ----------------------
+    // Checkpoints every `every` steps; the backward pass recomputes each
+    // segment's states, then per step the four stages with their fields, and
+    // pulls the cotangent back through the final combination, the
+    // normalisation and each stage's RHS (EPI_ADJ pass).
+    struct AdjState {
+        bool recorded = false;
+        int64_t nsteps = 0, every = 0, step0 = 0;
+        double h = 0, t0 = 0;
+        DevBuf<T> ck, seg, work;
+        std::vector<std::unique_ptr<DevBuf<T>>> g;
+    } adj;
+
+    static bool has_param_grad(int kind)
+    {
+        return kind == K_ANIS || kind == K_ANIS_S || kind == K_CUBIC || kind == K_DMI_INT || kind == K_EXCH_S;
+    }
+
+    int adj_record(int64_t nsteps, int64_t every)
+    {
+        if (C->ikind != MMX_RK4) return fail(MMX_ESTATE, "the adjoint needs the RungeKutta (RK4) integrator");
+        if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
+        if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
+        const int64_t nck = (nsteps + every - 1) / every;
+        adj.recorded = false;
+        adj.ck.alloc((size_t)nck * 3 * N);
+        adj.step0 = C->nsteps;
+        adj.t0 = C->t;
+        adj.h = C->step;
+        for (int64_t s = 0; s < nsteps; s++) {
+            if (s % every == 0)
+                HIPCHECK(hipMemcpyAsync(adj.ck.p + (s / every) * 3 * N, y, 3 * N * sizeof(T), hipMemcpyDeviceToDevice,
+                                        C->stream));
+            rk4_step();
+        }
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        adj.nsteps = nsteps;
+        adj.every = every;
+        adj.recorded = true;
+        return MMX_OK;
+    }
+
+    int adj_backward(const double *lamT, double *grad_m0)
+    {
+        if (!adj.recorded) return fail(MMX_ESTATE, "no recorded trajectory (call mmx_adjoint_record first)");
+        const int64_t V = 3 * N;
+        const double h = adj.h;
+        // work vectors: lam, lam0, ut, Y2..Y4, h1..h4, k1..k4, a1..a4
+        adj.work.alloc((size_t)18 * V);
+        HIPCHECK(hipMemsetAsync(adj.work.p, 0, adj.work.n * sizeof(T), C->stream));
+        T *w = adj.work.p;
+        T *lam = w, *lam0 = w + V, *U = w + 2 * V;
+        T *Ys[3] = {w + 3 * V, w + 4 * V, w + 5 * V};
+        T *hs[4], *ks[4], *as[4];
+        for (int q = 0; q < 4; q++) {
+            hs[q] = w + (6 + q) * V;
+            ks[q] = w + (10 + q) * V;
+            as[q] = w + (14 + q) * V;
+        }
+        adj.seg.alloc((size_t)adj.every * V);
+        adj.g.clear();
+        adj.g.resize(C->terms.size());
+        for (size_t tt = 0; tt < C->terms.size(); tt++)
+            if (has_param_grad(C->terms[tt].kind)) {
+                adj.g[tt] = std::make_unique<DevBuf<T>>();
+                adj.g[tt]->alloc(N);
+                HIPCHECK(hipMemsetAsync(adj.g[tt]->p, 0, N * sizeof(T), C->stream));
+            }
+        upload_soa(lamT, lam);
+        const int64_t save_n = C->nsteps;
+        const double save_t = C->t;
+        const int64_t nseg = (adj.nsteps + adj.every - 1) / adj.every;
+        for (int64_t sg = nseg - 1; sg >= 0; sg--) {
+            const int64_t s0 = sg * adj.every;
+            const int64_t len = std::min(adj.every, adj.nsteps - s0);
+            T *seg = adj.seg.p;
+            HIPCHECK(hipMemcpyAsync(seg, adj.ck.p + sg * V, V * sizeof(T), hipMemcpyDeviceToDevice, C->stream));
+            for (int64_t n = 0; n + 1 < len; n++) {
+                C->nsteps = adj.step0 + s0 + n;
+                rk4_into(seg + n * V, seg + (n + 1) * V, h);
+            }
+            for (int64_t n = len - 1; n >= 0; n--) {
+                const T *y0 = seg + n * V;
+                C->nsteps = adj.step0 + s0 + n;
+                // stages with their fields
+                Epi<T> E = epi(EPI_STAGE);
+                E.y0 = y0;
+                E.cself = 0.5 * h;
+                E.kstore = ks[0];
+                E.out = hs[0];
+                E.yout = Ys[0];
+                eval(y0, E, true);
+                E.kstore = ks[1];
+                E.out = hs[1];
+                E.yout = Ys[1];
+                eval(Ys[0], E, true);
+                E.cself = h;
+                E.kstore = ks[2];
+                E.out = hs[2];
+                E.yout = Ys[2];
+                eval(Ys[1], E, true);
+                E.cself = 0;
+                E.kstore = ks[3];
+                E.out = hs[3];
+                E.yout = U;
+                eval(Ys[2], E, true);
+                launch_adj_final<T>(N, y0, ks, 1.0 / 6 * h, lam, lam0, as, C->stream);
+                HIPCHECK(hipGetLastError());
+                const T *Yst[4] = {y0, Ys[0], Ys[1], Ys[2]};
+                const FieldParams<T> *P = params_dev();
+                for (int s = 3; s >= 0; s--) {
+                    launch_adj_pre<T>(N, P, Yst[s], as[s], U, C->stream);
+                    HIPCHECK(hipGetLastError());
+                    Epi<T> A = epi(EPI_ADJ);
+                    A.adj.Y = Yst[s];
+                    A.adj.hY = hs[s];
+                    A.adj.a = as[s];
+                    A.adj.lam = lam0;
+                    A.adj.next = s > 0 ? as[s - 1] : nullptr;
+                    A.adj.cnext = s == 3 ? h : 0.5 * h;
+                    for (size_t tt = 0; tt < adj.g.size(); tt++) A.adj.g[tt] = adj.g[tt] ? adj.g[tt]->p : nullptr;
+                    eval(U, A, true);
+                }
+                std::swap(lam, lam0);
+            }
+        }
+        C->nsteps = save_n;
+        C->t = save_t;
+        if (grad_m0) download_aos(lam, grad_m0);
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        return MMX_OK;
+    }
+
+    int adj_param_grad(int term, double *out)
+    {
+        if (term < 0 || term >= (int)adj.g.size() || !adj.g[term])
+            return fail(MMX_EINVAL, "no parameter gradient for term %d (run mmx_adjoint_backward; supported: "
+                                    "anisotropy Ku, cubic Kc, interfacial DMI D, spatial exchange A)", term);
+        download_scalar(adj.g[term]->p, out, N);
+        return MMX_OK;
+    }
+
     double max_dm()
     {
         int nb = launch_maxdiff<T>(stg[1], y, N, partial.p, C->stream);
@@ -822,6 +997,461 @@ double Engine<T>::maxabs(const T *v)
     return reduce_scalar(g, true);
 }
 
+
+
+// ---------------------------------------------------------------------------
+// Nudged elastic band with the LLG driver (src/neb/neb.jl).  The band's free
+// images live image-major on the device; the field and energy of an image
+// come from the simulation's own passes (any terms, demag included).  As in
+// the reference, the field, energies and distances used by every stage of a
+// step are those of the accepted images (neb.jl:196-230 read neb.spin), while
+// tangents and the LLG right-hand side use the stage vector; they are
+// computed once per accepted step.  Integrators: BS23 (default) or DOPRI54
+// is not offered; BS23 as neb.jl:146-152 sets it (tol 1e-5, alpha 0.2,
+// precession off).
+// ---------------------------------------------------------------------------
+template <class T>
+struct NebEngine : CtxBase {
+    mmx_neb *Nb;
+    mmx_ctx *C;
+    Engine<T> &e;
+    int n;          // free images
+    int64_t N, V;   // cells, 3N
+    int nb;         // blocks per image of the band kernels
+    DevBuf<T> L, R, Y, Ynext, Ytmp, F, tan, ecell, prespin;
+    DevBuf<T> kb[4];
+    T *k[4];
+    DevBuf<double> energy, dist, coef, part;
+    DevBuf<FieldParams<T>> Pneb;  // the sim's block with the band's LLG parameters
+    double spring = 1e5;
+    int climb = -1;
+    double alpha = 0.2, gamma = 2.21e5;
+    int precession = 0;
+    double tol = 1e-5;
+    // AdaptiveRK state (adaptive_rk.jl:37-55)
+    double t = 0, step = 0, step_next = 0;
+    int64_t nsteps = 0, nfevals = 0;
+    bool succeed = false, have_k4 = false, fresh = false;
+
+    NebEngine(mmx_neb *nb_, mmx_ctx *c, int nfree)
+        : Nb(nb_), C(c), e(*static_cast<Engine<T> *>(c->impl.get())), n(nfree), N(c->N), V(3 * c->N)
+    {
+        nb = (int)std::min<int64_t>((N + BLOCK - 1) / BLOCK, 512);
+        L.alloc(V);
+        R.alloc(V);
+        for (DevBuf<T> *b : {&Y, &Ynext, &Ytmp, &F, &tan, &prespin}) b->alloc((size_t)n * V);
+        for (int i = 0; i < 4; i++) {
+            kb[i].alloc((size_t)n * V);
+            k[i] = kb[i].p;
+        }
+        ecell.alloc(N);
+        energy.alloc(n + 2);
+        dist.alloc(n + 1);
+        coef.alloc(2 * n);
+        part.alloc((size_t)std::max<int64_t>((int64_t)(n + 1) * nb * 2, max_local_blocks()) + 64);
+    }
+
+    hipStream_t st() const { return C->stream; }
+
+    // field and energy of one image (effective_field_energy(sim, m), field.jl:538-547)
+    void field_energy(const T *m, T *out, double *E)
+    {
+        Epi<T> x = e.epi(EPI_FIELD);
+        x.out = out ? out : Ytmp.p;  // the field of an end image is not kept
+        x.e_out = ecell.p;
+        e.eval(m, x);
+        int g = launch_sum<T>(ecell.p, N, part.p, st());
+        launch_reduce(part.p, g, 1, 1, 0, E, st());
+        HIPCHECK(hipGetLastError());
+    }
+
+    void set_images(const double *aos)  // (n+2) x 3N AoS: left, free images, right
+    {
+        e.upload_soa(aos, L.p);
+        for (int q = 0; q < n; q++) e.upload_soa(aos + (size_t)(q + 1) * V, Y.p + (size_t)q * V);
+        e.upload_soa(aos + (size_t)(n + 1) * V, R.p);
+        HIPCHECK(hipMemcpyAsync(prespin.p, Y.p, (size_t)n * V * sizeof(T), hipMemcpyDeviceToDevice, st()));
+        field_energy(L.p, nullptr, energy.p);
+        field_energy(R.p, nullptr, energy.p + n + 1);
+        HIPCHECK(hipStreamSynchronize(st()));
+        fresh = false;
+    }
+
+    // fields, energies and distances of the accepted images
+    void refresh()
+    {
+        if (fresh) return;
+        for (int q = 0; q < n; q++) field_energy(Y.p + (size_t)q * V, F.p + (size_t)q * V, energy.p + q + 1);
+        launch_neb_dist<T>(N, n, Y.p, L.p, R.p, nb, part.p, dist.p, st());
+        HIPCHECK(hipGetLastError());
+        fresh = true;
+    }
+
+    // one right-hand side at the stage vector Ys with the stage combination
+    void rhs(const T *Ys, int mode, const double c[4], double outer, T *kout, const T *y0, const T *a1,
+             const T *a2, const T *a3, T *yout)
+    {
+        const FieldParams<T> *P = Pneb.p;
+        launch_neb_tangent<T>(N, n, Ys, L.p, R.p, energy.p, F.p, tan.p, nb, part.p, dist.p, spring, climb,
+                              coef.p, st());
+        launch_neb_stage<T>(P, N, n, nb, Ys, F.p, tan.p, coef.p, mode, c, outer, kout, y0, a1, a2, a3, yout,
+                            part.p, st());
+        HIPCHECK(hipGetLastError());
+        nfevals += 1;
+    }
+
+    double reduce_max()
+    {
+        launch_reduce(part.p, n * nb, 1, 1, 1, e.red.p, st());
+        HIPCHECK(hipGetLastError());
+        double v = 0;
+        HIPCHECK(hipMemcpyAsync(&v, e.red.p, sizeof(double), hipMemcpyDeviceToHost, st()));
+        HIPCHECK(hipStreamSynchronize(st()));
+        return v;
+    }
+
+    // rk_step_bs23!: src/integrator/adaptive_rk_steps.jl:11-47
+    double bs23_try(double h, double t0)
+    {
+        const double z[4] = {0, 0, 0, 0};
+        if (succeed && t0 >= t && have_k4) {
+            std::swap(k[0], k[3]);  // k1 .= k4
+            launch_axpy<T>(Y.p, h * 0.5, k[0], Ytmp.p, (int64_t)n * V, st());
+        } else {
+            double c[4] = {h * 0.5, 0, 0, 0};
+            rhs(Y.p, 0, c, 0, k[0], Y.p, nullptr, nullptr, nullptr, Ytmp.p);
+        }
+        {
+            double c[4] = {h * 0.75, 0, 0, 0};
+            rhs(Ytmp.p, 0, c, 0, k[1], Y.p, nullptr, nullptr, nullptr, Ynext.p);
+        }
+        std::swap(Ytmp.p, Ynext.p);  // y_temp now holds y + 3/4 h k2
+        {
+            double c[4] = {2.0 / 9, 1.0 / 3, 4.0 / 9, 0};
+            rhs(Ytmp.p, 1, c, h, k[2], Y.p, k[0], k[1], nullptr, Ynext.p);
+        }
+        {
+            double c[4] = {-(5.0 / 72), 1.0 / 12, 1.0 / 9, -1.0 / 8};
+            rhs(Ynext.p, 2, c, 0, k[3], nullptr, k[0], k[1], k[2], nullptr);
+        }
+        (void)z;
+        return reduce_max() * h / tol;
+    }
+
+    double initial_step(double dt)  // compute_initial_step, adaptive_rk.jl:151-165 (BS23: order 3)
+    {
+        double abs_step = dt, tmp = dt;
+        const double c[4] = {0, 0, 0, 0};
+        rhs(Y.p, 3, c, 0, Ytmp.p, nullptr, nullptr, nullptr, nullptr, nullptr);
+        const double r = reduce_max() / (0.824 * std::pow(tol, 1.0 / 3));
+        if (abs_step * r > 0.001) tmp = 0.001 / r;
+        return std::min(abs_step, tmp);
+    }
+
+    // the LLG parameters of the band (neb.jl:146-152) in a parameter block of its own
+    void upload_params()
+    {
+        FieldParams<T> P = e.params();
+        P.alpha = (T)alpha;
+        P.gamma = (T)gamma;
+        P.precession = precession;
+        if (!Pneb.p) Pneb.alloc(1);
+        HIPCHECK(hipStreamSynchronize(st()));
+        HIPCHECK(hipMemcpy(Pneb.p, &P, sizeof P, hipMemcpyHostToDevice));
+    }
+
+    int advance_step()  // adaptive_rk.jl:93-148
+    {
+        refresh();
+        const double t0 = t;
+        if (step_next <= 0) step_next = initial_step(1e-12);
+        step = step_next;
+        int nstep = 1;
+        for (;;) {
+            const double err = bs23_try(step, t0);
+            if (std::isnan(err)) {
+                step_next = 1e-14;
+                return fail(MMX_ENAN, "NaN error detected, reducing step size");
+            }
+            nstep += 1;
+            if (nstep > 50) return fail(MMX_ETRIES, "Too many inner steps (%d), system may not converge!", nstep);
+            succeed = err <= 1.0;
+            if (succeed) {
+                std::swap(Y.p, Ynext.p);
+                have_k4 = true;
+                nsteps += 1;
+                t += step;
+                const double factor = 0.824 * std::pow(1.0 / err, 1.0 / 3);
+                step_next = step * std::min(5.0, std::max(0.2, factor));
+                fresh = false;
+                break;
+            }
+            const double factor = 0.824 * std::pow(1.0 / err, 1.0 / 2);
+            step = step * std::min(5.0, std::max(0.2, factor));
+        }
+        return MMX_OK;
+    }
+
+    double max_dm_vs_prespin()
+    {
+        double mx = 0;
+        for (int q = 0; q < n; q++) {
+            int g = launch_maxdiff<T>(prespin.p + (size_t)q * V, Y.p + (size_t)q * V, N, e.partial.p, st());
+            HIPCHECK(hipGetLastError());
+            mx = std::max(mx, e.reduce_scalar(g, true));
+        }
+        return mx;
+    }
+
+    // relax(::NEB) with the LLG driver: src/neb/neb.jl:231-296 (no output)
+    int relax(int64_t max_steps, double stopping_dmdt, int64_t *steps_out, double *max_dmdt_out)
+    {
+        const double dmdt_factor = (2 * M_PI / 360) * 1e9;
+        double md = 0;
+        int64_t i = 0;
+        for (i = 0; i <= max_steps; i++) {
+            int rc = advance_step();
+            if (rc != MMX_OK) return rc;
+            md = max_dm_vs_prespin() / step;
+            if (md < stopping_dmdt * dmdt_factor) break;
+        }
+        if (steps_out) *steps_out = std::min(i, max_steps);
+        if (max_dmdt_out) *max_dmdt_out = md;
+        return MMX_OK;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Batches: many independent systems (ensemble members, parameter sweeps) of
+// any sizes advanced together.  Each stage of a step is ONE launch over every
+// member (k_batch_local); two steps are captured in a HIP graph and replayed.
+// The state of member m alternates between its y buffer (A) and its stg[1]
+// buffer (B); an odd remainder step is copied back so the state always ends
+// in A.  The launch shape (cells per thread, grid) is picked at creation by
+// timing the candidates on the batch itself.
+// ---------------------------------------------------------------------------
+template <class T>
+struct BatchEngine : CtxBase {
+    mmx_batch *B;
+    std::vector<Engine<T> *> E;
+    int M = 0, kind = MMX_RK4, S = 4;
+    DevBuf<int2> items;
+    int nitems = 0;
+    DevBuf<const FieldParams<T> *> Ps;
+    DevBuf<BatchStage<T>> st;  // [parity][stage][member]
+    DevBuf<Dyn> dyns;
+    int V[4] = {0, 0, 0, 0};
+    int cpt = 1, grid = 2048;
+    std::vector<float> tune_ms;
+    std::vector<T *> bufA, bufB;  // the two state buffers of each member
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    explicit BatchEngine(mmx_batch *b) : B(b) {}
+    ~BatchEngine() override
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    static int stages(int k) { return k == MMX_HEUN ? 2 : 4; }
+
+    void build()
+    {
+        M = (int)B->members.size();
+        kind = B->members[0]->ikind;
+        S = stages(kind);
+        for (auto *c : B->members) E.push_back(static_cast<Engine<T> *>(c->impl.get()));
+        for (int m = 0; m < M; m++) {
+            bufA.push_back(E[m]->y);
+            bufB.push_back(E[m]->stg[1]);
+        }
+        std::vector<int2> it;
+        items_for(4, it);
+        std::vector<const FieldParams<T> *> ps(M);
+        bool fast = true;
+        for (int m = 0; m < M; m++) {
+            E[m]->ensure_k();
+            ps[m] = E[m]->params_dev();
+            fast = fast && E[m]->Plast.fast;
+        }
+        Ps.alloc(M);
+        HIPCHECK(hipMemcpy(Ps.p, ps.data(), M * sizeof(ps[0]), hipMemcpyHostToDevice));
+        // stage descriptors for both parities
+        std::vector<BatchStage<T>> h(2 * S * M);
+        for (int par = 0; par < 2; par++)
+            for (int m = 0; m < M; m++) {
+                Engine<T> &e = *E[m];
+                const double dt = e.C->step;
+                T *A = e.y, *Bb = e.stg[1];
+                const T *yin = par ? Bb : A;
+                T *out = par ? A : Bb;
+                T *a = e.stg[0];
+                BatchStage<T> *d = &h[(size_t)par * S * M];
+                if (kind == MMX_RK4) {  // src/integrator/rk.jl:23-52, as rk4_into
+                    Epi<T> x = e.epi(EPI_STAGE);
+                    x.y0 = yin;
+                    x.kstore = e.k[0];
+                    x.cself = 0.5 * dt;
+                    x.yout = a;
+                    d[0 * M + m] = {x, yin};
+                    x.kstore = e.k[1];
+                    x.yout = out;
+                    d[1 * M + m] = {x, a};
+                    x.kstore = e.k[2];
+                    x.cself = dt;
+                    x.yout = a;
+                    d[2 * M + m] = {x, out};
+                    Epi<T> f = e.epi(EPI_RK4_FINAL);
+                    f.y0 = yin;
+                    f.kp[0] = e.k[0];
+                    f.kp[1] = e.k[1];
+                    f.kp[2] = e.k[2];
+                    f.np = 3;
+                    f.cself = 1.0 / 6 * dt;
+                    f.yout = out;
+                    f.normalize = 1;
+                    d[3 * M + m] = {f, a};
+                } else {  // src/integrator/heun.jl:28-49, as heun_step
+                    Epi<T> x = e.epi(EPI_STAGE);
+                    x.y0 = yin;
+                    x.kstore = e.k[0];
+                    x.cself = dt;
+                    x.yout = a;
+                    d[0 * M + m] = {x, yin};
+                    Epi<T> f = e.epi(EPI_STAGE);
+                    f.y0 = yin;
+                    f.kp[0] = e.k[0];
+                    f.cp[0] = 0.5 * dt;
+                    f.np = 1;
+                    f.cself = 0.5 * dt;
+                    f.yout = out;
+                    f.normalize = 1;
+                    d[1 * M + m] = {f, a};
+                }
+            }
+        st.alloc(h.size());
+        HIPCHECK(hipMemcpy(st.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+        for (int s = 0; s < S; s++) {
+            const int mode = (kind == MMX_RK4 && s == 3) ? EPI_RK4_FINAL : EPI_STAGE;
+            V[s] = batch_variant(mode, fast ? 1 : 0);
+        }
+        dyns.alloc(M);
+        upload_dyn();
+        tune();
+    }
+
+    void items_for(int c, std::vector<int2> &it)
+    {
+        it.clear();
+        for (int m = 0; m < M; m++)
+            for (int64_t c0 = 0; c0 < E[m]->N; c0 += (int64_t)BLOCK * c) it.push_back(make_int2(m, (int)c0));
+        nitems = (int)it.size();
+        items.alloc(it.size());
+        HIPCHECK(hipMemcpy(items.p, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+
+    void upload_dyn()
+    {
+        std::vector<Dyn> d(M);
+        for (int m = 0; m < M; m++) d[m] = E[m]->dyn();
+        HIPCHECK(hipMemcpyAsync(dyns.p, d.data(), M * sizeof(Dyn), hipMemcpyHostToDevice, B->stream));
+    }
+
+    void stage(int par, int s)
+    {
+        launch_batch_local<T>(items.p, nitems, Ps.p, st.p + ((size_t)par * S + s) * M, dyns.p, V[s], cpt, grid,
+                              B->stream);
+    }
+
+    void step(int par)
+    {
+        for (int s = 0; s < S; s++) stage(par, s);
+        launch_batch_tick(dyns.p, M, B->stream);
+    }
+
+    // time the first stage (it writes only stage buffers) for each launch shape
+    void tune()
+    {
+        const int cands_c[3] = {1, 2, 4};
+        const int cands_g[4] = {1024, 2048, 4096, 8192};
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        float best = 1e30f;
+        int bc = 1, bg = 2048;
+        std::vector<int2> it;
+        tune_ms.clear();
+        for (int c : cands_c) {
+            items_for(c, it);
+            for (int g : cands_g) {
+                cpt = c;
+                grid = g;
+                stage(0, 0);
+                HIPCHECK(hipEventRecord(a, B->stream));
+                for (int r = 0; r < 3; r++) stage(0, 0);
+                HIPCHECK(hipEventRecord(b, B->stream));
+                HIPCHECK(hipEventSynchronize(b));
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, a, b));
+                tune_ms.push_back(ms / 3);
+                if (ms < best) {
+                    best = ms;
+                    bc = c;
+                    bg = g;
+                }
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        cpt = bc;
+        grid = bg;
+        items_for(cpt, it);
+    }
+
+    void capture()
+    {
+        if (gexec) return;
+        HIPCHECK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        step(0);
+        step(1);
+        hipError_t e = hipStreamEndCapture(B->stream, &graph);
+        HIPCHECK(e);
+        HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
+
+    int advance(int64_t n)
+    {
+        for (auto *c : B->members) HIPCHECK(hipStreamSynchronize(c->stream));
+        for (int m = 0; m < M; m++) {
+            Engine<T> &e = *E[m];
+            e.params_dev();
+            if (e.y != bufA[m]) {  // the member stepped on its own since: its state is in B
+                HIPCHECK(hipMemcpyAsync(bufA[m], e.y, 3 * e.N * sizeof(T), hipMemcpyDeviceToDevice, B->stream));
+                e.y = bufA[m];
+                e.stg[1] = bufB[m];
+            }
+        }
+        upload_dyn();
+        capture();
+        for (int64_t i = 0; i + 1 < n; i += 2) HIPCHECK(hipGraphLaunch(gexec, B->stream));
+        if (n & 1) {
+            step(0);
+            for (int m = 0; m < M; m++)
+                HIPCHECK(hipMemcpyAsync(E[m]->y, E[m]->stg[1], 3 * E[m]->N * sizeof(T), hipMemcpyDeviceToDevice,
+                                        B->stream));
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(B->stream));
+        for (auto *c : B->members) {
+            c->nsteps += n;
+            c->nfevals += (int64_t)S * n;
+            c->t = c->nsteps * c->step;
+        }
+        return MMX_OK;
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -834,6 +1464,35 @@ static int with_engine(mmx_ctx *c, F &&f)
     try {
         if (c->mesh.precision == MMX_F64) return f(*static_cast<Engine<double> *>(c->impl.get()));
         return f(*static_cast<Engine<float> *>(c->impl.get()));
+    } catch (const HipError &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return fail(MMX_ENOMEM, "host allocation failed");
+    }
+}
+
+template <class F>
+static int with_batch(mmx_batch *b, F &&f)
+{
+    if (!b) return fail(MMX_EINVAL, "null batch");
+    try {
+        if (b->precision == MMX_F64) return f(*static_cast<BatchEngine<double> *>(b->impl.get()));
+        return f(*static_cast<BatchEngine<float> *>(b->impl.get()));
+    } catch (const HipError &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return fail(MMX_ENOMEM, "host allocation failed");
+    }
+}
+
+
+template <class F>
+static int with_neb(mmx_neb *b, F &&f)
+{
+    if (!b || !b->sim) return fail(MMX_EINVAL, "null band");
+    try {
+        if (b->sim->mesh.precision == MMX_F64) return f(*static_cast<NebEngine<double> *>(b->impl.get()));
+        return f(*static_cast<NebEngine<float> *>(b->impl.get()));
     } catch (const HipError &e) {
         return e.code;
     } catch (const std::bad_alloc &) {
@@ -920,6 +1579,7 @@ int mmx_set_Ms(mmx_ctx *c, const double *Ms)
     return with_engine(c, [&](auto &e) {
         for (int64_t i = 0; i < c->N; i++) c->mu0Ms[i] = Ms[i] * MU0;
         e.upload_scalar(c->mu0Ms.data(), e.mu0Ms.p, c->N);
+        e.update_mask();
         return MMX_OK;
     });
 }
@@ -932,6 +1592,7 @@ int mmx_set_pinning(mmx_ctx *c, const uint8_t *pins)
         HIPCHECK(hipMemcpyAsync(e.pins.p, pins, c->N, hipMemcpyHostToDevice, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
         e.have_pins = true;
+        e.update_mask();
         return MMX_OK;
     });
 }
@@ -1402,6 +2063,225 @@ int mmx_profile_read(mmx_ctx *c, double *ms, int64_t *launches)
         for (int i = 0; i < MMX_NPASS; i++) {
             if (ms) ms[i] = e.prof_ms[i];
             if (launches) launches[i] = e.prof_n[i];
+        }
+        return MMX_OK;
+    });
+}
+
+int mmx_set_term_param(mmx_ctx *c, int term, const double *values)
+{
+    if (!c || !values) return fail(MMX_EINVAL, "null argument");
+    if (term < 0 || term >= (int)c->terms.size()) return fail(MMX_EINVAL, "no term %d", term);
+    const int kind = c->terms[term].kind;
+    if (kind != K_EXCH_S && kind != K_DMI_INT && kind != K_ANIS && kind != K_ANIS_S && kind != K_CUBIC &&
+        kind != K_THERMAL)
+        return fail(MMX_EINVAL, "term %d has no scalar parameter field", term);
+    if (!finite_all(values, c->N)) return fail(MMX_EINVAL, "NaN in the parameter values");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm &h = c->terms[term];
+        if (h.uniform || !e.tA[term]) {
+            e.tA[term] = e.new_array(values, c->N, false);
+            h.uniform = 0;
+        } else {
+            e.upload_scalar(values, const_cast<TT *>(e.tA[term]), c->N);
+        }
+        return MMX_OK;
+    });
+}
+
+int mmx_adjoint_record(mmx_ctx *c, int64_t nsteps, int64_t every)
+{
+    return with_engine(c, [&](auto &e) { return e.adj_record(nsteps, every); });
+}
+
+int mmx_adjoint_backward(mmx_ctx *c, const double *lambda_T, double *grad_m0)
+{
+    if (!lambda_T) return fail(MMX_EINVAL, "null cotangent");
+    if (c && !finite_all(lambda_T, 3 * c->N)) return fail(MMX_EINVAL, "NaN in the cotangent");
+    return with_engine(c, [&](auto &e) { return e.adj_backward(lambda_T, grad_m0); });
+}
+
+int mmx_adjoint_param_grad(mmx_ctx *c, int term, double *grad)
+{
+    if (!grad) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) { return e.adj_param_grad(term, grad); });
+}
+
+int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n)
+{
+    if (!out || !members || n < 1) return fail(MMX_EINVAL, "need at least one member");
+    *out = nullptr;
+    const int prec = members[0] ? members[0]->mesh.precision : 0;
+    for (int i = 0; i < n; i++) {
+        mmx_ctx *c = members[i];
+        if (!c) return fail(MMX_EINVAL, "member %d is null", i);
+        if (c->mesh.precision != prec || c->mesh.device != members[0]->mesh.device)
+            return fail(MMX_EINVAL, "members must share precision and device");
+        if (c->demag_term >= 0) return fail(MMX_EINVAL, "member %d has a demag term; batches take local terms only", i);
+        if (c->ikind != MMX_RK4 && c->ikind != MMX_HEUN)
+            return fail(MMX_EINVAL, "member %d: batches run the fixed-step RungeKutta or Heun integrators", i);
+        if (c->ikind != members[0]->ikind) return fail(MMX_EINVAL, "members must use the same integrator");
+        if (!(c->step > 0)) return fail(MMX_EINVAL, "member %d has no step size", i);
+        for (int j = 0; j < i; j++)
+            if (members[j] == c) return fail(MMX_EINVAL, "member %d appears twice", i);
+    }
+    auto b = std::make_unique<mmx_batch>();
+    b->members.assign(members, members + n);
+    b->precision = prec;
+    b->device = members[0]->mesh.device;
+    try {
+        HIPCHECK(hipSetDevice(b->device));
+        HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        for (auto *c : b->members) HIPCHECK(hipStreamSynchronize(c->stream));
+        if (prec == MMX_F64) {
+            auto *e = new BatchEngine<double>(b.get());
+            b->impl.reset(e);
+            e->build();
+        } else {
+            auto *e = new BatchEngine<float>(b.get());
+            b->impl.reset(e);
+            e->build();
+        }
+        HIPCHECK(hipStreamSynchronize(b->stream));
+    } catch (const HipError &e) {
+        b->impl.reset();
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        b->impl.reset();
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        return fail(MMX_ENOMEM, "host allocation failed");
+    }
+    *out = b.release();
+    return MMX_OK;
+}
+
+int mmx_batch_destroy(mmx_batch *b)
+{
+    if (!b) return MMX_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->impl.reset();
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return MMX_OK;
+}
+
+int mmx_batch_advance(mmx_batch *b, int64_t nsteps)
+{
+    if (nsteps < 0) return fail(MMX_EINVAL, "nsteps must be >= 0");
+    return with_batch(b, [&](auto &e) { return e.advance(nsteps); });
+}
+
+int mmx_batch_info(mmx_batch *b, int32_t *cells_per_thread, int32_t *grid, int32_t *work_items, float *tune_ms,
+                   int32_t n_tune)
+{
+    return with_batch(b, [&](auto &e) {
+        if (cells_per_thread) *cells_per_thread = e.cpt;
+        if (grid) *grid = e.grid;
+        if (work_items) *work_items = e.nitems;
+        if (tune_ms)
+            for (int i = 0; i < n_tune && i < (int)e.tune_ms.size(); i++) tune_ms[i] = e.tune_ms[i];
+        return MMX_OK;
+    });
+}
+
+int mmx_neb_create(mmx_neb **out, mmx_ctx *sim, const double *images, int n_free, double spring_constant,
+                   int climbing_image)
+{
+    if (!out || !sim || !images) return fail(MMX_EINVAL, "null argument");
+    *out = nullptr;
+    if (n_free < 1) return fail(MMX_EINVAL, "The number of free images should larger than 1 (typically 10~20).");
+    if (!finite_all(images, (int64_t)(n_free + 2) * 3 * sim->N)) return fail(MMX_EINVAL, "NaN in the images");
+    auto b = std::make_unique<mmx_neb>();
+    b->sim = sim;
+    try {
+        HIPCHECK(hipSetDevice(sim->mesh.device));
+        if (sim->mesh.precision == MMX_F64) {
+            auto *x = new NebEngine<double>(b.get(), sim, n_free);
+            b->impl.reset(x);
+            x->spring = spring_constant;
+            x->climb = climbing_image;
+            x->set_images(images);
+        } else {
+            auto *x = new NebEngine<float>(b.get(), sim, n_free);
+            b->impl.reset(x);
+            x->spring = spring_constant;
+            x->climb = climbing_image;
+            x->set_images(images);
+        }
+    } catch (const HipError &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return fail(MMX_ENOMEM, "host allocation failed");
+    }
+    *out = b.release();
+    return MMX_OK;
+}
+
+int mmx_neb_destroy(mmx_neb *b)
+{
+    if (!b) return MMX_OK;
+    if (b->sim && b->sim->stream) (void)hipStreamSynchronize(b->sim->stream);
+    b->impl.reset();
+    delete b;
+    return MMX_OK;
+}
+
+int mmx_neb_set_driver(mmx_neb *b, double alpha, double gamma, int precession, double tol)
+{
+    if (!(tol > 0)) return fail(MMX_EINVAL, "tol must be > 0");
+    return with_neb(b, [&](auto &x) {
+        x.alpha = alpha;
+        x.gamma = gamma;
+        x.precession = precession;
+        x.tol = tol;
+        x.step_next = 0;
+        x.succeed = false;
+        x.have_k4 = false;
+        return MMX_OK;
+    });
+}
+
+int mmx_neb_advance(mmx_neb *b, int64_t nsteps)
+{
+    return with_neb(b, [&](auto &x) {
+        x.upload_params();
+        for (int64_t i = 0; i < nsteps; i++) {
+            int rc = x.advance_step();
+            if (rc != MMX_OK) return rc;
+        }
+        HIPCHECK(hipStreamSynchronize(x.st()));
+        return MMX_OK;
+    });
+}
+
+int mmx_neb_relax(mmx_neb *b, int64_t max_steps, double stopping_dmdt, int64_t *steps, double *max_dmdt)
+{
+    return with_neb(b, [&](auto &x) {
+        x.upload_params();
+        return x.relax(max_steps, stopping_dmdt, steps, max_dmdt);
+    });
+}
+
+int mmx_neb_get(mmx_neb *b, double *images, double *energies, double *distances, double *state)
+{
+    return with_neb(b, [&](auto &x) {
+        x.refresh();
+        HIPCHECK(hipStreamSynchronize(x.st()));
+        if (images)
+            for (int q = 0; q < x.n; q++) x.e.download_aos(x.Y.p + (size_t)q * x.V, images + (size_t)q * x.V);
+        if (energies)
+            HIPCHECK(hipMemcpy(energies, x.energy.p, (x.n + 2) * sizeof(double), hipMemcpyDeviceToHost));
+        if (distances)
+            HIPCHECK(hipMemcpy(distances, x.dist.p, (x.n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+        if (state) {
+            state[0] = x.t;
+            state[1] = x.step;
+            state[2] = x.step_next;
+            state[3] = (double)x.nsteps;
+            state[4] = (double)x.nfevals;
         }
         return MMX_OK;
     });

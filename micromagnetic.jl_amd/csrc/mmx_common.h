@@ -47,6 +47,10 @@ struct FieldParams {
     T volume;
     const T *mu0Ms;
     const uint8_t *pins;
+    // per cell: bit q (q = 0..5, order -x,+x,-y,+y,-z,+z) set when neighbour q
+    // exists and has Ms > 0 (the stencils' "use this neighbour" test,
+    // src/micro/kernels.jl:291); bit 6 set when the cell is pinned
+    const uint8_t *nbm;
     int nterms;
     TermDev<T> t[MAX_TERMS];
     // LLG driver (src/llg/llg.jl:6-40)
@@ -75,11 +79,31 @@ struct Dyn {
 enum EpiMode : int {
     EPI_FIELD = 0,   // write H (3 planes) and, if e != null, per-cell energy of the masked terms
     EPI_RHS,         // write dm/dt
-    EPI_STAGE,       // k = dm/dt; optionally store k; y_out = y0 + sum cp_j kp_j + cself k
+    EPI_STAGE,       // k = dm/dt; optionally store k (kstore) and H (out); y_out = y0 + sum cp_j kp_j + cself k
     EPI_STAGE_SCALED,// y_out = y0 + outer * (sum cp_j kp_j + cself k)    (BS23 stage 3)
     EPI_RK4_FINAL,   // y_out = y0 + cself * (((k1 + 2 k2) + 2 k3) + k)  (src/integrator/rk.jl:46)
     EPI_ERR,         // store k; err = sum cp_j kp_j + cself k; block max of |err|/(1+|y_in|)
     EPI_ENERGY,      // per-term energy block sums (FP64)
+    EPI_ADJ,         // adjoint of one RHS evaluation (AdjIO below)
+};
+
+// Adjoint of k = f(Y) = F(Y, H(Y)) applied to a cotangent a (EPI_ADJ).  The
+// pass's input vector is ut = (dF/dh)^T a / mu0Ms (zero where Ms = 0 or the
+// cell is pinned); the epilogue forms
+//   mu = mu0Ms * H_lin(ut) + (dF/dm)^T a
+// (H_lin: the field's Jacobian at Y, which for the energy-derived terms is
+// diag(mu0Ms)^-1-symmetric, so J_H^T u = mu0Ms * J_H(u / mu0Ms)), adds mu to
+// lam and cnext * mu to next, and adds the terms' parameter gradients
+// u . dH/dp to g[term].
+template <class T>
+struct AdjIO {
+    const T *Y;   // state the RHS was evaluated at
+    const T *hY;  // total field at Y
+    const T *a;   // cotangent of k
+    T *lam;       // += mu
+    T *next;      // += cnext * mu (nullable)
+    double cnext;
+    T *g[MAX_TERMS];  // per-cell parameter gradient of each term (nullable)
 };
 
 template <class T>
@@ -98,6 +122,15 @@ struct Epi {
     T *yout;
     int normalize;
     double *partial;     // one slot per block (ERR: max; ENERGY: nterms slots per block)
+    AdjIO<T> adj;
+};
+
+// One stage of one member of a batch (many independent systems advanced by
+// the same launches): the member's epilogue and the vector the stage reads.
+template <class T>
+struct BatchStage {
+    Epi<T> E;
+    const T *m;
 };
 
 template <class T>

@@ -428,10 +428,11 @@ __device__ __forceinline__ bool local_term(const TermDev<T> &D, const FieldParam
 template <class T>
 __device__ __forceinline__ void cell_field_slots(const FieldParams<T> &P, const Dyn &dyn, int I, int i, int j, int k,
                                                  const T *__restrict__ m, T hdx, T hdy, T hdz, T &mx, T &my, T &mz,
-                                                 T &Hx, T &Hy, T &Hz)
+                                                 T &Hx, T &Hy, T &Hz, uint32_t &msk)
 {
 #pragma clang fp contract(off)
     const int N = (int)P.N;
+    msk = P.nbm[I];
     mx = m[I];
     my = m[N + I];
     mz = m[2 * N + I];
@@ -486,13 +487,11 @@ __device__ __forceinline__ void cell_field_slots(const FieldParams<T> &P, const 
             if (q == 3) { t = j + 1; if (P.py && t >= ny) t -= ny; id = (t < ny) ? I + (t - j) * nx : -1; }
             if (q == 4) { t = k - 1; if (P.pz && t < 0) t += nz; id = (t >= 0) ? I + (t - k) * sxy : -1; }
             if (q == 5) { t = k + 1; if (P.pz && t >= nz) t -= nz; id = (t < nz) ? I + (t - k) * sxy : -1; }
-            // all loads of this neighbour are issued together from a clamped
-            // index; whether it counts is decided afterwards (no load waits
-            // on another load's value)
+            // the loads of this neighbour come from a clamped index; the
+            // cell's mask decides afterwards whether it counts
             const int idc = id >= 0 ? id : I;
-            const T nMs = P.mu0Ms[idc];
             const T ax = m[idc], ay = m[N + idc], az = m[2 * N + idc];
-            if (id < 0 || !(nMs > T(0))) continue;
+            if (!((msk >> q) & 1u)) continue;
             if (ex_on) {  // kernels.jl:269-304 (uniform), :226-266 (spatial)
                 const T nab = (q < 2) ? nab0 : ((q < 4) ? nab1 : nab2);
                 if (!ex_sp) {
@@ -658,10 +657,11 @@ __device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn 
             for (int c = 0; c < 3; c++) kpv[q][c] = E.kp[q][c * N + I];
     }
     T mx, my, mz, Hx, Hy, Hz;
-    cell_field_slots(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz);
+    uint32_t msk;
+    cell_field_slots(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk);
     // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
     T kk[3];
-    if (P.pins && P.pins[I]) {
+    if ((msk >> 6) & 1u) {
         kk[0] = kk[1] = kk[2] = T(0);
     } else {
         const T alpha = P.alpha, a = P.llg_a;
@@ -755,6 +755,221 @@ __device__ __forceinline__ void llg_rhs(const FieldParams<T> &P, int64_t I, T mx
     dz = a * (f3 - h3 * alpha);
 }
 
+// Pull-backs through the LLG right-hand side of llg_rhs,
+//   F(m, h) = a (m x p - alpha p),  p = |m|^2 h - (m.h) m,  a = -gamma/(1+alpha^2):
+//   (dF/dh)^T l = -a Pi q,              Pi v = |m|^2 v - (m.v) m
+//   (dF/dm)^T l = a (p x l - Pm^T q),   Pm^T v = 2 m (h.v) - h (m.v) - (m.h) v
+// with q = m x l + alpha l; the m x terms drop when precession is off.
+template <class T>
+__device__ __forceinline__ void llg_q(const FieldParams<T> &P, T mx, T my, T mz, T lx, T ly, T lz, T &qx, T &qy,
+                                      T &qz)
+{
+    const T alpha = P.alpha;
+    qx = alpha * lx;
+    qy = alpha * ly;
+    qz = alpha * lz;
+    if (P.precession) {
+        qx += my * lz - mz * ly;
+        qy += mz * lx - mx * lz;
+        qz += mx * ly - my * lx;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void llg_vjp_h(const FieldParams<T> &P, T mx, T my, T mz, T lx, T ly, T lz, T &ux,
+                                          T &uy, T &uz)
+{
+    const T alpha = P.alpha;
+    const T a = -P.gamma / (T(1) + alpha * alpha);
+    T qx, qy, qz;
+    llg_q(P, mx, my, mz, lx, ly, lz, qx, qy, qz);
+    const T mm = mx * mx + my * my + mz * mz;
+    const T mq = mx * qx + my * qy + mz * qz;
+    ux = -a * (mm * qx - mq * mx);
+    uy = -a * (mm * qy - mq * my);
+    uz = -a * (mm * qz - mq * mz);
+}
+
+template <class T>
+__device__ __forceinline__ void llg_vjp_m(const FieldParams<T> &P, T mx, T my, T mz, T hx, T hy, T hz, T lx, T ly,
+                                          T lz, T &rx, T &ry, T &rz)
+{
+    const T alpha = P.alpha;
+    const T a = -P.gamma / (T(1) + alpha * alpha);
+    T qx, qy, qz;
+    llg_q(P, mx, my, mz, lx, ly, lz, qx, qy, qz);
+    const T mm = mx * mx + my * my + mz * mz;
+    const T mh = mx * hx + my * hy + mz * hz;
+    const T hq = hx * qx + hy * qy + hz * qz;
+    const T mq = mx * qx + my * qy + mz * qz;
+    rx = -(T(2) * mx * hq - hx * mq - mh * qx);
+    ry = -(T(2) * my * hq - hy * mq - mh * qy);
+    rz = -(T(2) * mz * hq - hz * mq - mh * qz);
+    if (P.precession) {
+        const T px = mm * hx - mh * mx, py = mm * hy - mh * my, pz = mm * hz - mh * mz;
+        rx += py * lz - pz * ly;
+        ry += pz * lx - px * lz;
+        rz += px * ly - py * lx;
+    }
+    rx *= a;
+    ry *= a;
+    rz *= a;
+}
+
+// e_q x v for the interfacial DMI directions e = (0,1,0), (0,-1,0), (-1,0,0),
+// (1,0,0) of the -x, +x, -y, +y neighbours (src/micro/kernels.jl:426-466)
+template <class T>
+__device__ __forceinline__ void idmi_cross(int q, T vx, T vy, T vz, T &wx, T &wy, T &wz)
+{
+    wx = wy = wz = T(0);
+    if (q == 0) { wx = vz; wz = -vx; }
+    if (q == 1) { wx = -vz; wz = vx; }
+    if (q == 2) { wy = vz; wz = -vy; }
+    if (q == 3) { wy = -vz; wz = vy; }
+}
+
+// EPI_ADJ for cell I (see AdjIO): ut is this pass's input vector.
+template <class T>
+__device__ __forceinline__ void adj_cell(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I, int i,
+                                         int j, int k, const T *__restrict__ ut, T hdx, T hdy, T hdz)
+{
+    const int64_t N = P.N;
+    const AdjIO<T> &A = E.adj;
+    // constant terms have no Jacobian; the cubic one is linearised at Y below
+    uint32_t mask = E.term_mask;
+    for (int tt = 0; tt < P.nterms; tt++) {
+        const int kd = P.t[tt].kind;
+        if (kd == K_ZEEMAN || kd == K_ZEEMAN_U || kd == K_THERMAL || kd == K_CUBIC) mask &= ~(1u << tt);
+    }
+    T Hx, Hy, Hz;
+    cell_field(P, dyn, mask, I, i, j, k, ut, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
+    const T Ms = P.mu0Ms[I];
+    const T Yx = A.Y[I], Yy = A.Y[N + I], Yz = A.Y[2 * N + I];
+    const T ux = ut[I], uy = ut[N + I], uz = ut[2 * N + I];
+    T mux = 0, muy = 0, muz = 0;
+    if (Ms > T(0)) {
+        for (int tt = 0; tt < P.nterms; tt++) {
+            const TermDev<T> &D = P.t[tt];
+            if (D.kind != K_CUBIC || !((E.term_mask >> tt) & 1u)) continue;
+            // J = (12 Kc / mu0Ms) sum_r (Y.a_r)^2 a_r a_r^T  (cubic_anisotropy_kernel!, kernels.jl:110-137)
+            const T Kc = D.uniform ? D.s[9] : D.a[I];
+            const T c = T(12) * Kc / Ms;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const T ax = D.s[3 * r], ay = D.s[3 * r + 1], az = D.s[3 * r + 2];
+                const T yp = Yx * ax + Yy * ay + Yz * az;
+                const T up = ux * ax + uy * ay + uz * az;
+                const T f = c * yp * yp * up;
+                Hx += f * ax;
+                Hy += f * ay;
+                Hz += f * az;
+            }
+        }
+        mux = Ms * Hx;
+        muy = Ms * Hy;
+        muz = Ms * Hz;
+    }
+    if (!(P.pins && P.pins[I])) {
+        T rx, ry, rz;
+        llg_vjp_m(P, Yx, Yy, Yz, A.hY[I], A.hY[N + I], A.hY[2 * N + I], A.a[I], A.a[N + I], A.a[2 * N + I], rx, ry,
+                  rz);
+        mux += rx;
+        muy += ry;
+        muz += rz;
+    }
+    A.lam[I] += mux;
+    A.lam[N + I] += muy;
+    A.lam[2 * N + I] += muz;
+    if (A.next) {
+        const T c = (T)A.cnext;
+        A.next[I] += c * mux;
+        A.next[N + I] += c * muy;
+        A.next[2 * N + I] += c * muz;
+    }
+    // parameter gradients: g_p += sum_i u_i . dH_i/dp, u = mu0Ms ut
+    if (!(Ms > T(0))) return;
+    int64_t nb[6];
+    bool nbd = false;
+    for (int tt = 0; tt < P.nterms; tt++) {
+        T *g = A.g[tt];
+        if (!g) continue;
+        const TermDev<T> &D = P.t[tt];
+        T gv = 0;
+        switch (D.kind) {
+        case K_ANIS:
+        case K_ANIS_S: {  // dH/dKu = (2/mu0Ms)(m.u)u
+            T ax, ay, az;
+            if (D.kind == K_ANIS) {
+                ax = D.s[0];
+                ay = D.s[1];
+                az = D.s[2];
+            } else {
+                ax = D.b[I];
+                ay = D.b[N + I];
+                az = D.b[2 * N + I];
+            }
+            gv = T(2) * (ux * ax + uy * ay + uz * az) * (Yx * ax + Yy * ay + Yz * az);
+        } break;
+        case K_CUBIC: {  // dH/dKc = (4/mu0Ms) sum_r (m.a_r)^3 a_r
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const T ax = D.s[3 * r], ay = D.s[3 * r + 1], az = D.s[3 * r + 2];
+                const T yp = Yx * ax + Yy * ay + Yz * az;
+                gv += T(4) * yp * yp * yp * (ux * ax + uy * ay + uz * az);
+            }
+        } break;
+        case K_DMI_INT: {  // D_eff = 2 D_p D_j / (D_p + D_j) enters row p and row j
+            const T DI = D.uniform ? D.s[3] : D.a[I];
+            if (DI == T(0)) break;
+            if (!nbd) {
+                neighbours(P, i, j, k, nb);
+                nbd = true;
+            }
+            const T Dd[4] = {D.s[0], D.s[0], D.s[1], D.s[1]};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t id = nb[q];
+                if (id < 0 || !(P.mu0Ms[id] > T(0))) continue;
+                const T Dn = D.uniform ? D.s[3] : D.a[id];
+                if (Dn == T(0)) continue;
+                const T s = DI + Dn;
+                const T dDe = T(2) * Dn * Dn / (s * s);
+                const T jx = A.Y[id], jy = A.Y[N + id], jz = A.Y[2 * N + id];
+                const T vx = ut[id], vy = ut[N + id], vz = ut[2 * N + id];
+                T wx, wy, wz, zx, zy, zz;
+                idmi_cross(q, jx, jy, jz, wx, wy, wz);  // e_q x Y_j
+                idmi_cross(q, Yx, Yy, Yz, zx, zy, zz);  // e_q x Y_p
+                const T t = (ux * wx + uy * wy + uz * wz) - (vx * zx + vy * zy + vz * zz);
+                gv += dDe * Dd[q] * t;
+            }
+        } break;
+        case K_EXCH_S: {  // A_eff = 2 A_p A_j / (A_p + A_j)
+            const T AI = D.a[I];
+            if (AI == T(0)) break;
+            if (!nbd) {
+                neighbours(P, i, j, k, nb);
+                nbd = true;
+            }
+            const T nab[6] = {D.s[0], D.s[0], D.s[1], D.s[1], D.s[2], D.s[2]};
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int64_t id = nb[q];
+                if (id < 0 || !(P.mu0Ms[id] > T(0))) continue;
+                const T An = D.a[id];
+                if (An == T(0)) continue;
+                const T s = AI + An;
+                const T dAe = T(2) * An * An / (s * s);
+                const T dx = A.Y[id] - Yx, dy = A.Y[N + id] - Yy, dz = A.Y[2 * N + id] - Yz;
+                const T t = (ux - ut[id]) * dx + (uy - ut[N + id]) * dy + (uz - ut[2 * N + id]) * dz;
+                gv += dAe * nab[q] * t;
+            }
+        } break;
+        default: break;
+        }
+        g[I] += gv;
+    }
+}
+
 // The fused last step of one RHS evaluation for cell I.  `red` accumulates the
 // thread's reduction value (ERR: running max; ENERGY: sum of the masked terms).
 template <class T>
@@ -765,6 +980,10 @@ __device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Dyn &
 #pragma clang fp contract(off)
     const int64_t N = P.N;
     T Hx, Hy, Hz;
+    if (E.mode == EPI_ADJ) {
+        adj_cell<T>(P, dyn, E, I, i, j, k, m, hdx, hdy, hdz);
+        return;
+    }
     if (E.mode == EPI_ENERGY) {
         double es = 0;
         cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, &es);
@@ -798,6 +1017,11 @@ __device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Dyn &
         E.kstore[I] = kx;
         E.kstore[N + I] = ky;
         E.kstore[2 * N + I] = kz;
+    }
+    if (E.out) {  // the field at this stage (adjoint recomputation)
+        E.out[I] = Hx;
+        E.out[N + I] = Hy;
+        E.out[2 * N + I] = Hz;
     }
     const T kk[3] = {kx, ky, kz};
     if (E.mode == EPI_ERR) {  // vector_add6b / BS23 error combination, then |err|/(1+|y|)

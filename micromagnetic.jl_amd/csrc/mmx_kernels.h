@@ -55,6 +55,41 @@ void launch_axpy(const T *y, double c, const T *k, T *out, int64_t n, hipStream_
 template <class T>
 int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s);
 
+// adjoint of the RK4 step: the EPI_ADJ input vector, and the pull-back of
+// the final combination + normalisation
+template <class T>
+void launch_adj_pre(int64_t N, const FieldParams<T> *P, const T *Y, const T *a, T *ut, hipStream_t s);
+template <class T>
+void launch_adj_final(int64_t N, const T *y0, const T *const k[4], double c, const T *lam, T *lam0, T *const a[4],
+                      hipStream_t s);
+
+// batches without demag: one launch per stage over every member; V is the
+// epilogue variant (batch_variant), cpt the cells per thread of a work item
+template <class T>
+void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *const *Ps, const BatchStage<T> *st,
+                        const Dyn *dyns, int V, int cpt, int grid, hipStream_t s);
+void launch_batch_tick(Dyn *dyns, int M, hipStream_t s);
+int batch_variant(int mode, int fast);
+
+// nudged elastic band (per-image reductions over grids of nb x images)
+template <class T>
+int launch_sum(const T *v, int64_t n, double *partial, hipStream_t s);
+template <class T>
+void launch_neb_dist(int64_t N, int n, const T *Y, const T *L, const T *R, int nb, double *partial, double *dist,
+                     hipStream_t s);
+template <class T>
+void launch_neb_tangent(int64_t N, int n, const T *Ys, const T *L, const T *R, const double *energy, const T *F,
+                        T *tan, int nb, double *partial, const double *dist, double k, int climb, double *coef,
+                        hipStream_t s);
+template <class T>
+void launch_neb_stage(const FieldParams<T> *P, int64_t N, int n, int nb, const T *Ys, const T *F, const T *tan,
+                      const double *coef, int mode, const double c[4], double outer, T *kstore, const T *y0,
+                      const T *k1, const T *k2, const T *k3, T *yout, double *partial, hipStream_t s);
+
+// per-cell neighbour / pin mask (FieldParams::nbm)
+template <class T>
+void launch_nbmask(int64_t N, const FieldParams<T> *P, uint8_t *out, hipStream_t s);
+
 int max_local_blocks();
 // number of blocks (reduction slot groups) the last demag pass launches
 int p5_blocks(const DemagGeom &G);

@@ -331,6 +331,357 @@ __global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restric
     if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
 
+
+
+// ---------------------------------------------------------------------------
+// Batches of systems without demag: one persistent launch per stage covers
+// every member.  Work item w = (member, first cell) spans BLOCK*CPT cells of
+// one member; blocks b with b % 8 == x (one XCD) take the contiguous item
+// range x*ceil(n/8) ... so that the y/z neighbour rows of a member's item
+// were fetched by the same XCD's L2 shortly before.
+// ---------------------------------------------------------------------------
+template <class T, int V, int CPT>
+__global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ items, int nitems,
+                                                        const FieldParams<T> *const *__restrict__ Ps,
+                                                        const BatchStage<T> *__restrict__ st,
+                                                        const Dyn *__restrict__ dyns)
+{
+    const int G = gridDim.x;
+    const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+    const int Gx = (G - xcd + 7) >> 3;
+    const int per = (nitems + 7) >> 3;
+    const int lo = xcd * per, hi = min(nitems, lo + per);
+    for (int it = lo + r; it < hi; it += Gx) {
+        const int2 w = items[it];
+        const FieldParams<T> &P = *Ps[w.x];
+        const BatchStage<T> &S = st[w.x];
+        const Dyn d = dyns[w.x];
+        const int nx = P.nx, ny = P.ny;
+        double red = 0;
+        for (int c = 0; c < CPT; c++) {
+            const int64_t I = (int64_t)w.y + c * BLOCK + threadIdx.x;
+            if (I < P.N) {
+                const int i = (int)(I % nx);
+                const int64_t q = I / nx;
+                const int j = (int)(q % ny);
+                const int k = (int)(q / ny);
+                epilogue<T, V>(P, d, S.E, I, i, j, k, S.m, T(0), T(0), T(0), red);
+            }
+        }
+    }
+}
+
+// the thermal noise of every member moves to its next integrator step
+__global__ void k_batch_tick(Dyn *__restrict__ dyns, int M)
+{
+    for (int m = blockIdx.x * BLOCK + threadIdx.x; m < M; m += gridDim.x * BLOCK) dyns[m].noise_step += 1;
+}
+
+
+// ---------------------------------------------------------------------------
+// Nudged elastic band (src/neb/neb.jl, neb_kernels.jl).  Images are stored
+// image-major, each as three planes of N values.  Grids are (blocks per
+// image, images); per-image block partials go to partial[(q*gridDim.x+b)*S+v].
+// ---------------------------------------------------------------------------
+// sum of n values (per-cell energies -> image energy)
+template <class T>
+__global__ void k_sum(const T *__restrict__ v, int64_t n, double *__restrict__ partial)
+{
+    double r[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        r[0] += (double)v[i];
+    block_reduce_store<1>(r, 1, false, partial, 1);
+}
+
+// compute_distance_kernel! (neb_kernels.jl:66-81): squared geodesic angles of
+// image pair q = (q-1, q) (q = 0: left end, q = n: right end), summed per block
+template <class T>
+__global__ void k_neb_dist(int64_t N, int n, const T *__restrict__ Y, const T *__restrict__ L,
+                           const T *__restrict__ R, double *__restrict__ partial)
+{
+    const int q = blockIdx.y;
+    const T *a = q == 0 ? L : Y + (int64_t)(q - 1) * 3 * N;
+    const T *b = q == n ? R : Y + (int64_t)q * 3 * N;
+    double r[1] = {0};
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        const T mx1 = a[I], my1 = a[N + I], mz1 = a[2 * N + I];
+        const T mx2 = b[I], my2 = b[N + I], mz2 = b[2 * N + I];
+        const T cx = -my2 * mz1 + my1 * mz2;
+        const T cy = mx2 * mz1 - mx1 * mz2;
+        const T cz = -mx2 * my1 + mx1 * my2;
+        const T mm = mx1 * mx2 + my1 * my2 + mz1 * mz2;
+        const T d = sqrt(cx * cx + cy * cy + cz * cz);
+        const double ang = (double)atan2(d, mm);
+        r[0] += ang * ang;
+    }
+    block_reduce_store<1>(r, 1, false, partial + (size_t)q * gridDim.x, 1);
+}
+
+// compute_tangents_kernel! + reduce_tangent_kernel! (neb_kernels.jl:84-109)
+// at the stage vector Ys; per-block partials of |t|^2 and f.t
+template <class T>
+__global__ void k_neb_tangent(int64_t N, int n, const T *__restrict__ Ys, const T *__restrict__ L,
+                              const T *__restrict__ R, const double *__restrict__ energy, const T *__restrict__ F,
+                              T *__restrict__ tan, double *__restrict__ partial)
+{
+#pragma clang fp contract(off)
+    const int q = blockIdx.y;
+    const int64_t V = 3 * N;
+    const double E1 = energy[q], E2 = energy[q + 1], E3 = energy[q + 2];
+    const double dEmax = fmax(fabs(E3 - E2), fabs(E2 - E1));
+    const double dEmin = fmin(fabs(E3 - E2), fabs(E2 - E1));
+    int br;
+    if (E1 > E2 && E2 > E3) br = 0;
+    else if (E3 > E2 && E2 > E1) br = 1;
+    else if (E3 > E1) br = 2;
+    else if (E3 < E1) br = 3;
+    else br = 4;
+    const T *m = Ys + q * V;
+    const T *mp = q == n - 1 ? R : Ys + (q + 1) * V;
+    const T *mm = q == 0 ? L : Ys + (q - 1) * V;
+    const T *f = F + q * V;
+    T *t = tan + q * V;
+    double r[2] = {0, 0};
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        T tv[3], mv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t o = c * N + I;
+            const T mk = m[o];
+            const T tip = mp[o] - mk;
+            const T tim = mk - mm[o];
+            T x;
+            if (br == 0) x = tim;
+            else if (br == 1) x = tip;
+            else if (br == 2) x = (T)dEmax * tip + (T)dEmin * tim;
+            else if (br == 3) x = (T)dEmin * tip + (T)dEmax * tim;
+            else x = tim + tip;
+            tv[c] = x;
+            mv[c] = mk;
+        }
+        const T d = tv[0] * mv[0] + tv[1] * mv[1] + tv[2] * mv[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t o = c * N + I;
+            tv[c] -= d * mv[c];
+            t[o] = tv[c];
+            r[0] += (double)tv[c] * (double)tv[c];
+            r[1] += (double)f[o] * (double)tv[c];
+        }
+    }
+    block_reduce_store<2>(r, 2, false, partial + (size_t)q * gridDim.x * 2, 2);
+}
+
+// per image: reduce the block partials; coef[q] = {1/|t|, s_q}, where the NEB
+// force is f + s_q t/|t| with s_q = k (d[q+1] - d[q]) - f.t/|t| (or -2 f.t/|t|
+// for the climbing image), src/neb/neb.jl:196-216
+__global__ void k_neb_coef(const double *__restrict__ partial, int nb, const double *__restrict__ dist, double k,
+                           int climb, double *__restrict__ coef)
+{
+    __shared__ double s0[BLOCK], s1[BLOCK];
+    const int q = blockIdx.x;
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nb; i += BLOCK) {
+        a += partial[((size_t)q * nb + i) * 2];
+        b += partial[((size_t)q * nb + i) * 2 + 1];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s0[threadIdx.x] += s0[threadIdx.x + h];
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double nrm = sqrt(s0[0]);
+        const double ft = s1[0] / nrm;
+        coef[2 * q] = nrm;
+        coef[2 * q + 1] = (q + 1 == climb) ? -2.0 * ft : k * (dist[q + 1] - dist[q]) - ft;
+    }
+}
+
+// per image pair: distance = sqrt(sum of block partials)
+__global__ void k_neb_dist_final(const double *__restrict__ partial, int nb, double *__restrict__ dist)
+{
+    __shared__ double s[BLOCK];
+    const int q = blockIdx.x;
+    double a = 0;
+    for (int i = threadIdx.x; i < nb; i += BLOCK) a += partial[(size_t)q * nb + i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dist[q] = sqrt(s[0]);
+}
+
+// NEB force + LLG (precession as set) at the stage vector, then the BS23
+// stage combination over the whole band (adaptive_rk_steps.jl:11-47):
+//   mode 0: kstore = k; yout = y0 + c0 k
+//   mode 1: kstore = k; yout = normalise(y0 + outer ((c0 k1 + c1 k2) + c2 k))
+//   mode 2: kstore = k; err = ((e0 k1 + e1 k2) + e2 k3) + e3 k; max |err|/(1+|Ys|)
+//   mode 3: kstore = k; max |k|
+struct NebStage {
+    int mode;
+    double c[4];
+    double outer;
+};
+
+template <class T>
+__global__ void k_neb_stage(const FieldParams<T> *__restrict__ Pp, int64_t N, const T *__restrict__ Ys,
+                            const T *__restrict__ F, const T *__restrict__ tan, const double *__restrict__ coef,
+                            NebStage S, T *__restrict__ kstore, const T *__restrict__ y0,
+                            const T *__restrict__ k1, const T *__restrict__ k2, const T *__restrict__ k3,
+                            T *__restrict__ yout, double *__restrict__ partial)
+{
+#pragma clang fp contract(off)
+    const FieldParams<T> &P = *Pp;
+    const int q = blockIdx.y;
+    const int64_t V = 3 * N, base = q * V;
+    const double nrm = coef[2 * q], sq = coef[2 * q + 1];
+    double red = 0;
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        T mv[3], hv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t o = base + c * N + I;
+            const T th = tan[o] / (T)nrm;
+            hv[c] = F[o] + (T)sq * th;
+            mv[c] = Ys[o];
+        }
+        T kk[3];
+        llg_rhs(P, I, mv[0], mv[1], mv[2], hv[0], hv[1], hv[2], kk[0], kk[1], kk[2]);
+        if (S.mode == 1) {
+            T y[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int64_t o = base + c * N + I;
+                const T s = ((T)S.c[0] * k1[o] + (T)S.c[1] * k2[o]) + (T)S.c[2] * kk[c];
+                y[c] = y0[o] + (T)S.outer * s;
+            }
+            const T m2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+            if (m2 > T(0)) {
+                const T len = T(1) / sqrt(m2);
+                y[0] *= len;
+                y[1] *= len;
+                y[2] *= len;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) yout[base + c * N + I] = y[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t o = base + c * N + I;
+            kstore[o] = kk[c];
+            if (S.mode == 0) {
+                yout[o] = y0[o] + (T)S.c[0] * kk[c];
+            } else if (S.mode == 2) {
+                const double e = ((S.c[0] * (double)k1[o] + S.c[1] * (double)k2[o]) + S.c[2] * (double)k3[o]) +
+                                 S.c[3] * (double)kk[c];
+                red = pmax(red, fabs(e / (1.0 + fabs((double)mv[c]))));
+            } else if (S.mode == 3) {
+                red = pmax(red, fabs((double)kk[c]));
+            }
+        }
+    }
+    if (S.mode >= 2) {
+        double r[1] = {red};
+        block_reduce_store<1>(r, 1, true, partial + (size_t)q * gridDim.x, 1);
+    }
+}
+
+
+// neighbour / pin mask of every cell (FieldParams::nbm), from the same rules
+// as neighbours() and the Ms > 0 test of the stencil kernels
+template <class T>
+__global__ void k_nbmask(const FieldParams<T> *__restrict__ Pp, uint8_t *__restrict__ out)
+{
+    const FieldParams<T> &P = *Pp;
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < P.N; I += (int64_t)gridDim.x * BLOCK) {
+        const int i = (int)(I % P.nx);
+        const int64_t q = I / P.nx;
+        const int j = (int)(q % P.ny);
+        const int k = (int)(q / P.ny);
+        int64_t nb[6];
+        neighbours(P, i, j, k, nb);
+        uint32_t msk = 0;
+#pragma unroll
+        for (int d = 0; d < 6; d++)
+            if (nb[d] >= 0 && P.mu0Ms[nb[d]] > T(0)) msk |= 1u << d;
+        if (P.pins && P.pins[I]) msk |= 1u << 6;
+        out[I] = (uint8_t)msk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint helpers (RK4 step pull-back)
+// ---------------------------------------------------------------------------
+// ut = (dF/dh)^T a / mu0Ms at state Y (zero where Ms = 0 or pinned): the
+// input vector of the EPI_ADJ pass.
+template <class T>
+__global__ void k_adj_pre(const FieldParams<T> *__restrict__ Pp, const T *__restrict__ Y, const T *__restrict__ a,
+                          T *__restrict__ ut)
+{
+    const FieldParams<T> &P = *Pp;
+    const int64_t N = P.N;
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        const T Ms = P.mu0Ms[I];
+        T ux = 0, uy = 0, uz = 0;
+        if (Ms > T(0) && !(P.pins && P.pins[I])) {
+            llg_vjp_h(P, Y[I], Y[N + I], Y[2 * N + I], a[I], a[N + I], a[2 * N + I], ux, uy, uz);
+            ux /= Ms;
+            uy /= Ms;
+            uz /= Ms;
+        }
+        ut[I] = ux;
+        ut[N + I] = uy;
+        ut[2 * N + I] = uz;
+    }
+}
+
+// Pull-back of y1 = normalise(y0 + c (((k1 + 2 k2) + 2 k3) + k4)) (rk.jl:46,
+// helper.jl:255-271): lam_z = (lam - (lam.y1) y1) / |z|; writes lam0 = lam_z
+// and the stage cotangents a1 = a4 = c lam_z, a2 = a3 = 2 c lam_z.
+template <class T>
+__global__ void k_adj_final(int64_t N, const T *__restrict__ y0, const T *__restrict__ k1, const T *__restrict__ k2,
+                            const T *__restrict__ k3, const T *__restrict__ k4, double c, const T *__restrict__ lam,
+                            T *__restrict__ lam0, T *__restrict__ a1, T *__restrict__ a2, T *__restrict__ a3,
+                            T *__restrict__ a4)
+{
+#pragma clang fp contract(off)
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        T z[3], l[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int64_t o = q * N + I;
+            T s = k1[o] + T(2) * k2[o] + T(2) * k3[o] + k4[o];
+            z[q] = y0[o] + (T)c * s;
+            l[q] = lam[o];
+        }
+        const T m2 = z[0] * z[0] + z[1] * z[1] + z[2] * z[2];
+        if (m2 > T(0)) {
+            const T len = T(1) / sqrt(m2);
+            const T y1[3] = {z[0] * len, z[1] * len, z[2] * len};
+            const T ly = l[0] * y1[0] + l[1] * y1[1] + l[2] * y1[2];
+#pragma unroll
+            for (int q = 0; q < 3; q++) l[q] = (l[q] - ly * y1[q]) * len;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int64_t o = q * N + I;
+            lam0[o] = l[q];
+            a1[o] = (T)c * l[q];
+            a2[o] = T(2) * (T)c * l[q];
+            a3[o] = T(2) * (T)c * l[q];
+            a4[o] = (T)c * l[q];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
@@ -667,6 +1018,87 @@ int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s)
     return g;
 }
 
+template <class T>
+void launch_adj_pre(int64_t N, const FieldParams<T> *P, const T *Y, const T *a, T *ut, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_pre<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, P, Y, a, ut);
+}
+
+template <class T>
+void launch_adj_final(int64_t N, const T *y0, const T *const k[4], double c, const T *lam, T *lam0, T *const a[4],
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_final<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, N, y0, k[0], k[1], k[2], k[3], c, lam,
+                       lam0, a[0], a[1], a[2], a[3]);
+}
+
+template <class T>
+void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *const *Ps, const BatchStage<T> *st,
+                        const Dyn *dyns, int V, int cpt, int grid, hipStream_t s)
+{
+#define BL_CALL(VV, CC) hipLaunchKernelGGL((k_batch_local<T, VV, CC>), dim3(grid), dim3(BLOCK), 0, s, items, nitems, Ps, st, dyns)
+#define BL_CPT(VV)                      \
+    if (cpt == 1) BL_CALL(VV, 1);       \
+    else if (cpt == 2) BL_CALL(VV, 2);  \
+    else BL_CALL(VV, 4);
+    if (V == 1) { BL_CPT(1) }
+    else if (V == 2) { BL_CPT(2) }
+    else { BL_CPT(0) }
+#undef BL_CPT
+#undef BL_CALL
+}
+
+void launch_batch_tick(Dyn *dyns, int M, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_tick, dim3((M + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, dyns, M);
+}
+
+int batch_variant(int mode, int fast) { return variant(mode, fast); }
+
+template <class T>
+int launch_sum(const T *v, int64_t n, double *partial, hipStream_t s)
+{
+    int g = grid_for(n, max_local_blocks());
+    hipLaunchKernelGGL(k_sum<T>, dim3(g), dim3(BLOCK), 0, s, v, n, partial);
+    return g;
+}
+
+template <class T>
+void launch_neb_dist(int64_t N, int n, const T *Y, const T *L, const T *R, int nb, double *partial, double *dist,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neb_dist<T>, dim3(nb, n + 1), dim3(BLOCK), 0, s, N, n, Y, L, R, partial);
+    hipLaunchKernelGGL(k_neb_dist_final, dim3(n + 1), dim3(BLOCK), 0, s, partial, nb, dist);
+}
+
+template <class T>
+void launch_neb_tangent(int64_t N, int n, const T *Ys, const T *L, const T *R, const double *energy, const T *F,
+                        T *tan, int nb, double *partial, const double *dist, double k, int climb, double *coef,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neb_tangent<T>, dim3(nb, n), dim3(BLOCK), 0, s, N, n, Ys, L, R, energy, F, tan, partial);
+    hipLaunchKernelGGL(k_neb_coef, dim3(n), dim3(BLOCK), 0, s, partial, nb, dist, k, climb, coef);
+}
+
+template <class T>
+void launch_neb_stage(const FieldParams<T> *P, int64_t N, int n, int nb, const T *Ys, const T *F, const T *tan,
+                      const double *coef, int mode, const double c[4], double outer, T *kstore, const T *y0,
+                      const T *k1, const T *k2, const T *k3, T *yout, double *partial, hipStream_t s)
+{
+    NebStage S{};
+    S.mode = mode;
+    for (int i = 0; i < 4; i++) S.c[i] = c[i];
+    S.outer = outer;
+    hipLaunchKernelGGL(k_neb_stage<T>, dim3(nb, n), dim3(BLOCK), 0, s, P, N, Ys, F, tan, coef, S, kstore, y0, k1,
+                       k2, k3, yout, partial);
+}
+
+template <class T>
+void launch_nbmask(int64_t N, const FieldParams<T> *P, uint8_t *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nbmask<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, P, out);
+}
+
 #define MMX_INST(T)                                                                                          \
     template void launch_aos_to_soa<T>(const double *, T *, int64_t, hipStream_t);                           \
     template void launch_soa_to_aos<T>(const T *, double *, int64_t, hipStream_t);                           \
@@ -686,7 +1118,22 @@ int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s)
                                            hipStream_t);                                                     \
     template int launch_maxdiff<T>(const T *, const T *, int64_t, double *, hipStream_t);          \
     template void launch_axpy<T>(const T *, double, const T *, T *, int64_t, hipStream_t);        \
-    template int launch_maxabs<T>(const T *, int64_t, double *, hipStream_t);
+    template int launch_maxabs<T>(const T *, int64_t, double *, hipStream_t);                    \
+    template void launch_batch_local<T>(const int2 *, int, const FieldParams<T> *const *, const BatchStage<T> *, \
+                                        const Dyn *, int, int, int, hipStream_t);                              \
+    template void launch_nbmask<T>(int64_t, const FieldParams<T> *, uint8_t *, hipStream_t);      \
+    template int launch_sum<T>(const T *, int64_t, double *, hipStream_t);                          \
+    template void launch_neb_dist<T>(int64_t, int, const T *, const T *, const T *, int, double *, double *,  \
+                                     hipStream_t);                                                          \
+    template void launch_neb_tangent<T>(int64_t, int, const T *, const T *, const T *, const double *,        \
+                                        const T *, T *, int, double *, const double *, double, int, double *, \
+                                        hipStream_t);                                                       \
+    template void launch_neb_stage<T>(const FieldParams<T> *, int64_t, int, int, const T *, const T *,        \
+                                      const T *, const double *, int, const double[4], double, T *,          \
+                                      const T *, const T *, const T *, const T *, T *, double *, hipStream_t); \
+    template void launch_adj_pre<T>(int64_t, const FieldParams<T> *, const T *, const T *, T *, hipStream_t); \
+    template void launch_adj_final<T>(int64_t, const T *, const T *const[4], double, const T *, T *,          \
+                                      T *const[4], hipStream_t);
 
 MMX_INST(double)
 MMX_INST(float)

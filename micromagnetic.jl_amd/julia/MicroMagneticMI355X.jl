@@ -226,4 +226,107 @@ function detach!(sim)
     return nothing
 end
 
+# ---- differentiation form: a ChainRules rrule over the RK4 trajectory ------------
+"""
+    evolve_rk4(sim, nsteps; every=50) -> m(T)
+
+Advance an attached simulation `nsteps` RK4 steps on the GPU.  With
+ChainRulesCore loaded, `rrule(evolve_rk4, sim, nsteps)` returns m(T) and a
+pullback mapping dL/dm(T) to (dL/dm0, per-term parameter gradients) through
+the hand-written adjoint kernels (mmx_adjoint_record / _backward).
+"""
+function evolve_rk4(sim, nsteps::Integer; every::Integer=50)
+    h = contexts[sim]
+    check(ccall((:mmx_adjoint_record, libmmx), Cint, (Ptr{Cvoid}, Int64, Int64), h, nsteps, every))
+    pull_spin!(sim)
+    return copy(sim.spin)
+end
+
+function evolve_rk4_pullback(sim, dL_dmT::AbstractVector)
+    h = contexts[sim]
+    n = sim.n_total
+    g0 = Vector{Float64}(undef, 3n)
+    check(ccall((:mmx_adjoint_backward, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h,
+                Vector{Float64}(dL_dmT), g0))
+    grads = Dict{Int,Vector{Float64}}()
+    nterms = ccall((:mmx_num_terms, libmmx), Cint, (Ptr{Cvoid},), h)
+    for t in 0:(nterms - 1)
+        g = Vector{Float64}(undef, n)
+        rc = ccall((:mmx_adjoint_param_grad, libmmx), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), h, t, g)
+        rc >= 0 && (grads[t + 1] = g)
+    end
+    return g0, grads
+end
+
+if isdefined(Main, :ChainRulesCore)
+    import ChainRulesCore
+    function ChainRulesCore.rrule(::typeof(evolve_rk4), sim, nsteps::Integer; every::Integer=50)
+        mT = evolve_rk4(sim, nsteps; every=every)
+        function pullback(ΔmT)
+            g0, grads = evolve_rk4_pullback(sim, ChainRulesCore.unthunk(ΔmT))
+            return (ChainRulesCore.NoTangent(), ChainRulesCore.Tangent{typeof(sim)}(; spin=g0, params=grads),
+                    ChainRulesCore.NoTangent())
+        end
+        return mT, pullback
+    end
+end
+
+# ---- batching form ------------------------------------------------------------------
+"""
+    batch_advance!(sims, nsteps)
+
+Advance several attached simulations (RungeKutta or Heun, no demag) together:
+one launch per stage over all of them, replayed from a HIP graph.
+"""
+function batch_advance!(sims::AbstractVector, nsteps::Integer)
+    hs = Ptr{Cvoid}[contexts[s] for s in sims]
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:mmx_batch_create, libmmx), Cint, (Ref{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint), b, hs, length(hs)))
+    try
+        check(ccall((:mmx_batch_advance, libmmx), Cint, (Ptr{Cvoid}, Int64), b[], nsteps))
+    finally
+        ccall((:mmx_batch_destroy, libmmx), Cint, (Ptr{Cvoid},), b[])
+    end
+    foreach(pull_spin!, sims)
+    return nothing
+end
+
+# ---- nudged elastic band (src/neb/neb.jl) ----------------------------------------------
+"""
+    neb_relax!(neb; max_steps, stopping_dmdt)
+
+Run relax(neb) for a reference `NEB` built with the LLG driver: its images
+(image_l, spin, image_r) go to the GPU band, the band relaxes there, and
+neb.spin, neb.energy_cpu and neb.distance are copied back.
+"""
+function neb_relax!(neb; max_steps=10000, stopping_dmdt=0.01)
+    sim = neb.sim
+    haskey(contexts, sim) || attach!(sim)
+    n, dof = neb.n_images, neb.dof
+    imgs = vcat(host(neb.image_l), host(neb.spin), host(neb.image_r))
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:mmx_neb_create, libmmx), Cint, (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Float64}, Cint, Cdouble, Cint),
+                b, contexts[sim], imgs, n, neb.spring_constant, neb.clib_image))
+    try
+        d = neb.driver
+        check(ccall((:mmx_neb_set_driver, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cdouble), b[],
+                    d.alpha, d.gamma, Cint(d.precession), d.integrator.tol))
+        steps = Ref{Int64}(0)
+        md = Ref{Float64}(0)
+        check(ccall((:mmx_neb_relax, libmmx), Cint, (Ptr{Cvoid}, Int64, Cdouble, Ref{Int64}, Ref{Float64}), b[],
+                    max_steps, stopping_dmdt, steps, md))
+        out = Vector{Float64}(undef, n * dof)
+        E = Vector{Float64}(undef, n + 2)
+        dist = Vector{Float64}(undef, n + 1)
+        check(ccall((:mmx_neb_get, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    b[], out, E, dist, C_NULL))
+        copyto!(neb.spin, out)
+        neb.energy_cpu .= E
+        neb.distance .= dist
+    finally
+        ccall((:mmx_neb_destroy, libmmx), Cint, (Ptr{Cvoid},), b[])
+    end
+    return nothing
+end
+
 end # module

@@ -10,6 +10,7 @@ the reference's init_scalar! / init_vector! do (src/helper.jl:5-102) and moves
 arrays across the C ABI.
 """
 import ctypes as C
+import os
 import math
 
 import numpy as np
@@ -385,16 +386,43 @@ def run_step(sim, driver=None, n=1):
 advance_step = run_step
 
 
+def _saver(sim):
+    if getattr(sim, "saver", None) is None:
+        from .io import DataSaver
+
+        sim.saver = DataSaver.for_sim(sim)
+    return sim.saver
+
+
 def run_until(sim, t_end, save_data=False):
+    """run_until(sim, t_end; save_data): src/sim.jl:547-579 (one table row when save_data)."""
     check(_lib.load().mmx_run_until(sim._h, float(t_end)))
+    sv = _saver(sim)
+    sv.t = float(t_end)
+    if save_data:
+        sv.write(sim)
+    sv.nsteps += 1
 
 
-def run_sim(sim, steps=10, dt=1e-10, save_data=False, call_back=None):
-    """run_sim: src/sim.jl:813-857 (data saving is not part of this path)."""
+def run_sim(sim, steps=10, dt=1e-10, save_data=True, save_m_every=-1, saver_item=None, call_back=None,
+            output="ovf"):
+    """run_sim: src/sim.jl:813-857 (tables and OVF snapshots; no VTK or live view)."""
+    from .io import SaverItem, save_ovf
+
+    sv = _saver(sim)
+    if isinstance(saver_item, SaverItem):
+        sv.items.append(saver_item)
+    elif isinstance(saver_item, (list, tuple)):
+        sv.items.extend(saver_item)
+    folder = f"{sim.name}_LLG"
+    if save_m_every >= 0:
+        os.makedirs(folder, exist_ok=True)
     for i in range(0, steps + 1):
-        run_until(sim, i * dt)
+        run_until(sim, i * dt, save_data=save_data)
         if call_back is not None:
             call_back(sim, i * dt)
+        if save_m_every > 0 and (i - 1) % save_m_every == 0 and output == "ovf":
+            save_ovf(sim, os.path.join(folder, "m_%08d.ovf" % i))
 
 
 def relax(sim, max_steps=10000, stopping_dmdt=0.01):
@@ -415,3 +443,167 @@ def demag_tensor(sim, term):
     out = np.empty(6 * sim.n_total)
     check(_lib.load().mmx_demag_tensor(sim._h, term.index, dptr(out)))
     return out.reshape(6, sim.n_total)
+
+
+def set_term_param(sim, term, values):
+    """Replace the per-cell scalar parameter of a term (A, D, Ku, Kc or T) in place."""
+    v = init_scalar(sim.mesh, values)
+    check(_lib.load().mmx_set_term_param(sim._h, term.index, dptr(v)))
+
+
+# ---- differentiation form ----------------------------------------------------
+def rk4_trajectory_rrule(sim, nsteps, every=50):
+    """Run `nsteps` RK4 steps from the current state and return (m_T, pullback).
+
+    The analogue of a ChainRules `rrule` for the map m0, {Ku, D, Kc, A} -> m(T):
+    `pullback(dL_dmT)` takes the cotangent of the final state (3N, AoS) and
+    returns {"m0": dL/dm0 (3N, AoS), term name: dL/dp per cell (N)} for every
+    term with a scalar parameter field (anisotropy Ku, cubic Kc, interfacial
+    DMI D, spatial exchange A).  The backward pass recomputes the trajectory
+    from checkpoints kept every `every` steps; it may be called more than once.
+    """
+    L = _lib.load()
+    check(L.mmx_adjoint_record(sim._h, int(nsteps), int(every)))
+    mT = sim.spin
+    N = sim.n_total
+
+    def pullback(dL_dmT):
+        lam = np.ascontiguousarray(np.asarray(dL_dmT, dtype=np.float64).reshape(-1))
+        if lam.size != 3 * N:
+            raise ValueError(f"expected {3 * N} cotangent values, got {lam.size}")
+        g0 = np.empty(3 * N)
+        check(L.mmx_adjoint_backward(sim._h, dptr(lam), dptr(g0)))
+        out = {"m0": g0}
+        for t in sim.interactions:
+            g = np.empty(N)
+            if L.mmx_adjoint_param_grad(sim._h, t.index, dptr(g)) >= 0:
+                out[t.name] = g
+        return out
+
+    return mT, pullback
+
+
+# ---- batching form -------------------------------------------------------------
+class Batch:
+    """Many independent Sims advanced together (one launch per RK stage over all
+    members, replayed from a HIP graph).  Each member must use the RungeKutta or
+    Heun integrator with a step set, share precision and device with the others
+    and have no demag term; after `advance(n)` each member is exactly where
+    `run_step(sim, n=n)` would have left it."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        check(_lib.load().mmx_batch_create(C.byref(h), arr, len(self.sims)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().mmx_batch_destroy(h)
+            self._h = None
+
+    def advance(self, nsteps=1):
+        check(_lib.load().mmx_batch_advance(self._h, int(nsteps)))
+
+    def info(self):
+        """Launch shape chosen by the set-up's timing: (cells per thread, grid,
+        work items, candidate times in ms)."""
+        c, g, w = C.c_int32(), C.c_int32(), C.c_int32()
+        t = (C.c_float * 12)()
+        check(_lib.load().mmx_batch_info(self._h, C.byref(c), C.byref(g), C.byref(w), t, 12))
+        return c.value, g.value, w.value, list(t)
+
+
+# ---- nudged elastic band -----------------------------------------------------
+def _slerp(x1, x2, t):
+    """slerp_kernel!: src/neb/math.jl:45-61 (cells with theta == 0 stay zero)."""
+    a = x1.reshape(-1, 3)
+    b = x2.reshape(-1, 3)
+    inner = np.clip(a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2], -1.0, 1.0)
+    theta = np.arccos(inner)
+    res = np.zeros_like(a)
+    nz = theta != 0.0
+    s = np.sin(theta[nz])
+    res[nz] = (np.sin((1 - t) * theta[nz]) / s)[:, None] * a[nz] + (np.sin(t * theta[nz]) / s)[:, None] * b[nz]
+    return res.reshape(-1)
+
+
+class NEB:
+    """NEB(sim, init_images, frames_between_images; spring_constant, clib_image):
+    src/neb/neb.jl:84-153 with the LLG driver (BS23, tol 1e-5, alpha 0.2,
+    precession off).  The images are aligned as init_m0 does (normalised, zero
+    where Ms = 0) and interpolated with slerp (init_images, neb.jl:155-175) on
+    the host; stepping runs on the GPU."""
+
+    def __init__(self, sim, init_images, frames_between_images, spring_constant=1.0e5, clib_image=-1):
+        N = sim.n_total
+        ms = sim.mu0_Ms
+        aligned = []
+        for img in init_images:
+            v = init_vector(sim.mesh, img).reshape(-1, 3).copy()
+            m2 = (v * v).sum(axis=1)
+            nz = m2 > 0
+            v[nz] *= (1 / np.sqrt(m2[nz]))[:, None]
+            v[ms == 0] = 0.0
+            aligned.append(v.reshape(-1))
+        images = [aligned[0]]
+        for i in range(len(aligned) - 1):
+            if frames_between_images[i] > 0:
+                for j in range(1, frames_between_images[i] + 1):
+                    images.append(_slerp(aligned[i], aligned[i + 1], j / (frames_between_images[i] + 1)))
+                images.append(aligned[i + 1])
+        self.n_images = len(images) - 2
+        if self.n_images <= 0:
+            raise ValueError("The number of free images should larger than 1 (typically 10~20).")
+        self.sim, self.N = sim, N
+        arr = np.ascontiguousarray(np.concatenate(images))
+        h = C.c_void_p()
+        check(_lib.load().mmx_neb_create(C.byref(h), sim._h, dptr(arr), self.n_images, float(spring_constant),
+                                         int(clib_image)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().mmx_neb_destroy(h)
+            self._h = None
+
+    def set_driver(self, alpha=0.2, gamma=2.21e5, precession=False, tol=1e-5):
+        check(_lib.load().mmx_neb_set_driver(self._h, float(alpha), float(gamma), int(bool(precession)),
+                                             float(tol)))
+
+    def advance(self, nsteps=1):
+        check(_lib.load().mmx_neb_advance(self._h, int(nsteps)))
+
+    def relax(self, max_steps=10000, stopping_dmdt=0.01):
+        st, md = C.c_int64(), C.c_double()
+        check(_lib.load().mmx_neb_relax(self._h, int(max_steps), float(stopping_dmdt), C.byref(st), C.byref(md)))
+        return st.value, md.value
+
+    def _get(self):
+        n, N = self.n_images, self.N
+        imgs = np.empty(n * 3 * N)
+        E = np.empty(n + 2)
+        d = np.empty(n + 1)
+        s = np.empty(5)
+        check(_lib.load().mmx_neb_get(self._h, dptr(imgs), dptr(E), dptr(d), dptr(s)))
+        return imgs.reshape(n, 3 * N), E, d, s
+
+    @property
+    def spin(self):
+        return self._get()[0]
+
+    @property
+    def energy(self):
+        return self._get()[1]
+
+    @property
+    def distance(self):
+        return self._get()[2]
+
+    @property
+    def state(self):
+        s = self._get()[3]
+        return dict(t=s[0], step=s[1], step_next=s[2], nsteps=int(s[3]), nfevals=int(s[4]))

@@ -69,6 +69,20 @@ SIGNATURES = {
     "mmx_synchronize": [_vp],
     "mmx_profile": [_vp, C.c_int],
     "mmx_profile_read": [_vp, _dp, _i64p],
+    "mmx_set_term_param": [_vp, C.c_int, _dp],
+    "mmx_adjoint_record": [_vp, C.c_int64, C.c_int64],
+    "mmx_adjoint_backward": [_vp, _dp, _dp],
+    "mmx_adjoint_param_grad": [_vp, C.c_int, _dp],
+    "mmx_batch_create": [C.POINTER(_vp), C.POINTER(_vp), C.c_int],
+    "mmx_batch_destroy": [_vp],
+    "mmx_batch_advance": [_vp, C.c_int64],
+    "mmx_batch_info": [_vp, _i32p, _i32p, _i32p, C.POINTER(C.c_float), C.c_int32],
+    "mmx_neb_create": [C.POINTER(_vp), _vp, _dp, C.c_int, C.c_double, C.c_int],
+    "mmx_neb_destroy": [_vp],
+    "mmx_neb_set_driver": [_vp, C.c_double, C.c_double, C.c_int, C.c_double],
+    "mmx_neb_advance": [_vp, C.c_int64],
+    "mmx_neb_relax": [_vp, C.c_int64, C.c_double, _i64p, _dp],
+    "mmx_neb_get": [_vp, _dp, _dp, _dp, _dp],
 }
 
 _lib = None

@@ -1,0 +1,120 @@
+"""Batching form: many independent systems of uneven size in one launch per
+stage must land exactly where each system advanced alone lands, and the
+single-system path is itself pinned to the oracle (test_gpu_parity)."""
+import numpy as np
+import pytest
+
+import mmx
+from oracle import OracleSim, RK4
+
+pytestmark = pytest.mark.gpu
+
+MU0 = 4e-7 * np.pi
+
+
+def _member(rng, integrator, precision=mmx.F64, thermal=False, params=None):
+    nx, ny, nz = (int(v) for v in rng.integers([5, 4, 1], [41, 33, 6]))
+    p = params or dict(Ms=rng.uniform(4e5, 1.2e6), A=rng.uniform(5e-12, 2e-11), D=rng.uniform(0, 4e-3),
+                       Ku=rng.uniform(0, 1e6))
+    pbc = "xy" if rng.uniform() < 0.5 else ""
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz, pbc=pbc)
+    sim = mmx.Sim(mesh, integrator=integrator, precision=precision)
+    N = nx * ny * nz
+    Ms = np.full(N, p["Ms"])
+    Ms[rng.integers(0, N)] = 0.0
+    mmx.set_Ms(sim, Ms)
+    m0 = rng.standard_normal((N, 3)) + np.array([0, 0, 2.0])
+    mmx.init_m0(sim, m0.reshape(-1))
+    mmx.add_exch(sim, p["A"])
+    mmx.add_dmi(sim, p["D"], type="interfacial")
+    mmx.add_anis(sim, p["Ku"], axis=(0, 0, 1))
+    mmx.add_zeeman(sim, (0, 0, 0.05 / MU0))
+    if thermal:
+        mmx.add_thermal_noise(sim, 300.0, seed=int(rng.integers(1, 1 << 30)))
+    mmx.set_driver(sim, integrator=integrator, step=1e-14 if integrator == "Heun" else 2e-14)
+    return sim, dict(mesh=(nx, ny, nz), pbc=pbc, Ms=Ms, m0=m0.reshape(-1), params=p)
+
+
+def _build(seed, n, integrator, precision=mmx.F64, thermal=False):
+    rng = np.random.default_rng(seed)
+    return [_member(rng, integrator, precision, thermal) for _ in range(n)]
+
+
+@pytest.mark.parametrize("integrator,thermal", [("RungeKutta", False), ("Heun", True)])
+def test_batch_matches_members_alone(integrator, thermal):
+    batch_members = _build(7, 9, integrator, thermal=thermal)
+    alone = _build(7, 9, integrator, thermal=thermal)
+    b = mmx.Batch([s for s, _ in batch_members])
+    cpt, grid, items, _ = b.info()
+    print(f"[batch] tuned: {cpt} cells/thread, grid {grid}, {items} work items", flush=True)
+    for n in (7, 4, 1):  # odd and even counts, several calls
+        b.advance(n)
+        for s, _ in alone:
+            mmx.run_step(s, n=n)
+        for (sb, _), (sa, _) in zip(batch_members, alone):
+            assert np.array_equal(sb.spin, sa.spin)
+            assert sb._integrator.nsteps == sa._integrator.nsteps
+            assert sb._integrator.t == sa._integrator.t
+
+
+def test_batch_member_steps_alone_in_between():
+    mem = _build(8, 4, "RungeKutta")
+    ref = _build(8, 4, "RungeKutta")
+    b = mmx.Batch([s for s, _ in mem])
+    b.advance(3)
+    mmx.run_step(mem[1][0], n=1)  # one member moves on its own (odd number of steps)
+    b.advance(2)
+    for i, (s, _) in enumerate(ref):
+        mmx.run_step(s, n=6 if i == 1 else 5)
+    for (sb, _), (sa, _) in zip(mem, ref):
+        assert np.array_equal(sb.spin, sa.spin)
+
+
+def test_batch_member_against_oracle():
+    mem = _build(9, 3, "RungeKutta")
+    b = mmx.Batch([s for s, _ in mem])
+    b.advance(10)
+    s, c = mem[2]
+    nx, ny, nz = c["mesh"]
+    p = c["params"]
+    o = OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9, pbc=c["pbc"])
+    o.set_Ms(c["Ms"])
+    o.init_m0(c["m0"])
+    o.add_exch(p["A"])
+    o.add_dmi(np.full(nx * ny * nz, p["D"]), type="interfacial")
+    o.add_anis(np.full(nx * ny * nz, p["Ku"]), axis=(0, 0, 1))
+    o.add_zeeman((0, 0, 0.05 / MU0))
+    o.set_llg(alpha=0.1, gamma=2.21e5, precession=True)
+    o.set_integrator(RK4, step=2e-14)
+    for _ in range(10):
+        o.advance_step()
+    assert np.abs(s.spin - o.spin).max() < 1e-12
+
+
+def test_batch_fp32_matches_members_alone():
+    mem = _build(10, 5, "RungeKutta", precision=mmx.F32)
+    ref = _build(10, 5, "RungeKutta", precision=mmx.F32)
+    b = mmx.Batch([s for s, _ in mem])
+    b.advance(6)
+    for s, _ in ref:
+        mmx.run_step(s, n=6)
+    for (sb, _), (sa, _) in zip(mem, ref):
+        assert np.array_equal(sb.spin, sa.spin)
+
+
+def test_batch_rejects_bad_members():
+    rng = np.random.default_rng(3)
+    s1, _ = _member(rng, "RungeKutta")
+    s2, _ = _member(rng, "Heun")
+    with pytest.raises(mmx.MmxError):
+        mmx.Batch([s1, s2])
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=4, ny=4, nz=2)
+    s3 = mmx.Sim(mesh, integrator="RungeKutta")
+    mmx.set_Ms(s3, 8e5)
+    mmx.init_m0(s3, (0, 0, 1))
+    mmx.add_demag(s3)
+    mmx.set_driver(s3, integrator="RungeKutta", step=1e-14)
+    with pytest.raises(mmx.MmxError):
+        mmx.Batch([s1, s3])
+    with pytest.raises(mmx.MmxError):
+        mmx.Batch([s1, s1])
