@@ -40,6 +40,7 @@ extern "C" {
 #define MMX_HEUN 1    /* "Heun"         src/integrator/heun.jl */
 #define MMX_DOPRI54 2 /* "DormandPrince" src/integrator/adaptive_rk_steps.jl:49 */
 #define MMX_BS23 3    /* "BS23"         src/integrator/adaptive_rk_steps.jl:11 */
+#define MMX_SD 4      /* driver "SD": steepest descent, src/sd.jl (step and tol unused) */
 
 typedef struct mmx_ctx mmx_ctx;
 
@@ -107,7 +108,10 @@ int mmx_set_llg(mmx_ctx *ctx, double alpha, double gamma, int precession);
 int mmx_llg_rhs(mmx_ctx *ctx, double t, const double *m, double *dmdt);
 
 /* create_driver("LLG", integrator): src/driver.jl:126-145.  step: fixed step of
- * RK4/Heun; tol: tolerance of DOPRI54/BS23.  Resets t, nsteps and nfevals. */
+ * RK4/Heun; tol: tolerance of DOPRI54/BS23.  Resets t, nsteps and nfevals.
+ * kind MMX_SD selects create_driver("SD") (src/driver.jl:93-100) instead:
+ * advance_step is then run_step(sim, ::SD) and relax uses tau/(gamma/2) as its
+ * step size; mmx_get_state reports tau as `step`. */
 int mmx_set_integrator(mmx_ctx *ctx, int kind, double step, double tol);
 /* advance_step(sim) n times: src/integrator/rk.jl:23, heun.jl:28, adaptive_rk.jl:93 */
 int mmx_advance_step(mmx_ctx *ctx, int64_t n);

@@ -116,6 +116,9 @@ struct mmx_ctx {
     int succeed = 0;
     int demag_term = -1;
     bool profile = false;
+    // SD driver (src/driver.jl:3-12): tau, min/max tau, step count
+    double sd_tau = 0, sd_min_tau = 1e-10, sd_max_tau = 1.0;
+    int64_t sd_steps = 0;
 };
 
 struct mmx_neb {
@@ -818,11 +821,51 @@ struct Engine : CtxBase {
         return MMX_OK;
     }
 
+
+    // run_step(sim, ::SD): src/sd.jl:68-81 (field at t = 0, Barzilai-Borwein tau,
+    // prespin = spin, update, normalise every 10 steps)
+    DevBuf<T> sd_g;
+    DevBuf<double> sd_dev;  // [sums(3), tau]
+    void sd_step()
+    {
+        ensure_k();
+        if (!sd_g.p) {
+            sd_g.alloc(3 * N);
+            HIPCHECK(hipMemsetAsync(sd_g.p, 0, 3 * N * sizeof(T), C->stream));
+            sd_dev.alloc(8);
+        }
+        T *h = k[6];
+        Epi<T> E = epi(EPI_FIELD);
+        E.out = h;
+        const double save_step = C->step;
+        C->step = 0;  // the stochastic field factor needs a time step; SD has none
+        eval(y, E);
+        C->step = save_step;
+        if (C->sd_steps == 0) {  // compute_gk_kernel_1!, tau = min_tau
+            launch_sd_g<T>(N, y, nullptr, h, sd_g.p, partial.p, C->stream);
+            const double t0 = C->sd_min_tau;
+            HIPCHECK(hipMemcpyAsync(sd_dev.p + 3, &t0, sizeof(double), hipMemcpyHostToDevice, C->stream));
+        } else {
+            const int g = launch_sd_g<T>(N, y, stg[1], h, sd_g.p, partial.p, C->stream);
+            launch_reduce(partial.p, g, 3, 3, 0, sd_dev.p, C->stream);
+            launch_sd_tau(sd_dev.p, (int)(C->sd_steps % 2), C->sd_min_tau, C->sd_max_tau, sd_dev.p + 3, C->stream);
+        }
+        launch_sd_update<T>(N, params_dev(), sd_g.p, y, stg[1], h, sd_dev.p + 3, C->stream);
+        C->sd_steps += 1;
+        if (C->sd_steps % 10 == 0) launch_normalise<T>(N, y, C->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(&C->sd_tau, sd_dev.p + 3, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+        HIPCHECK(hipStreamSynchronize(C->stream));
+        C->nfevals += 1;
+        C->nsteps += 1;
+    }
+
     int advance(int64_t n)
     {
         for (int64_t i = 0; i < n; i++) {
             if (C->ikind == MMX_RK4) rk4_step();
             else if (C->ikind == MMX_HEUN) heun_step();
+            else if (C->ikind == MMX_SD) sd_step();
             else {
                 int rc = adaptive_step();
                 if (rc != MMX_OK) return rc;
@@ -1938,7 +1981,7 @@ int mmx_llg_rhs(mmx_ctx *c, double t, const double *m, double *dmdt)
 int mmx_set_integrator(mmx_ctx *c, int kind, double step, double tol)
 {
     if (!c) return fail(MMX_EINVAL, "null context");
-    if (kind < MMX_RK4 || kind > MMX_BS23) return fail(MMX_EINVAL, "unknown integrator %d", kind);
+    if (kind < MMX_RK4 || kind > MMX_SD) return fail(MMX_EINVAL, "unknown integrator %d", kind);
     if ((kind == MMX_RK4 || kind == MMX_HEUN) && !(step > 0)) return fail(MMX_EINVAL, "fixed-step integrators need step > 0");
     if ((kind == MMX_DOPRI54 || kind == MMX_BS23) && !(tol > 0)) return fail(MMX_EINVAL, "tol must be > 0");
     c->ikind = kind;
@@ -1949,6 +1992,10 @@ int mmx_set_integrator(mmx_ctx *c, int kind, double step, double tol)
     c->step_next = 0;
     c->tol = tol;
     c->step = (kind == MMX_RK4 || kind == MMX_HEUN) ? step : 0.0;
+    if (kind == MMX_SD) {  // create_driver("SD"): src/driver.jl:93-100
+        c->sd_tau = 0;
+        c->sd_steps = 0;
+    }
     return MMX_OK;
 }
 
@@ -1990,7 +2037,9 @@ int mmx_relax(mmx_ctx *c, int64_t max_steps, double stopping_dmdt, int64_t *step
         for (i = 1; i <= max_steps; i++) {
             rc = e.advance(1);
             if (rc != MMX_OK) break;
-            max_dmdt = e.max_dm() / c->step;  // compute_dm!: helper.jl:273-286
+            // step_size: the integrator's step, or tau / (gamma/2) for SD (sim.jl:404-405, :430)
+            const double step_size = c->ikind == MMX_SD ? c->sd_tau / (2.21e5 / 2) : c->step;
+            max_dmdt = e.max_dm() / step_size;  // compute_dm!: helper.jl:273-286
             if (max_dmdt < stopping_dmdt * dmdt_factor) break;
         }
         if (steps) *steps = i > max_steps ? max_steps : i;
@@ -2004,7 +2053,7 @@ int mmx_get_state(mmx_ctx *c, double *t, double *step, double *step_next, int64_
 {
     if (!c) return fail(MMX_EINVAL, "null context");
     if (t) *t = c->t;
-    if (step) *step = c->step;
+    if (step) *step = c->ikind == MMX_SD ? c->sd_tau : c->step;
     if (step_next) *step_next = c->step_next;
     if (nsteps) *nsteps = c->nsteps;
     if (nfevals) *nfevals = c->nfevals;

@@ -90,6 +90,16 @@ void launch_neb_stage(const FieldParams<T> *P, int64_t N, int n, int nb, const T
 template <class T>
 void launch_nbmask(int64_t N, const FieldParams<T> *P, uint8_t *out, hipStream_t s);
 
+// steepest descent (src/sd.jl)
+template <class T>
+int launch_sd_g(int64_t N, const T *m, const T *mpre, const T *h, T *g, double *partial, hipStream_t s);
+void launch_sd_tau(const double *sums, int odd, double min_tau, double max_tau, double *tau, hipStream_t s);
+template <class T>
+void launch_sd_update(int64_t N, const FieldParams<T> *P, const T *g, T *m, T *mpre, const T *h, const double *tau,
+                      hipStream_t s);
+template <class T>
+void launch_normalise(int64_t N, T *m, hipStream_t s);
+
 int max_local_blocks();
 // number of blocks (reduction slot groups) the last demag pass launches
 int p5_blocks(const DemagGeom &G);

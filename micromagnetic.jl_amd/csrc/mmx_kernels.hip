@@ -617,6 +617,102 @@ __global__ void k_nbmask(const FieldParams<T> *__restrict__ Pp, uint8_t *__restr
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Steepest descent with Barzilai-Borwein steps (src/sd.jl)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void sd_cross(T x1, T x2, T x3, T y1, T y2, T y3, T &a, T &b, T &c)
+{
+    // cross_product: src/util.jl:30-33
+    a = -x3 * y2 + x2 * y3;
+    b = x3 * y1 - x1 * y3;
+    c = -x2 * y1 + x1 * y2;
+}
+
+// compute_gk_kernel_1! / _2! (sd.jl:9-37): g = m x (m x h); with prev != null
+// also the block sums of s.s, s.f, f.f (s = m - m_pre, f = g - g_old)
+template <class T>
+__global__ void k_sd_g(int64_t N, const T *__restrict__ m, const T *__restrict__ mpre, const T *__restrict__ h,
+                       T *__restrict__ g, double *__restrict__ partial)
+{
+#pragma clang fp contract(off)
+    double r[3] = {0, 0, 0};
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        const T mx = m[I], my = m[N + I], mz = m[2 * N + I];
+        T fx, fy, fz, gx, gy, gz;
+        sd_cross(mx, my, mz, h[I], h[N + I], h[2 * N + I], fx, fy, fz);
+        sd_cross(mx, my, mz, fx, fy, fz, gx, gy, gz);
+        if (mpre) {
+            const T sx = mx - mpre[I], sy = my - mpre[N + I], sz = mz - mpre[2 * N + I];
+            const T dx = gx - g[I], dy = gy - g[N + I], dz = gz - g[2 * N + I];
+            r[0] += (double)(sx * sx + sy * sy + sz * sz);
+            r[1] += (double)(sx * dx + sy * dy + sz * dz);
+            r[2] += (double)(dx * dx + dy * dy + dz * dz);
+        }
+        g[I] = gx;
+        g[N + I] = gy;
+        g[2 * N + I] = gz;
+    }
+    if (mpre) block_reduce_store<3>(r, 3, false, partial, 3);
+}
+
+// compute_tau (sd.jl:39-66) from the three sums; one thread
+__global__ void k_sd_tau(const double *__restrict__ sums, int odd, double min_tau, double max_tau,
+                         double *__restrict__ tau)
+{
+    const double s1 = sums[0], s2 = sums[1], s3 = sums[2];
+    const double tau1 = s2 != 0.0 ? s1 / s2 : min_tau;
+    const double tau2 = s3 != 0.0 ? s2 / s3 : min_tau;
+    double t = odd ? fabs(tau1) : fabs(tau2);
+    if (t > max_tau) t = max_tau;
+    tau[0] = t;
+}
+
+// run_step_kernel! (sd.jl:39-55): prespin = m, then the Cayley-like update
+template <class T>
+__global__ void k_sd_update(const FieldParams<T> *__restrict__ Pp, const T *__restrict__ g, T *__restrict__ m,
+                            T *__restrict__ mpre, const T *__restrict__ h, const double *__restrict__ taup)
+{
+#pragma clang fp contract(off)
+    const FieldParams<T> &P = *Pp;
+    const int64_t N = P.N;
+    const T tau = (T)taup[0];
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        const T mx = m[I], my = m[N + I], mz = m[2 * N + I];
+        mpre[I] = mx;
+        mpre[N + I] = my;
+        mpre[2 * N + I] = mz;
+        if (P.pins && P.pins[I]) continue;
+        T fx, fy, fz;
+        sd_cross(mx, my, mz, h[I], h[N + I], h[2 * N + I], fx, fy, fz);
+        const T factor = T(0.25) * (fx * fx + fy * fy + fz * fz) * (tau * tau);
+        const T ax = (T(1) - factor) * mx - tau * g[I];
+        const T ay = (T(1) - factor) * my - tau * g[N + I];
+        const T az = (T(1) - factor) * mz - tau * g[2 * N + I];
+        m[I] = ax / (T(1) + factor);
+        m[N + I] = ay / (T(1) + factor);
+        m[2 * N + I] = az / (T(1) + factor);
+    }
+}
+
+// normalise: src/helper.jl:255-271
+template <class T>
+__global__ void k_normalise(int64_t N, T *__restrict__ m)
+{
+#pragma clang fp contract(off)
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        const T x = m[I], y = m[N + I], z = m[2 * N + I];
+        const T m2 = x * x + y * y + z * z;
+        if (m2 > T(0)) {
+            const T len = T(1) / sqrt(m2);
+            m[I] = x * len;
+            m[N + I] = y * len;
+            m[2 * N + I] = z * len;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // adjoint helpers (RK4 step pull-back)
 // ---------------------------------------------------------------------------
@@ -1099,6 +1195,32 @@ void launch_nbmask(int64_t N, const FieldParams<T> *P, uint8_t *out, hipStream_t
     hipLaunchKernelGGL(k_nbmask<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, P, out);
 }
 
+template <class T>
+int launch_sd_g(int64_t N, const T *m, const T *mpre, const T *h, T *g, double *partial, hipStream_t s)
+{
+    int gr = grid_for(N, max_local_blocks());
+    hipLaunchKernelGGL(k_sd_g<T>, dim3(gr), dim3(BLOCK), 0, s, N, m, mpre, h, g, partial);
+    return gr;
+}
+
+void launch_sd_tau(const double *sums, int odd, double min_tau, double max_tau, double *tau, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sd_tau, dim3(1), dim3(1), 0, s, sums, odd, min_tau, max_tau, tau);
+}
+
+template <class T>
+void launch_sd_update(int64_t N, const FieldParams<T> *P, const T *g, T *m, T *mpre, const T *h, const double *tau,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sd_update<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, P, g, m, mpre, h, tau);
+}
+
+template <class T>
+void launch_normalise(int64_t N, T *m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_normalise<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, N, m);
+}
+
 #define MMX_INST(T)                                                                                          \
     template void launch_aos_to_soa<T>(const double *, T *, int64_t, hipStream_t);                           \
     template void launch_soa_to_aos<T>(const T *, double *, int64_t, hipStream_t);                           \
@@ -1121,6 +1243,10 @@ void launch_nbmask(int64_t N, const FieldParams<T> *P, uint8_t *out, hipStream_t
     template int launch_maxabs<T>(const T *, int64_t, double *, hipStream_t);                    \
     template void launch_batch_local<T>(const int2 *, int, const FieldParams<T> *const *, const BatchStage<T> *, \
                                         const Dyn *, int, int, int, hipStream_t);                              \
+    template int launch_sd_g<T>(int64_t, const T *, const T *, const T *, T *, double *, hipStream_t); \
+    template void launch_sd_update<T>(int64_t, const FieldParams<T> *, const T *, T *, T *, const T *,        \
+                                      const double *, hipStream_t);                                         \
+    template void launch_normalise<T>(int64_t, T *, hipStream_t);                                         \
     template void launch_nbmask<T>(int64_t, const FieldParams<T> *, uint8_t *, hipStream_t);      \
     template int launch_sum<T>(const T *, int64_t, double *, hipStream_t);                          \
     template void launch_neb_dist<T>(int64_t, int, const T *, const T *, const T *, int, double *, double *,  \

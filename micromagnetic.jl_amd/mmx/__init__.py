@@ -183,8 +183,8 @@ class Sim:
     """Sim(mesh; driver="LLG", integrator="DormandPrince"): src/sim.jl:48-107."""
 
     def __init__(self, mesh, driver="LLG", name="dyn", integrator="DormandPrince", precision=F64, device=0):
-        if driver != "LLG":
-            raise ValueError(f"driver {driver!r} is not available on this path (LLG only)")
+        if driver not in ("LLG", "SD"):
+            raise ValueError(f"driver {driver!r} is not available on this path (LLG and SD)")
         if integrator not in _INTEGRATORS:
             raise ValueError("Supported integrators: " + " ".join(_INTEGRATORS))
         self.mesh, self.name = mesh, name
@@ -199,7 +199,8 @@ class Sim:
         self.interactions = []
         self.mu0_Ms = np.zeros(self.n_total)
         self.driver = LLGDriver(self)
-        kind = _INTEGRATORS[integrator]
+        kind = _lib.SD if driver == "SD" else _INTEGRATORS[integrator]
+        self.driver_name = driver
         self._integrator = Integrator(self, kind, _DEFAULT_STEP.get(kind, 0.0), 1e-6)
         check(L.mmx_set_integrator(self._h, kind, _DEFAULT_STEP.get(kind, 0.0), 1e-6))
         self._push_llg()
@@ -370,9 +371,11 @@ def llg_rhs(sim, spin=None, t=0.0):
 
 
 def set_driver(sim, driver="LLG", integrator="DormandPrince", step=None, tol=1e-6):
-    if driver != "LLG":
-        raise ValueError("LLG only")
-    kind = _INTEGRATORS[integrator]
+    """set_driver: src/sim.jl:589-620 ("LLG" with an integrator, or "SD")."""
+    if driver not in ("LLG", "SD"):
+        raise ValueError("Supported drivers: LLG SD")
+    kind = _lib.SD if driver == "SD" else _INTEGRATORS[integrator]
+    sim.driver_name = driver
     st = _DEFAULT_STEP.get(kind, 0.0) if step is None else float(step)
     sim._integrator = Integrator(sim, kind, st, tol)
     check(_lib.load().mmx_set_integrator(sim._h, kind, st, float(tol)))

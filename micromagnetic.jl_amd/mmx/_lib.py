@@ -16,7 +16,7 @@ HEADER = os.path.join(REPO_ROOT, "include", "mmx.h")
 OK, EINVAL, EHIP, ENAN, ETRIES, ESTATE, ENOMEM = 0, -1, -2, -3, -4, -5, -6
 F64, F32 = 64, 32
 DMI_BULK, DMI_INTERFACIAL = 0, 1
-RK4, HEUN, DOPRI54, BS23 = 0, 1, 2, 3
+RK4, HEUN, DOPRI54, BS23, SD = 0, 1, 2, 3, 4
 PASSES = ("P1", "P2", "P3", "P4", "P5", "LOCAL")
 
 
