@@ -32,7 +32,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_system(mmx, nx, ny, nz, precision, images=5, seed=2002, dt=1e-14):
+def build_system(mmx, nx, ny, nz, precision, images=5, seed=2002, dt=1e-14, demag=True):
     mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz, pbc="xy")
     sim = mmx.Sim(mesh, integrator="RungeKutta", precision=precision)
     rng = np.random.default_rng(seed)
@@ -47,7 +47,8 @@ def build_system(mmx, nx, ny, nz, precision, images=5, seed=2002, dt=1e-14):
     mmx.add_dmi(sim, 3e-3, type="interfacial")
     mmx.add_anis(sim, 8e5, axis=(0, 0, 1))
     mmx.add_zeeman(sim, (0, 0, 0.1 / MU0))
-    mmx.add_demag(sim, Nx=images, Ny=images, Nz=0)
+    if demag:
+        mmx.add_demag(sim, Nx=images, Ny=images, Nz=0)
     mmx.set_driver(sim, integrator="RungeKutta", step=dt)
     return sim, m0
 
@@ -280,7 +281,7 @@ def bench_adjoint(mmx, precision, nsteps=2000, every=50, n_fd=8, seed=5005):
             worst = 0.0
             for i in pr.choice(3 * N if key == "m0" else N, n_fd, replace=False):
                 if key == "m0":
-                    eps = 1e-6
+                    eps = 1e-4
                     p_, q_ = m0.copy(), m0.copy()
                     p_[i] += eps
                     q_[i] -= eps
@@ -329,6 +330,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batching and differentiation forms")
+    ap.add_argument("--local-only", action="store_true",
+                    help="time the same film without its demag term (local-field kernel only) and exit")
     ap.add_argument("--fd-cells", type=int, default=8, help="finite-difference probes per gradient quantity")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
@@ -353,6 +356,14 @@ def main():
 
     nx, ny, nz = (int(v) for v in args.mesh.split("x"))
     N = nx * ny * nz
+    if args.local_only:
+        sim, _ = build_system(mmx, nx, ny, nz, mmx.F64, demag=False)
+        el = time_steps(sim, mmx, args.steps, args.warmup)
+        passes = profile_passes(sim, mmx, max(5, min(args.steps, 20)))
+        ms = passes["LOCAL"][0]
+        log(f"[bench] local-only FP64: {N * 4 * args.steps / el / 1e6:.1f} Mcell-evals/s, "
+            f"local kernel {ms * 1e3:.1f} us per launch")
+        return
     log(f"[bench] FP64 set-up {nx}x{ny}x{nz}, images {args.images}")
     t0 = time.perf_counter()
     sim, m0 = build_system(mmx, nx, ny, nz, mmx.F64, images=args.images)

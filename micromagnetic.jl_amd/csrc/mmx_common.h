@@ -13,6 +13,21 @@
 namespace mmx {
 
 constexpr int MAX_TERMS = 10;
+
+// The parameter block is read through the constant address space: the
+// compiler then knows that no store of a kernel can change it, keeps its
+// fields in scalar registers across the cell loops and loads them with
+// scalar loads (the block stays in ordinary device memory).
+#define MMX_CONST __attribute__((address_space(4)))
+
+// Loads from per-cell arrays through the global address space (plain
+// pointers would otherwise become flat accesses, which also count against
+// the scalar/LDS wait counter).
+template <class X>
+__device__ __forceinline__ X gld(const X *p, int64_t i)
+{
+    return ((const __attribute__((address_space(1))) X *)p)[i];
+}
 constexpr int BLOCK = 256;
 
 enum TermKind : int {

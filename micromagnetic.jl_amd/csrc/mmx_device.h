@@ -68,7 +68,7 @@ __device__ __forceinline__ double pmax(double a, double b)
 // ---- neighbours: indexpbc of src/micro/mesh.jl:31-43 ---------------------
 // order (-x, +x, -y, +y, -z, +z); -1 when there is no neighbour.
 template <class T>
-__device__ __forceinline__ void neighbours(const FieldParams<T> &P, int i, int j, int k, int64_t nb[6])
+__device__ __forceinline__ void neighbours(const MMX_CONST FieldParams<T> &P, int i, int j, int k, int64_t nb[6])
 {
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const int64_t sxy = (int64_t)nx * ny;
@@ -100,7 +100,7 @@ __device__ __forceinline__ void neighbours(const FieldParams<T> &P, int i, int j
 // order; hd = demag field of this cell when a demag term is present.
 // If esum != nullptr, it receives the summed energy (J) of the selected terms.
 template <class T>
-__device__ __forceinline__ void cell_field(const FieldParams<T> &P, const Dyn &dyn, uint32_t mask, int64_t I,
+__device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, uint32_t mask, int64_t I,
                                            int i, int j, int k, const T *__restrict__ m, T hdx, T hdy,
                                            T hdz, T &Hx, T &Hy, T &Hz, double *esum)
 {
@@ -128,7 +128,7 @@ __device__ __forceinline__ void cell_field(const FieldParams<T> &P, const Dyn &d
     Hx = Hy = Hz = T(0);
     for (int tt = 0; tt < P.nterms; tt++) {
         if (!((mask >> tt) & 1u)) continue;
-        const TermDev<T> &D = P.t[tt];
+        const MMX_CONST TermDev<T> &D = P.t[tt];
         T fx = 0, fy = 0, fz = 0;  // "f" of the exchange/DMI kernels before 1/Ms
         T hx = 0, hy = 0, hz = 0;
         double e = 0;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void cell_field(const FieldParams<T> &P, const Dyn &d
 // Field and energy of a term that needs only this cell (Zeeman, anisotropies,
 // demag, thermal); returns false for the neighbour terms.
 template <class T>
-__device__ __forceinline__ bool local_term(const TermDev<T> &D, const FieldParams<T> &P, const Dyn &dyn, int64_t I,
+__device__ __forceinline__ bool local_term(const MMX_CONST TermDev<T> &D, const MMX_CONST FieldParams<T> &P, const Dyn &dyn, int64_t I,
                                            T mx, T my, T mz, T Ms, T hdx, T hdy, T hdz, T &hx, T &hy, T &hz,
                                            double &e)
 {
@@ -420,206 +420,329 @@ __device__ __forceinline__ bool local_term(const TermDev<T> &D, const FieldParam
     return true;
 }
 
-// Slot form of cell_field for meshes whose terms are at most one of each kind:
-// each neighbour is loaded once and contributes to the exchange and DMI slots
-// in the reference's neighbour order (-x,+x,-y,+y,-z,+z) per term; every
-// slot's field is then added in insertion order, exactly as cell_field does.
-// Returns m of the cell in mx,my,mz.
+// ---- slot form ---------------------------------------------------------------
+// When every kind of term appears at most once, the kernels take this form.
+// SlotCfg gathers what the cell loop needs from the parameter block once per
+// block (wave-uniform values, kept in scalar registers); a cell then issues all
+// its loads (own values, six neighbours, spatial parameters) before any
+// arithmetic, and the field of each slot is added in insertion order exactly
+// as cell_field adds the terms.
 template <class T>
-__device__ __forceinline__ void cell_field_slots(const FieldParams<T> &P, const Dyn &dyn, int I, int i, int j, int k,
-                                                 const T *__restrict__ m, T hdx, T hdy, T hdz, T &mx, T &my, T &mz,
-                                                 T &Hx, T &Hy, T &Hz, uint32_t &msk)
+struct SlotCfg {
+    int nx, ny, nz, N, px, py, pz;
+    const T *mu0Ms;
+    const uint8_t *nbm;
+    int ex;  // exchange: 0 none, 1 uniform, 2 spatial A
+    T nab0, nab1, nab2;
+    const T *exA;
+    int bd;  // bulk DMI: 0 none, 1 uniform, 2 spatial
+    T bs0, bs1, bs2;
+    const T *bDx, *bDy, *bDz;
+    int idm;  // interfacial DMI: 0 none, 1 uniform, 2 spatial D
+    T id0, id1, iDe, iDu;
+    const T *iDa;
+    int an, anAx;  // uniaxial present; per-cell axes
+    T ax, ay, az, Ku;
+    const T *KuA, *axA;
+    int cu, sc;
+    T Kc;
+    const T *KcA;
+    int ze;  // Zeeman: 0 none, 1 uniform, 2 per cell
+    T zx, zy, zz;
+    const T *zA, *zB, *zC;
+    int th;
+    T T0, Tu;
+    const T *TA;
+    uint64_t seed;
+    int nterms;
+    T alpha, llg_a;
+    int precession;
+};
+
+template <class T>
+__device__ __forceinline__ SlotCfg<T> slot_cfg(const MMX_CONST FieldParams<T> &P)
+{
+    SlotCfg<T> C;
+    C.nx = P.nx;
+    C.ny = P.ny;
+    C.nz = P.nz;
+    C.N = (int)P.N;
+    C.px = P.px;
+    C.py = P.py;
+    C.pz = P.pz;
+    C.mu0Ms = P.mu0Ms;
+    C.nbm = P.nbm;
+    const int se = P.slot[SL_EXCH], sb = P.slot[SL_DMIB], si = P.slot[SL_DMII], sa = P.slot[SL_ANIS];
+    const int sc = P.slot[SL_CUBIC], sz = P.slot[SL_ZEE], st = P.slot[SL_THERM];
+    {
+        const MMX_CONST TermDev<T> &D = P.t[se >= 0 ? se : 0];
+        C.ex = se < 0 ? 0 : (D.kind == K_EXCH_S ? 2 : 1);
+        C.nab0 = D.s[0];
+        C.nab1 = D.s[1];
+        C.nab2 = D.s[2];
+        C.exA = D.a;
+    }
+    {
+        const MMX_CONST TermDev<T> &D = P.t[sb >= 0 ? sb : 0];
+        C.bd = sb < 0 ? 0 : (D.kind == K_DMI_BULK_S ? 2 : 1);
+        C.bs0 = D.s[0];
+        C.bs1 = D.s[1];
+        C.bs2 = D.s[2];
+        C.bDx = D.a;
+        C.bDy = D.b;
+        C.bDz = D.c;
+    }
+    {
+        const MMX_CONST TermDev<T> &D = P.t[si >= 0 ? si : 0];
+        C.idm = si < 0 ? 0 : (D.uniform ? 1 : 2);
+        C.id0 = D.s[0];
+        C.id1 = D.s[1];
+        C.iDu = D.s[3];
+        C.iDe = D.s[4];
+        C.iDa = D.a;
+    }
+    {
+        const MMX_CONST TermDev<T> &D = P.t[sa >= 0 ? sa : 0];
+        C.an = sa >= 0;
+        C.anAx = D.kind == K_ANIS_S;
+        C.ax = D.s[0];
+        C.ay = D.s[1];
+        C.az = D.s[2];
+        C.Ku = D.s[3];
+        C.KuA = D.uniform ? nullptr : D.a;
+        C.axA = D.b;
+    }
+    {
+        const MMX_CONST TermDev<T> &D = P.t[sc >= 0 ? sc : 0];
+        C.cu = sc >= 0;
+        C.sc = sc >= 0 ? sc : 0;
+        C.Kc = D.s[9];
+        C.KcA = D.uniform ? nullptr : D.a;
+    }
+    {
+        const MMX_CONST TermDev<T> &D = P.t[sz >= 0 ? sz : 0];
+        C.ze = sz < 0 ? 0 : (D.kind == K_ZEEMAN ? 2 : 1);
+        C.zx = D.s[0];
+        C.zy = D.s[1];
+        C.zz = D.s[2];
+        C.zA = D.a;
+        C.zB = D.b;
+        C.zC = D.c;
+    }
+    {
+        const MMX_CONST TermDev<T> &D = P.t[st >= 0 ? st : 0];
+        C.th = st >= 0;
+        C.T0 = D.s[0];
+        C.Tu = D.s[1];
+        C.TA = D.uniform ? nullptr : D.a;
+    }
+    C.seed = P.seed;
+    C.nterms = P.nterms;
+    C.alpha = P.alpha;
+    C.llg_a = P.llg_a;
+    C.precession = P.precession;
+    return C;
+}
+
+template <class X>
+__device__ __forceinline__ void gst(X *p, int64_t i, X v)
+{
+    ((__attribute__((address_space(1))) X *)p)[i] = v;
+}
+
+// Field of cell I in slot form (see SlotCfg); returns m of the cell and its
+// neighbour / pin mask.
+template <class T>
+__device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, int I, int i, int j,
+                                                int k, const T *__restrict__ m, T hdx, T hdy, T hdz, T &mx, T &my,
+                                                T &mz, T &Hx, T &Hy, T &Hz, uint32_t &msk)
 {
 #pragma clang fp contract(off)
-    const int N = (int)P.N;
-    msk = P.nbm[I];
-    mx = m[I];
-    my = m[N + I];
-    mz = m[2 * N + I];
-    const T Ms = P.mu0Ms[I];
-    const int se = P.slot[SL_EXCH], sb = P.slot[SL_DMIB], si = P.slot[SL_DMII];
-    bool ex_on = false, ex_sp = false;
-    T exA = 0, nab0 = 0, nab1 = 0, nab2 = 0;
-    if (se >= 0) {
-        const TermDev<T> &D = P.t[se];
-        ex_sp = D.kind == K_EXCH_S;
-        nab0 = D.s[0];
-        nab1 = D.s[1];
-        nab2 = D.s[2];
-        if (ex_sp) exA = D.a[I];
-        ex_on = (Ms != T(0)) && (!ex_sp || exA != T(0));
+    // every parameter read is unconditional and loop-invariant, so the
+    // compiler hoists them out of the callers' cell loops
+    const SlotCfg<T> C = slot_cfg(P);
+    const int N = C.N;
+    // ---- loads ----
+    msk = gld(C.nbm, I);
+    const T Ms = gld(C.mu0Ms, I);
+    mx = gld(m, I);
+    my = gld(m, N + I);
+    mz = gld(m, 2 * N + I);
+    const bool stencil = C.ex || C.bd || C.idm;
+    int nb[6];
+    {
+        const int nx = C.nx, ny = C.ny, nz = C.nz, sxy = nx * ny;
+        int t;
+        t = i - 1;
+        if (C.px && t < 0) t += nx;
+        nb[0] = (t >= 0) ? I - i + t : I;
+        t = i + 1;
+        if (C.px && t >= nx) t -= nx;
+        nb[1] = (t < nx) ? I - i + t : I;
+        t = j - 1;
+        if (C.py && t < 0) t += ny;
+        nb[2] = (t >= 0) ? I + (t - j) * nx : I;
+        t = j + 1;
+        if (C.py && t >= ny) t -= ny;
+        nb[3] = (t < ny) ? I + (t - j) * nx : I;
+        t = k - 1;
+        if (C.pz && t < 0) t += nz;
+        nb[4] = (t >= 0) ? I + (t - k) * sxy : I;
+        t = k + 1;
+        if (C.pz && t >= nz) t -= nz;
+        nb[5] = (t < nz) ? I + (t - k) * sxy : I;
     }
-    bool bd_on = false, bd_sp = false;
-    T bs0 = 0, bs1 = 0, bs2 = 0, bDx = 0, bDy = 0, bDz = 0;
-    if (sb >= 0) {
-        const TermDev<T> &D = P.t[sb];
-        bd_sp = D.kind == K_DMI_BULK_S;
-        bs0 = D.s[0];
-        bs1 = D.s[1];
-        bs2 = D.s[2];
-        if (bd_sp) {
-            bDx = D.a[I];
-            bDy = D.b[I];
-            bDz = D.c[I];
-        }
-        bd_on = Ms != T(0);
-    }
-    bool id_on = false, id_un = false;
-    T iDI = 0, id0 = 0, id1 = 0, iDe = 0;
-    if (si >= 0) {
-        const TermDev<T> &D = P.t[si];
-        id_un = D.uniform != 0;
-        id0 = D.s[0];
-        id1 = D.s[1];
-        iDe = D.s[4];
-        iDI = id_un ? D.s[3] : D.a[I];
-        id_on = (Ms != T(0)) && (iDI != T(0));
-    }
-    T fex = 0, fey = 0, fez = 0, fbx = 0, fby = 0, fbz = 0, fix = 0, fiy = 0, fiz = 0;
-    if (ex_on || bd_on || id_on) {
-        const int nx = P.nx, ny = P.ny, nz = P.nz, sxy = nx * ny;
+    T nmx[6], nmy[6], nmz[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) nmx[q] = nmy[q] = nmz[q] = T(0);
+    if (stencil) {
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            int id = -1, t;
-            if (q == 0) { t = i - 1; if (P.px && t < 0) t += nx; id = (t >= 0) ? I - i + t : -1; }
-            if (q == 1) { t = i + 1; if (P.px && t >= nx) t -= nx; id = (t < nx) ? I - i + t : -1; }
-            if (q == 2) { t = j - 1; if (P.py && t < 0) t += ny; id = (t >= 0) ? I + (t - j) * nx : -1; }
-            if (q == 3) { t = j + 1; if (P.py && t >= ny) t -= ny; id = (t < ny) ? I + (t - j) * nx : -1; }
-            if (q == 4) { t = k - 1; if (P.pz && t < 0) t += nz; id = (t >= 0) ? I + (t - k) * sxy : -1; }
-            if (q == 5) { t = k + 1; if (P.pz && t >= nz) t -= nz; id = (t < nz) ? I + (t - k) * sxy : -1; }
-            // the loads of this neighbour come from a clamped index; the
-            // cell's mask decides afterwards whether it counts
-            const int idc = id >= 0 ? id : I;
-            const T ax = m[idc], ay = m[N + idc], az = m[2 * N + idc];
+            nmx[q] = gld(m, nb[q]);
+            nmy[q] = gld(m, N + nb[q]);
+            nmz[q] = gld(m, 2 * N + nb[q]);
+        }
+    }
+    T exAI = 0, exAn[6] = {0, 0, 0, 0, 0, 0};
+    if (C.ex == 2) {
+        exAI = gld(C.exA, I);
+#pragma unroll
+        for (int q = 0; q < 6; q++) exAn[q] = gld(C.exA, nb[q]);
+    }
+    T bDI[3] = {0, 0, 0}, bDn[6] = {0, 0, 0, 0, 0, 0};
+    if (C.bd == 2) {
+        bDI[0] = gld(C.bDx, I);
+        bDI[1] = gld(C.bDy, I);
+        bDI[2] = gld(C.bDz, I);
+#pragma unroll
+        for (int q = 0; q < 6; q++) bDn[q] = gld(q < 2 ? C.bDx : (q < 4 ? C.bDy : C.bDz), nb[q]);
+    }
+    T iDI = C.iDu, iDn[4] = {C.iDu, C.iDu, C.iDu, C.iDu};
+    if (C.idm == 2) {
+        iDI = gld(C.iDa, I);
+#pragma unroll
+        for (int q = 0; q < 4; q++) iDn[q] = gld(C.iDa, nb[q]);
+    }
+    T ax = C.ax, ay = C.ay, az = C.az, Ku = C.Ku;
+    if (C.an) {
+        if (C.anAx) {
+            ax = gld(C.axA, I);
+            ay = gld(C.axA, N + I);
+            az = gld(C.axA, 2 * N + I);
+        }
+        if (C.KuA) Ku = gld(C.KuA, I);
+    }
+    T Kc = C.Kc;
+    if (C.cu && C.KcA) Kc = gld(C.KcA, I);
+    T zhx = 0, zhy = 0, zhz = 0;
+    if (C.ze == 1) {
+        zhx = C.zx;
+        zhy = C.zy;
+        zhz = C.zz;
+    } else if (C.ze == 2) {
+        zhx = gld(C.zA, I);
+        zhy = gld(C.zB, I);
+        zhz = gld(C.zC, I);
+    }
+    T Tl = 0;
+    if (C.th) Tl = (C.TA ? gld(C.TA, I) : C.Tu) + C.T0;
+    // ---- arithmetic ----
+    const bool ex_on = C.ex && (Ms != T(0)) && (C.ex == 1 || exAI != T(0));
+    const bool bd_on = C.bd && (Ms != T(0));
+    const bool id_on = C.idm && (Ms != T(0)) && (iDI != T(0));
+    T fex = 0, fey = 0, fez = 0, fbx = 0, fby = 0, fbz = 0, fix = 0, fiy = 0, fiz = 0;
+    if (stencil) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
             if (!((msk >> q) & 1u)) continue;
-            if (ex_on) {  // kernels.jl:269-304 (uniform), :226-266 (spatial)
-                const T nab = (q < 2) ? nab0 : ((q < 4) ? nab1 : nab2);
-                if (!ex_sp) {
-                    fex += nab * (ax - mx);
-                    fey += nab * (ay - my);
-                    fez += nab * (az - mz);
-                } else {
-                    const T A_nb = P.t[se].a[id];
-                    if (A_nb != T(0)) {
-                        const T coeff = (T(2) * exA * A_nb / (exA + A_nb)) * nab;
-                        fex += coeff * (ax - mx);
-                        fey += coeff * (ay - my);
-                        fez += coeff * (az - mz);
-                    }
+            const T ax_ = nmx[q], ay_ = nmy[q], az_ = nmz[q];
+            if (C.ex == 1) {  // kernels.jl:269-304
+                const T nab = (q < 2) ? C.nab0 : ((q < 4) ? C.nab1 : C.nab2);
+                fex += nab * (ax_ - mx);
+                fey += nab * (ay_ - my);
+                fez += nab * (az_ - mz);
+            } else if (C.ex == 2) {  // kernels.jl:226-266
+                const T nab = (q < 2) ? C.nab0 : ((q < 4) ? C.nab1 : C.nab2);
+                const T A_nb = exAn[q];
+                if (A_nb != T(0)) {
+                    const T coeff = (T(2) * exAI * A_nb / (exAI + A_nb)) * nab;
+                    fex += coeff * (ax_ - mx);
+                    fey += coeff * (ay_ - my);
+                    fez += coeff * (az_ - mz);
                 }
             }
-            if (bd_on) {
-                if (!bd_sp) {  // bulkdmi_kernel!: kernels.jl:306-344
-                    if (q == 0) { fby += bs0 * (-az); fbz += bs0 * ay; }
-                    if (q == 1) { fby += bs0 * az;    fbz += bs0 * (-ay); }
-                    if (q == 2) { fbx += bs1 * az;    fbz += bs1 * (-ax); }
-                    if (q == 3) { fbx += bs1 * (-az); fbz += bs1 * ax; }
-                    if (q == 4) { fbx += bs2 * (-ay); fby += bs2 * ax; }
-                    if (q == 5) { fbx += bs2 * ay;    fby += bs2 * (-ax); }
-                } else {  // spatial_vector_bulkdmi_kernel!: kernels.jl:347-422
-                    const TermDev<T> &D = P.t[sb];
+            if (C.bd == 1) {  // bulkdmi_kernel!: kernels.jl:306-344
+                if (q == 0) { fby += C.bs0 * (-az_); fbz += C.bs0 * ay_; }
+                if (q == 1) { fby += C.bs0 * az_;    fbz += C.bs0 * (-ay_); }
+                if (q == 2) { fbx += C.bs1 * az_;    fbz += C.bs1 * (-ax_); }
+                if (q == 3) { fbx += C.bs1 * (-az_); fbz += C.bs1 * ax_; }
+                if (q == 4) { fbx += C.bs2 * (-ay_); fby += C.bs2 * ax_; }
+                if (q == 5) { fbx += C.bs2 * ay_;    fby += C.bs2 * (-ax_); }
+            } else if (C.bd == 2) {  // spatial_vector_bulkdmi_kernel!: kernels.jl:347-422
+                const T DI = bDI[q / 2], Dn = bDn[q];
+                if (DI != T(0) && Dn != T(0)) {
+                    const T De = T(2) * DI * Dn / (DI + Dn);
                     if (q < 2) {
-                        const T Dn = D.a[id];
-                        if (bDx != T(0) && Dn != T(0)) {
-                            const T De = T(2) * bDx * Dn / (bDx + Dn);
-                            const T a = q == 0 ? bs0 : -bs0;
-                            fby += De * (-a * az);
-                            fbz += De * (a * ay);
-                        }
+                        const T a = q == 0 ? C.bs0 : -C.bs0;
+                        fby += De * (-a * az_);
+                        fbz += De * (a * ay_);
                     } else if (q < 4) {
-                        const T Dn = D.b[id];
-                        if (bDy != T(0) && Dn != T(0)) {
-                            const T De = T(2) * bDy * Dn / (bDy + Dn);
-                            const T a = q == 2 ? bs1 : -bs1;
-                            fbx += De * (a * az);
-                            fbz += De * (-a * ax);
-                        }
+                        const T a = q == 2 ? C.bs1 : -C.bs1;
+                        fbx += De * (a * az_);
+                        fbz += De * (-a * ax_);
                     } else {
-                        const T Dn = D.c[id];
-                        if (bDz != T(0) && Dn != T(0)) {
-                            const T De = T(2) * bDz * Dn / (bDz + Dn);
-                            const T a = q == 4 ? bs2 : -bs2;
-                            fbx += De * (-a * ay);
-                            fby += De * (a * ax);
-                        }
+                        const T a = q == 4 ? C.bs2 : -C.bs2;
+                        fbx += De * (-a * ay_);
+                        fby += De * (a * ax_);
                     }
                 }
             }
-            if (id_on && q < 4) {  // interfacial_dmi_kernel!: kernels.jl:426-466
-                const T Dn = id_un ? iDI : P.t[si].a[id];
+            if (C.idm && q < 4) {  // interfacial_dmi_kernel!: kernels.jl:426-466
+                const T Dn = iDn[q];
                 if (Dn != T(0)) {
-                    const T De = id_un ? iDe : T(2) * iDI * Dn / (iDI + Dn);
-                    const T c = De * (q < 2 ? id0 : id1);
-                    if (q == 0) { fix += c * az;    fiz += c * (-ax); }
-                    if (q == 1) { fix += c * (-az); fiz += c * ax; }
-                    if (q == 2) { fiy += c * az;    fiz += c * (-ay); }
-                    if (q == 3) { fiy += c * (-az); fiz += c * ay; }
+                    const T De = C.idm == 1 ? C.iDe : T(2) * iDI * Dn / (iDI + Dn);
+                    const T c = De * (q < 2 ? C.id0 : C.id1);
+                    if (q == 0) { fix += c * az_;    fiz += c * (-ax_); }
+                    if (q == 1) { fix += c * (-az_); fiz += c * ax_; }
+                    if (q == 2) { fiy += c * az_;    fiz += c * (-ay_); }
+                    if (q == 3) { fiy += c * (-az_); fiz += c * ay_; }
                 }
             }
         }
     }
     const T Ms_inv = (Ms != T(0)) ? T(1.0) / Ms : T(0);
-    // local slots
-    T ahx = 0, ahy = 0, ahz = 0;  // uniaxial
-    if (P.slot[SL_ANIS] >= 0 && Ms != T(0)) {
-        const TermDev<T> &D = P.t[P.slot[SL_ANIS]];
-        T ax, ay, az;
-        if (D.kind == K_ANIS) {
-            ax = D.s[0];
-            ay = D.s[1];
-            az = D.s[2];
-        } else {
-            ax = D.b[I];
-            ay = D.b[N + I];
-            az = D.b[2 * N + I];
-        }
-        const T Ku = D.uniform ? D.s[3] : D.a[I];
-        const T Ms2 = T(2.0) / Ms;
+    T ahx = 0, ahy = 0, ahz = 0;  // uniaxial, kernels.jl:54-105 (2/Ms == 2 * (1/Ms) exactly)
+    if (C.an && Ms != T(0)) {
+        const T Ms2 = T(2.0) * Ms_inv;
         const T sa = mx * ax + my * ay + mz * az;
         ahx = Ku * sa * ax * Ms2;
         ahy = Ku * sa * ay * Ms2;
         ahz = Ku * sa * az * Ms2;
     }
-    T chx = 0, chy = 0, chz = 0;  // cubic
-    if (P.slot[SL_CUBIC] >= 0 && Ms != T(0)) {
-        const TermDev<T> &D = P.t[P.slot[SL_CUBIC]];
-        const T Kc = D.uniform ? D.s[9] : D.a[I];
+    T chx = 0, chy = 0, chz = 0;  // cubic, kernels.jl:110-137
+    if (C.cu && Ms != T(0)) {
+        const MMX_CONST T *ca = P.t[C.sc].s;
         const T Ms4 = T(4.0) * Kc / Ms;
-        const T mxp = D.s[0] * mx + D.s[1] * my + D.s[2] * mz;
-        const T myp = D.s[3] * mx + D.s[4] * my + D.s[5] * mz;
-        const T mzp = D.s[6] * mx + D.s[7] * my + D.s[8] * mz;
+        const T mxp = ca[0] * mx + ca[1] * my + ca[2] * mz;
+        const T myp = ca[3] * mx + ca[4] * my + ca[5] * mz;
+        const T mzp = ca[6] * mx + ca[7] * my + ca[8] * mz;
         const T mxp3 = mxp * mxp * mxp, myp3 = myp * myp * myp, mzp3 = mzp * mzp * mzp;
-        chx = Ms4 * (mxp3 * D.s[0] + myp3 * D.s[3] + mzp3 * D.s[6]);
-        chy = Ms4 * (mxp3 * D.s[1] + myp3 * D.s[4] + mzp3 * D.s[7]);
-        chz = Ms4 * (mxp3 * D.s[2] + myp3 * D.s[5] + mzp3 * D.s[8]);
+        chx = Ms4 * (mxp3 * ca[0] + myp3 * ca[3] + mzp3 * ca[6]);
+        chy = Ms4 * (mxp3 * ca[1] + myp3 * ca[4] + mzp3 * ca[7]);
+        chz = Ms4 * (mxp3 * ca[2] + myp3 * ca[5] + mzp3 * ca[8]);
     }
-    T zhx = 0, zhy = 0, zhz = 0;  // Zeeman
-    if (P.slot[SL_ZEE] >= 0) {
-        const TermDev<T> &D = P.t[P.slot[SL_ZEE]];
-        if (D.kind == K_ZEEMAN) {
-            zhx = D.a[I];
-            zhy = D.b[I];
-            zhz = D.c[I];
-        } else {
-            zhx = D.s[0];
-            zhy = D.s[1];
-            zhz = D.s[2];
-        }
-    }
-    T thx = 0, thy = 0, thz = 0;  // thermal
-    if (P.slot[SL_THERM] >= 0 && Ms > T(0)) {
-        const TermDev<T> &D = P.t[P.slot[SL_THERM]];
-        const T Tl = (D.uniform ? D.s[1] : D.a[I]) + D.s[0];
+    T thx = 0, thy = 0, thz = 0;  // thermal, kernels.jl:469-494
+    if (C.th && Ms > T(0)) {
         const T scale = (T)sqrt((T)dyn.thermal_factor * Tl / Ms);
         const uint64_t base = 3 * (uint64_t)I;
-        thx = (T)philox_normal(P.seed, dyn.noise_step, base) * scale;
-        thy = (T)philox_normal(P.seed, dyn.noise_step, base + 1) * scale;
-        thz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
+        thx = (T)philox_normal(C.seed, dyn.noise_step, base) * scale;
+        thy = (T)philox_normal(C.seed, dyn.noise_step, base + 1) * scale;
+        thz = (T)philox_normal(C.seed, dyn.noise_step, base + 2) * scale;
     }
     Hx = Hy = Hz = T(0);
 #pragma unroll
     for (int q = 0; q < MAX_TERMS; q++) {
-        if (q >= P.nterms) break;
+        if (q >= C.nterms) break;
         const int sk = P.ord[q];
         T hx, hy, hz;
         if (sk == SL_EXCH) { hx = ex_on ? fex * Ms_inv : T(0); hy = ex_on ? fey * Ms_inv : T(0); hz = ex_on ? fez * Ms_inv : T(0); }
@@ -638,38 +761,40 @@ __device__ __forceinline__ void cell_field_slots(const FieldParams<T> &P, const 
 
 // Epilogue of the specialised kernels: MODE is EPI_STAGE, EPI_RK4_FINAL or EPI_ERR.
 template <class T, int MODE>
-__device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I, int i,
-                                             int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz, double &red)
+__device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I,
+                                            int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
+                                            double &red)
 {
 #pragma clang fp contract(off)
     const int N = (int)P.N;
-    // operands of the stage combination, loaded before the field so their
-    // latency overlaps the neighbour loads
-    T y0v[3] = {0, 0, 0}, kpv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const T alpha_ = P.alpha, llg_a_ = P.llg_a;
+    const int prec_ = P.precession;
+    // operands of the stage combination, loaded with the field's loads
+    T y0v[3] = {0, 0, 0}, kpv[3][3];
     if constexpr (MODE != EPI_ERR) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) y0v[c] = E.y0[c * N + I];
+        for (int c = 0; c < 3; c++) y0v[c] = gld(E.y0, c * N + I);
     }
     if constexpr (MODE == EPI_RK4_FINAL) {
 #pragma unroll
         for (int q = 0; q < 3; q++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) kpv[q][c] = E.kp[q][c * N + I];
+            for (int c = 0; c < 3; c++) kpv[q][c] = gld(E.kp[q], c * N + I);
     }
     T mx, my, mz, Hx, Hy, Hz;
     uint32_t msk;
-    cell_field_slots(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk);
+    cell_field_fast(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk);
     // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
     T kk[3];
     if ((msk >> 6) & 1u) {
         kk[0] = kk[1] = kk[2] = T(0);
     } else {
-        const T alpha = P.alpha, a = P.llg_a;
+        const T alpha = alpha_, a = llg_a_;
         const T mm = mx * mx + my * my + mz * mz;
         const T mh = mx * Hx + my * Hy + mz * Hz;
         const T h1 = mm * Hx - mh * mx, h2 = mm * Hy - mh * my, h3 = mm * Hz - mh * mz;
         T f1 = 0, f2 = 0, f3 = 0;
-        if (P.precession) {
+        if (prec_) {
             f1 = -mz * h2 + my * h3;
             f2 = mz * h1 - mx * h3;
             f3 = -my * h1 + mx * h2;
@@ -679,28 +804,30 @@ __device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn 
         kk[2] = a * (f3 - h3 * alpha);
     }
     if (E.kstore) {
-        E.kstore[I] = kk[0];
-        E.kstore[N + I] = kk[1];
-        E.kstore[2 * N + I] = kk[2];
+        gst(E.kstore, I, kk[0]);
+        gst(E.kstore, N + I, kk[1]);
+        gst(E.kstore, 2 * N + I, kk[2]);
     }
     if constexpr (MODE == EPI_ERR) {
+        const T mv[3] = {mx, my, mz};
         double mval = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const int o = c * N + I;
             double err = 0;
 #pragma unroll
             for (int q = 0; q < 6; q++)
-                if (q < E.np) err = (q == 0) ? E.cp[0] * (double)E.kp[0][o] : err + E.cp[q] * (double)E.kp[q][o];
+                if (q < E.np) {
+                    const double kq = (double)gld(E.kp[q], c * N + I);
+                    err = (q == 0) ? E.cp[0] * kq : err + E.cp[q] * kq;
+                }
             err = err + E.cself * (double)kk[c];
-            mval = pmax(mval, fabs(err / (1.0 + fabs((double)m[o]))));
+            mval = pmax(mval, fabs(err / (1.0 + fabs((double)mv[c]))));
         }
         red = pmax(red, mval);
     } else {
         T y[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const int o = c * N + I;
             if constexpr (MODE == EPI_RK4_FINAL) {
                 const T s = kpv[0][c] + T(2) * kpv[1][c] + T(2) * kpv[2][c] + kk[c];
                 y[c] = y0v[c] + (T)E.cself * s;
@@ -708,7 +835,7 @@ __device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn 
                 T s = y0v[c];
 #pragma unroll
                 for (int q = 0; q < 6; q++)
-                    if (q < E.np) s = s + (T)E.cp[q] * E.kp[q][o];
+                    if (q < E.np) s = s + (T)E.cp[q] * gld(E.kp[q], c * N + I);
                 y[c] = s + (T)E.cself * kk[c];
             }
         }
@@ -721,15 +848,15 @@ __device__ __forceinline__ void finish_slots(const FieldParams<T> &P, const Dyn 
                 y[2] *= len;
             }
         }
-        E.yout[I] = y[0];
-        E.yout[N + I] = y[1];
-        E.yout[2 * N + I] = y[2];
+        gst(E.yout, I, y[0]);
+        gst(E.yout, N + I, y[1]);
+        gst(E.yout, 2 * N + I, y[2]);
     }
 }
 
 // llg_rhs_kernel!: src/llg/llg.jl:6-40
 template <class T>
-__device__ __forceinline__ void llg_rhs(const FieldParams<T> &P, int64_t I, T mx, T my, T mz, T hx,
+__device__ __forceinline__ void llg_rhs(const MMX_CONST FieldParams<T> &P, int64_t I, T mx, T my, T mz, T hx,
                                         T hy, T hz, T &dx, T &dy, T &dz)
 {
 #pragma clang fp contract(off)
@@ -761,7 +888,7 @@ __device__ __forceinline__ void llg_rhs(const FieldParams<T> &P, int64_t I, T mx
 //   (dF/dm)^T l = a (p x l - Pm^T q),   Pm^T v = 2 m (h.v) - h (m.v) - (m.h) v
 // with q = m x l + alpha l; the m x terms drop when precession is off.
 template <class T>
-__device__ __forceinline__ void llg_q(const FieldParams<T> &P, T mx, T my, T mz, T lx, T ly, T lz, T &qx, T &qy,
+__device__ __forceinline__ void llg_q(const MMX_CONST FieldParams<T> &P, T mx, T my, T mz, T lx, T ly, T lz, T &qx, T &qy,
                                       T &qz)
 {
     const T alpha = P.alpha;
@@ -776,7 +903,7 @@ __device__ __forceinline__ void llg_q(const FieldParams<T> &P, T mx, T my, T mz,
 }
 
 template <class T>
-__device__ __forceinline__ void llg_vjp_h(const FieldParams<T> &P, T mx, T my, T mz, T lx, T ly, T lz, T &ux,
+__device__ __forceinline__ void llg_vjp_h(const MMX_CONST FieldParams<T> &P, T mx, T my, T mz, T lx, T ly, T lz, T &ux,
                                           T &uy, T &uz)
 {
     const T alpha = P.alpha;
@@ -791,7 +918,7 @@ __device__ __forceinline__ void llg_vjp_h(const FieldParams<T> &P, T mx, T my, T
 }
 
 template <class T>
-__device__ __forceinline__ void llg_vjp_m(const FieldParams<T> &P, T mx, T my, T mz, T hx, T hy, T hz, T lx, T ly,
+__device__ __forceinline__ void llg_vjp_m(const MMX_CONST FieldParams<T> &P, T mx, T my, T mz, T hx, T hy, T hz, T lx, T ly,
                                           T lz, T &rx, T &ry, T &rz)
 {
     const T alpha = P.alpha;
@@ -830,7 +957,7 @@ __device__ __forceinline__ void idmi_cross(int q, T vx, T vy, T vz, T &wx, T &wy
 
 // EPI_ADJ for cell I (see AdjIO): ut is this pass's input vector.
 template <class T>
-__device__ __forceinline__ void adj_cell(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I, int i,
+__device__ __forceinline__ void adj_cell(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I, int i,
                                          int j, int k, const T *__restrict__ ut, T hdx, T hdy, T hdz)
 {
     const int64_t N = P.N;
@@ -849,7 +976,7 @@ __device__ __forceinline__ void adj_cell(const FieldParams<T> &P, const Dyn &dyn
     T mux = 0, muy = 0, muz = 0;
     if (Ms > T(0)) {
         for (int tt = 0; tt < P.nterms; tt++) {
-            const TermDev<T> &D = P.t[tt];
+            const MMX_CONST TermDev<T> &D = P.t[tt];
             if (D.kind != K_CUBIC || !((E.term_mask >> tt) & 1u)) continue;
             // J = (12 Kc / mu0Ms) sum_r (Y.a_r)^2 a_r a_r^T  (cubic_anisotropy_kernel!, kernels.jl:110-137)
             const T Kc = D.uniform ? D.s[9] : D.a[I];
@@ -893,7 +1020,7 @@ __device__ __forceinline__ void adj_cell(const FieldParams<T> &P, const Dyn &dyn
     for (int tt = 0; tt < P.nterms; tt++) {
         T *g = A.g[tt];
         if (!g) continue;
-        const TermDev<T> &D = P.t[tt];
+        const MMX_CONST TermDev<T> &D = P.t[tt];
         T gv = 0;
         switch (D.kind) {
         case K_ANIS:
@@ -973,7 +1100,7 @@ __device__ __forceinline__ void adj_cell(const FieldParams<T> &P, const Dyn &dyn
 // The fused last step of one RHS evaluation for cell I.  `red` accumulates the
 // thread's reduction value (ERR: running max; ENERGY: sum of the masked terms).
 template <class T>
-__device__ __forceinline__ void finish_cell(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
+__device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
                                             int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
                                             double &red)
 {

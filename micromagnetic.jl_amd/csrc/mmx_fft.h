@@ -83,64 +83,76 @@ __device__ __forceinline__ void dftR(cplx<T> *a)
     else if constexpr (R == 2) dft2<T, INV>(a);
 }
 
-// One Stockham stage of radix Rs at span Ns.  Butterfly jb = j + q*TPS
-// (q < EPT/Rs) reads v[q + r*EPT/Rs] = x[jb + r*L/Rs], applies W_{Ns Rs}^{r (jb mod Ns)},
-// and writes y[(jb/Ns) Ns Rs + jb mod Ns + r Ns].  The last stage (Ns Rs = L)
-// leaves its outputs in v at the input positions.
-template <class T, int L, int E, int Rs, int Ns, bool INV>
-__device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw,
+// One Stockham stage of radix Rs at span Ns over NC sequences held by the same
+// threads (v[c*EPT + m]; sequence c exchanges through lds + c*cs).  Butterfly
+// jb = j + q*TPS (q < EPT/Rs) reads v[q + r*EPT/Rs] = x[jb + r*L/Rs], applies
+// W_{Ns Rs}^{r (jb mod Ns)}, and writes y[(jb/Ns) Ns Rs + jb mod Ns + r Ns].  The
+// last stage (Ns Rs = L) leaves its outputs in v at the input positions.  The
+// NC sequences share each exchange's barriers and twiddles.
+template <class T, int L, int E, int Rs, int Ns, bool INV, int NC>
+__device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
                                           int twsh)
 {
     constexpr int EPT = FftShape<L, E>::EPT, TPS = FftShape<L, E>::TPS;
     constexpr int NQ = EPT / Rs;
     constexpr bool LAST = (Ns * Rs == L);
     constexpr int TWSTEP = L / (Ns * Rs);
-    cplx<T> o[EPT];
+    cplx<T> o[LAST ? NC * EPT : 1];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         const int jb = j + q * TPS;
         const int kk = jb & (Ns - 1);
-        cplx<T> a[Rs];
-#pragma unroll
-        for (int r = 0; r < Rs; r++) a[r] = v[q + r * NQ];
+        cplx<T> w[Rs];
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < Rs; r++) {
-                cplx<T> w = tw[(r * kk * TWSTEP) << twsh];
-                if (INV) w.y = -w.y;
-                a[r] = cmul(a[r], w);
+                w[r] = tw[(r * kk * TWSTEP) << twsh];
+                if (INV) w[r].y = -w[r].y;
             }
         }
-        dftR<T, Rs, INV>(a);
-        if constexpr (LAST) {
 #pragma unroll
-            for (int r = 0; r < Rs; r++) o[q + r * NQ] = a[r];
-        } else {
-            const int base = (jb / Ns) * Ns * Rs + kk;
+        for (int c = 0; c < NC; c++) {
+            cplx<T> a[Rs];
 #pragma unroll
-            for (int r = 0; r < Rs; r++) lds[base + r * Ns] = a[r];
+            for (int r = 0; r < Rs; r++) a[r] = v[c * EPT + q + r * NQ];
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < Rs; r++) a[r] = cmul(a[r], w[r]);
+            }
+            dftR<T, Rs, INV>(a);
+            if constexpr (LAST) {
+#pragma unroll
+                for (int r = 0; r < Rs; r++) o[c * EPT + q + r * NQ] = a[r];
+            } else {
+                const int base = (jb / Ns) * Ns * Rs + kk;
+#pragma unroll
+                for (int r = 0; r < Rs; r++) lds[c * cs + base + r * Ns] = a[r];
+            }
         }
     }
     if constexpr (LAST) {
 #pragma unroll
-        for (int m = 0; m < EPT; m++) v[m] = o[m];
+        for (int m = 0; m < NC * EPT; m++) v[m] = o[m];
     } else {
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < EPT; m++) v[m] = lds[j + m * TPS];
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int m = 0; m < EPT; m++) v[c * EPT + m] = lds[c * cs + j + m * TPS];
         __syncthreads();
     }
 }
 
-template <class T, int L, int E, int Ns, bool INV>
-__device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int twsh)
+template <class T, int L, int E, int Ns, bool INV, int NC>
+__device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
+                                           int twsh)
 {
     if constexpr (Ns < L) {
         constexpr int rem = L / Ns;
         constexpr int EPT = FftShape<L, E>::EPT;
         constexpr int Rs = rem >= EPT ? EPT : rem;
-        fft_stage<T, L, E, Rs, Ns, INV>(v, j, lds, tw, twsh);
-        fft_stages<T, L, E, Ns * Rs, INV>(v, j, lds, tw, twsh);
+        fft_stage<T, L, E, Rs, Ns, INV, NC>(v, j, lds, cs, tw, twsh);
+        fft_stages<T, L, E, Ns * Rs, INV, NC>(v, j, lds, cs, tw, twsh);
     }
 }
 
@@ -150,7 +162,15 @@ __device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, cons
 template <class T, int L, bool INV, int E = 8>
 __device__ __forceinline__ void fft_reg(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int lgLmax)
 {
-    if constexpr (L > 1) fft_stages<T, L, E, 1, INV>(v, j, lds, tw, lgLmax - FftShape<L, E>::LG);
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, 1>(v, j, lds, 0, tw, lgLmax - FftShape<L, E>::LG);
+}
+
+// NC sequences per thread (v[c*EPT + m]), exchanging through lds + c*cs.
+template <class T, int L, bool INV, int E, int NC>
+__device__ __forceinline__ void fft_reg_multi(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
+                                              int lgLmax)
+{
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC>(v, j, lds, cs, tw, lgLmax - FftShape<L, E>::LG);
 }
 
 }  // namespace mmx

@@ -147,6 +147,15 @@ __global__ void __launch_bounds__(Shape<L>::BY) k_coly(DemagGeom G, const cplx<T
 // rounding residue of the Newell sums that is not odd; Lline holds these three
 // even transforms: [Pz2 | Py2 | Kx].
 // ---------------------------------------------------------------------------
+// MMX_P3_MC: the three components' transforms share each exchange's barriers
+// (one LDS buffer per component); MMX_P3_PF: the tensor entries are read
+// before the forward transform so that their latency overlaps it.
+#ifndef MMX_P3_MC
+#define MMX_P3_MC 1
+#endif
+#ifndef MMX_P3_PF
+#define MMX_P3_PF 1
+#endif
 template <class T, int L, bool FWD_ONLY>
 __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__restrict__ B,
                                                          const T *__restrict__ Kt, const T *__restrict__ Lline,
@@ -156,7 +165,10 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
     constexpr int EPT = FftShape<L, 4>::EPT, TPS = FftShape<L, 4>::TPS, TZ = S::TZ;
     constexpr int LS = L + 1;
     constexpr int NC = FWD_ONLY ? 1 : 3;
-    __shared__ cplx<T> lds[2 * TZ * LS];
+    // one exchange buffer per component when the three transforms share barriers
+    constexpr int CS = 2 * TZ * LS;
+    constexpr int NB = (MMX_P3_MC && !FWD_ONLY && 3 * CS * sizeof(cplx<T>) <= 65536) ? 3 : 1;
+    __shared__ cplx<T> lds[NB * CS];
     const int half = threadIdx.x / (TZ * TPS);
     const int tl = threadIdx.x % (TZ * TPS);
     const int t = tl % TZ, j = tl / TZ;
@@ -178,9 +190,8 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
             v[c][m] = (ok && z < n_in) ? col[c * cs_b + z * zs_b] : cplx<T>{0, 0};
         }
     cplx<T> *buf = lds + (half * TZ + t) * LS;
-#pragma unroll
-    for (int c = 0; c < NC; c++) fft_reg<T, L, false, 4>(v[c], j, buf, tw, G.lgLmax);
     if constexpr (FWD_ONLY) {
+        fft_reg<T, L, false, 4>(v[0], j, buf, tw, G.lgLmax);
         if (!ok) return;
 #pragma unroll
         for (int m = 0; m < EPT; m++) col[(int64_t)(j + m * TPS) * zs_b] = v[0][m];
@@ -190,25 +201,48 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
         const T sy = (ky <= Py / 2) ? T(1) : T(-1);
         const int64_t cs = (int64_t)G.Pz2 * G.Py2 * KS;  // component stride of Kt
         const int kxs = ok ? kx : 0;
-        const T lyz = Lline[G.Pz2 + G.Py2 + kxs];
-        const T lxz = Lline[G.Pz2 + kyf];
+        // tensor entries of this thread's kz (read before the transform when
+        // MMX_P3_PF so that their latency overlaps it)
+        T kxx[EPT], kyy[EPT], kzz[EPT], kxy[EPT], kxz[EPT], kyz[EPT];
+        auto load_tensor = [&]() {
+            const T lyz = Lline[G.Pz2 + G.Py2 + kxs];
+            const T lxz = Lline[G.Pz2 + kyf];
+#pragma unroll
+            for (int m = 0; m < EPT; m++) {
+                const int kz = j + m * TPS;
+                const int kzf = (kz <= L / 2) ? kz : L - kz;
+                const T sz = (kz <= L / 2) ? T(1) : T(-1);
+                const T *kp = Kt + ((int64_t)kzf * G.Py2 + kyf) * KS + kxs;
+                kxx[m] = kp[0];
+                kyy[m] = kp[cs];
+                kzz[m] = kp[2 * cs];
+                kxy[m] = sy * kp[3 * cs] + Lline[kzf];
+                kxz[m] = sz * kp[4 * cs] + lxz;
+                kyz[m] = sy * sz * kp[5 * cs] + lyz;
+            }
+        };
+        if constexpr (MMX_P3_PF) load_tensor();
+        if constexpr (NB == 3) {
+            fft_reg_multi<T, L, false, 4, 3>(&v[0][0], j, buf, CS, tw, G.lgLmax);
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; c++) fft_reg<T, L, false, 4>(v[c], j, buf, tw, G.lgLmax);
+        }
+        if constexpr (!MMX_P3_PF) load_tensor();
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
-            const int kz = j + m * TPS;
-            const int kzf = (kz <= L / 2) ? kz : L - kz;
-            const T sz = (kz <= L / 2) ? T(1) : T(-1);
-            const T *kp = Kt + ((int64_t)kzf * G.Py2 + kyf) * KS + kxs;
-            const T xx = kp[0], yy = kp[cs], zz = kp[2 * cs];
-            const T xy = sy * kp[3 * cs] + Lline[kzf];
-            const T xz = sz * kp[4 * cs] + lxz;
-            const T yz = sy * sz * kp[5 * cs] + lyz;
+            const T xx = kxx[m], yy = kyy[m], zz = kzz[m], xy = kxy[m], xz = kxz[m], yz = kyz[m];
             const cplx<T> Mx = v[0][m], My = v[1][m], Mz = v[2][m];
             v[0][m] = {xx * Mx.x + xy * My.x + xz * Mz.x, xx * Mx.y + xy * My.y + xz * Mz.y};
             v[1][m] = {xy * Mx.x + yy * My.x + yz * Mz.x, xy * Mx.y + yy * My.y + yz * Mz.y};
             v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
         }
+        if constexpr (NB == 3) {
+            fft_reg_multi<T, L, true, 4, 3>(&v[0][0], j, buf, CS, tw, G.lgLmax);
+        } else {
 #pragma unroll
-        for (int c = 0; c < NC; c++) fft_reg<T, L, true, 4>(v[c], j, buf, tw, G.lgLmax);
+            for (int c = 0; c < NC; c++) fft_reg<T, L, true, 4>(v[c], j, buf, tw, G.lgLmax);
+        }
         if (!ok) return;
         const int nz = G.nz;
 #pragma unroll
@@ -234,15 +268,21 @@ struct VMode {
 };
 
 template <class T, int V>
-__device__ __forceinline__ void epilogue(const FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I, int i,
-                                         int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
+__device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
+                                         int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
 {
     if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
-    else finish_slots<T, VMode<V>::mode>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
+    else finish_fast<T, VMode<V>::mode>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
 }
 
+// at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
+// latency-bound, and the register budget this sets costs no spills
+#ifndef MMX_P5_WPE
+#define MMX_P5_WPE 3
+#endif
+#define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu(MMX_P5_WPE)))
 template <class T, int H, int V>
-__global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
+__global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
                                                       const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
                                                       const T *__restrict__ m, const cplx<T> *__restrict__ tw)
 {
@@ -294,10 +334,11 @@ __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> 
 #pragma unroll
     for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
     __syncthreads();
-    const FieldParams<T> &P = *Pp;
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     double red = 0;
+    // (row, x) of this thread's cell, advanced without dividing
+    int rr = (int)threadIdx.x / nx, x = (int)threadIdx.x - rr * nx;
     for (int idx = threadIdx.x; idx < nrows * nx; idx += blockDim.x) {
-        const int rr = idx / nx, x = idx - rr * nx;
         const int y = y0, z = z0 + rr;
         const int rw = z * ny + y;
         const cplx<T> h0 = lds[(0 * RX + rr) * HS + (x >> 1)];
@@ -307,6 +348,11 @@ __global__ void __launch_bounds__(Shape<H>::B5) k_p5(DemagGeom G, const cplx<T> 
         const T hy = (x & 1) ? h1.y : h1.x;
         const T hz = (x & 1) ? h2.y : h2.x;
         epilogue<T, V>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
+        x += blockDim.x;
+        while (x >= nx) {
+            x -= nx;
+            rr++;
+        }
     }
     if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
@@ -318,15 +364,16 @@ template <class T, int V>
 __global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
                                                  const T *__restrict__ m)
 {
-    const FieldParams<T> &P = *Pp;
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     double red = 0;
-    const int nx = P.nx, ny = P.ny;
-    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < P.N; I += (int64_t)gridDim.x * BLOCK) {
-        int i = (int)(I % nx);
-        int64_t q = I / nx;
-        int j = (int)(q % ny);
-        int k = (int)(q / ny);
-        epilogue<T, V>(P, dyn, E, I, i, j, k, m, T(0), T(0), T(0), red);
+    const unsigned nx = P.nx, ny = P.ny;
+    const int N = (int)P.N;  // meshes hold fewer than 2^31 cells
+    for (int I = blockIdx.x * BLOCK + threadIdx.x; I < N; I += gridDim.x * BLOCK) {
+        const unsigned q = (unsigned)I / nx;
+        const int i = (int)((unsigned)I - q * nx);
+        const unsigned k = q / ny;
+        const int j = (int)(q - k * ny);
+        epilogue<T, V>(P, dyn, E, I, i, j, (int)k, m, T(0), T(0), T(0), red);
     }
     if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
@@ -353,19 +400,21 @@ __global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ 
     const int lo = xcd * per, hi = min(nitems, lo + per);
     for (int it = lo + r; it < hi; it += Gx) {
         const int2 w = items[it];
-        const FieldParams<T> &P = *Ps[w.x];
-        const BatchStage<T> &S = st[w.x];
+        const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Ps[w.x];
+        const Epi<T> E = st[w.x].E;
+        const T *m = st[w.x].m;
         const Dyn d = dyns[w.x];
-        const int nx = P.nx, ny = P.ny;
+        const unsigned nx = P.nx, ny = P.ny;
+        const int N = (int)P.N;
         double red = 0;
         for (int c = 0; c < CPT; c++) {
-            const int64_t I = (int64_t)w.y + c * BLOCK + threadIdx.x;
-            if (I < P.N) {
-                const int i = (int)(I % nx);
-                const int64_t q = I / nx;
-                const int j = (int)(q % ny);
-                const int k = (int)(q / ny);
-                epilogue<T, V>(P, d, S.E, I, i, j, k, S.m, T(0), T(0), T(0), red);
+            const int I = w.y + c * BLOCK + threadIdx.x;
+            if (I < N) {
+                const unsigned q = (unsigned)I / nx;
+                const int i = (int)((unsigned)I - q * nx);
+                const unsigned k = q / ny;
+                const int j = (int)(q - k * ny);
+                epilogue<T, V>(P, d, E, I, i, j, (int)k, m, T(0), T(0), T(0), red);
             }
         }
     }
@@ -539,7 +588,7 @@ __global__ void k_neb_stage(const FieldParams<T> *__restrict__ Pp, int64_t N, co
                             T *__restrict__ yout, double *__restrict__ partial)
 {
 #pragma clang fp contract(off)
-    const FieldParams<T> &P = *Pp;
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     const int q = blockIdx.y;
     const int64_t V = 3 * N, base = q * V;
     const double nrm = coef[2 * q], sq = coef[2 * q + 1];
@@ -600,7 +649,7 @@ __global__ void k_neb_stage(const FieldParams<T> *__restrict__ Pp, int64_t N, co
 template <class T>
 __global__ void k_nbmask(const FieldParams<T> *__restrict__ Pp, uint8_t *__restrict__ out)
 {
-    const FieldParams<T> &P = *Pp;
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < P.N; I += (int64_t)gridDim.x * BLOCK) {
         const int i = (int)(I % P.nx);
         const int64_t q = I / P.nx;
@@ -675,7 +724,7 @@ __global__ void k_sd_update(const FieldParams<T> *__restrict__ Pp, const T *__re
                             T *__restrict__ mpre, const T *__restrict__ h, const double *__restrict__ taup)
 {
 #pragma clang fp contract(off)
-    const FieldParams<T> &P = *Pp;
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     const int64_t N = P.N;
     const T tau = (T)taup[0];
     for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
@@ -722,7 +771,7 @@ template <class T>
 __global__ void k_adj_pre(const FieldParams<T> *__restrict__ Pp, const T *__restrict__ Y, const T *__restrict__ a,
                           T *__restrict__ ut)
 {
-    const FieldParams<T> &P = *Pp;
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     const int64_t N = P.N;
     for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
         const T Ms = P.mu0Ms[I];
