@@ -313,6 +313,34 @@ struct Engine : CtxBase {
             P.slot[sk] = i;
             P.ord[i] = sk;
         }
+        // packed switches of the slot form
+        {
+            uint32_t f = (P.px ? FL_PX : 0u) | (P.py ? FL_PY : 0u) | (P.pz ? FL_PZ : 0u);
+            auto kind_at = [&](int sk) { return P.slot[sk] >= 0 ? C->terms[P.slot[sk]].kind : (int)K_NONE; };
+            auto uni_at = [&](int sk) { return P.slot[sk] >= 0 ? C->terms[P.slot[sk]].uniform : 1; };
+            const int ke = kind_at(SL_EXCH), kb = kind_at(SL_DMIB), ki = kind_at(SL_DMII), ka = kind_at(SL_ANIS);
+            const int kc = kind_at(SL_CUBIC), kz = kind_at(SL_ZEE);
+            f |= (uint32_t)(ke == K_NONE ? 0 : (ke == K_EXCH_S ? 2 : 1)) << FL_EX_SHIFT;
+            f |= (uint32_t)(kb == K_NONE ? 0 : (kb == K_DMI_BULK_S ? 2 : 1)) << FL_BD_SHIFT;
+            f |= (uint32_t)(ki == K_NONE ? 0 : (uni_at(SL_DMII) ? 1 : 2)) << FL_ID_SHIFT;
+            if (ka != K_NONE) {
+                f |= FL_AN;
+                if (ka == K_ANIS_S) f |= FL_AN_AXIS;
+                if (!uni_at(SL_ANIS)) f |= FL_AN_KU;
+            }
+            if (kc != K_NONE) {
+                f |= FL_CU;
+                if (!uni_at(SL_CUBIC)) f |= FL_CU_KC;
+            }
+            f |= (uint32_t)(kz == K_NONE ? 0 : (kz == K_ZEEMAN ? 2 : 1)) << FL_ZE_SHIFT;
+            if (P.slot[SL_THERM] >= 0) f |= FL_TH;
+            if (C->precession) f |= FL_PREC;
+            f |= (uint32_t)P.nterms << FL_NT_SHIFT;
+            P.flags = f;
+            uint32_t o = 0;
+            for (int i = 0; i < P.nterms; i++) o |= (uint32_t)P.ord[i] << (3 * i);
+            P.ordp = o;
+        }
         // -gamma/(1+alpha^2) rounded as the kernel computes it (llg.jl:18)
         volatile T al = (T)C->alpha;
         T den = T(1) + al * al;
@@ -1376,8 +1404,8 @@ struct BatchEngine : CtxBase {
         st.alloc(h.size());
         HIPCHECK(hipMemcpy(st.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
         for (int s = 0; s < S; s++) {
-            const int mode = (kind == MMX_RK4 && s == 3) ? EPI_RK4_FINAL : EPI_STAGE;
-            V[s] = batch_variant(mode, fast ? 1 : 0);
+            const Epi<T> &x = h[(size_t)s * M].E;  // every member's stage s has the same mode and np
+            V[s] = batch_variant(x.mode, fast ? 1 : 0, x.np);
         }
         dyns.alloc(M);
         upload_dyn();

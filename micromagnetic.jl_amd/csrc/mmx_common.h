@@ -79,6 +79,22 @@ struct FieldParams {
     int slot[8];
     int ord[MAX_TERMS];
     T llg_a;  // -gamma / (1 + alpha^2), src/llg/llg.jl:18
+    // the slot form's switches and insertion order packed into two words
+    // (FL_* bits below; ordp holds ord[q] in bits 3q..3q+2)
+    uint32_t flags, ordp;
+};
+
+enum SlotFlags : uint32_t {
+    FL_PX = 1u << 0, FL_PY = 1u << 1, FL_PZ = 1u << 2,
+    FL_EX_SHIFT = 3,   // 2 bits: 0 none, 1 uniform, 2 spatial A
+    FL_BD_SHIFT = 5,   // 2 bits: bulk DMI
+    FL_ID_SHIFT = 7,   // 2 bits: interfacial DMI
+    FL_AN = 1u << 9, FL_AN_AXIS = 1u << 10, FL_AN_KU = 1u << 11,
+    FL_CU = 1u << 12, FL_CU_KC = 1u << 13,
+    FL_ZE_SHIFT = 14,  // 2 bits: 0 none, 1 uniform, 2 per cell
+    FL_TH = 1u << 16,
+    FL_PREC = 1u << 18,
+    FL_NT_SHIFT = 20,  // 4 bits: number of terms
 };
 
 enum SlotKind : int { SL_EXCH = 0, SL_DMIB, SL_DMII, SL_ANIS, SL_CUBIC, SL_ZEE, SL_DEMAG, SL_THERM };

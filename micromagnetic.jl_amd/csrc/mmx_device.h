@@ -422,125 +422,59 @@ __device__ __forceinline__ bool local_term(const MMX_CONST TermDev<T> &D, const 
 
 // ---- slot form ---------------------------------------------------------------
 // When every kind of term appears at most once, the kernels take this form.
-// SlotCfg gathers what the cell loop needs from the parameter block once per
-// block (wave-uniform values, kept in scalar registers); a cell then issues all
-// its loads (own values, six neighbours, spatial parameters) before any
-// arithmetic, and the field of each slot is added in insertion order exactly
-// as cell_field adds the terms.
+// SlotCfg holds what every cell needs from the parameter block (wave-uniform
+// values that the compiler keeps in scalar registers across the cell loop);
+// the switches and the insertion order come packed (FieldParams::flags,
+// ::ordp), and what only spatial or rarely used terms need is read inside
+// their branches.  A cell issues its loads (own values, six neighbours,
+// spatial parameters) before its arithmetic, and the field of each slot is
+// added in insertion order exactly as cell_field adds the terms.
 template <class T>
 struct SlotCfg {
-    int nx, ny, nz, N, px, py, pz;
+    uint32_t fl, ordp;
+    int nx, ny, nz, N;
     const T *mu0Ms;
     const uint8_t *nbm;
-    int ex;  // exchange: 0 none, 1 uniform, 2 spatial A
+    int se, sb, si, sa, sc, sz, st;
     T nab0, nab1, nab2;
-    const T *exA;
-    int bd;  // bulk DMI: 0 none, 1 uniform, 2 spatial
-    T bs0, bs1, bs2;
-    const T *bDx, *bDy, *bDz;
-    int idm;  // interfacial DMI: 0 none, 1 uniform, 2 spatial D
     T id0, id1, iDe, iDu;
-    const T *iDa;
-    int an, anAx;  // uniaxial present; per-cell axes
     T ax, ay, az, Ku;
-    const T *KuA, *axA;
-    int cu, sc;
-    T Kc;
-    const T *KcA;
-    int ze;  // Zeeman: 0 none, 1 uniform, 2 per cell
     T zx, zy, zz;
-    const T *zA, *zB, *zC;
-    int th;
-    T T0, Tu;
-    const T *TA;
-    uint64_t seed;
-    int nterms;
-    T alpha, llg_a;
-    int precession;
 };
 
 template <class T>
 __device__ __forceinline__ SlotCfg<T> slot_cfg(const MMX_CONST FieldParams<T> &P)
 {
     SlotCfg<T> C;
+    C.fl = P.flags;
+    C.ordp = P.ordp;
     C.nx = P.nx;
     C.ny = P.ny;
     C.nz = P.nz;
     C.N = (int)P.N;
-    C.px = P.px;
-    C.py = P.py;
-    C.pz = P.pz;
     C.mu0Ms = P.mu0Ms;
     C.nbm = P.nbm;
-    const int se = P.slot[SL_EXCH], sb = P.slot[SL_DMIB], si = P.slot[SL_DMII], sa = P.slot[SL_ANIS];
-    const int sc = P.slot[SL_CUBIC], sz = P.slot[SL_ZEE], st = P.slot[SL_THERM];
-    {
-        const MMX_CONST TermDev<T> &D = P.t[se >= 0 ? se : 0];
-        C.ex = se < 0 ? 0 : (D.kind == K_EXCH_S ? 2 : 1);
-        C.nab0 = D.s[0];
-        C.nab1 = D.s[1];
-        C.nab2 = D.s[2];
-        C.exA = D.a;
-    }
-    {
-        const MMX_CONST TermDev<T> &D = P.t[sb >= 0 ? sb : 0];
-        C.bd = sb < 0 ? 0 : (D.kind == K_DMI_BULK_S ? 2 : 1);
-        C.bs0 = D.s[0];
-        C.bs1 = D.s[1];
-        C.bs2 = D.s[2];
-        C.bDx = D.a;
-        C.bDy = D.b;
-        C.bDz = D.c;
-    }
-    {
-        const MMX_CONST TermDev<T> &D = P.t[si >= 0 ? si : 0];
-        C.idm = si < 0 ? 0 : (D.uniform ? 1 : 2);
-        C.id0 = D.s[0];
-        C.id1 = D.s[1];
-        C.iDu = D.s[3];
-        C.iDe = D.s[4];
-        C.iDa = D.a;
-    }
-    {
-        const MMX_CONST TermDev<T> &D = P.t[sa >= 0 ? sa : 0];
-        C.an = sa >= 0;
-        C.anAx = D.kind == K_ANIS_S;
-        C.ax = D.s[0];
-        C.ay = D.s[1];
-        C.az = D.s[2];
-        C.Ku = D.s[3];
-        C.KuA = D.uniform ? nullptr : D.a;
-        C.axA = D.b;
-    }
-    {
-        const MMX_CONST TermDev<T> &D = P.t[sc >= 0 ? sc : 0];
-        C.cu = sc >= 0;
-        C.sc = sc >= 0 ? sc : 0;
-        C.Kc = D.s[9];
-        C.KcA = D.uniform ? nullptr : D.a;
-    }
-    {
-        const MMX_CONST TermDev<T> &D = P.t[sz >= 0 ? sz : 0];
-        C.ze = sz < 0 ? 0 : (D.kind == K_ZEEMAN ? 2 : 1);
-        C.zx = D.s[0];
-        C.zy = D.s[1];
-        C.zz = D.s[2];
-        C.zA = D.a;
-        C.zB = D.b;
-        C.zC = D.c;
-    }
-    {
-        const MMX_CONST TermDev<T> &D = P.t[st >= 0 ? st : 0];
-        C.th = st >= 0;
-        C.T0 = D.s[0];
-        C.Tu = D.s[1];
-        C.TA = D.uniform ? nullptr : D.a;
-    }
-    C.seed = P.seed;
-    C.nterms = P.nterms;
-    C.alpha = P.alpha;
-    C.llg_a = P.llg_a;
-    C.precession = P.precession;
+    C.se = max(P.slot[SL_EXCH], 0);
+    C.sb = max(P.slot[SL_DMIB], 0);
+    C.si = max(P.slot[SL_DMII], 0);
+    C.sa = max(P.slot[SL_ANIS], 0);
+    C.sc = max(P.slot[SL_CUBIC], 0);
+    C.sz = max(P.slot[SL_ZEE], 0);
+    C.st = max(P.slot[SL_THERM], 0);
+    C.nab0 = P.t[C.se].s[0];
+    C.nab1 = P.t[C.se].s[1];
+    C.nab2 = P.t[C.se].s[2];
+    C.id0 = P.t[C.si].s[0];
+    C.id1 = P.t[C.si].s[1];
+    C.iDu = P.t[C.si].s[3];
+    C.iDe = P.t[C.si].s[4];
+    C.ax = P.t[C.sa].s[0];
+    C.ay = P.t[C.sa].s[1];
+    C.az = P.t[C.sa].s[2];
+    C.Ku = P.t[C.sa].s[3];
+    C.zx = P.t[C.sz].s[0];
+    C.zy = P.t[C.sz].s[1];
+    C.zz = P.t[C.sz].s[2];
     return C;
 }
 
@@ -558,38 +492,39 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
                                                 T &mz, T &Hx, T &Hy, T &Hz, uint32_t &msk)
 {
 #pragma clang fp contract(off)
-    // every parameter read is unconditional and loop-invariant, so the
-    // compiler hoists them out of the callers' cell loops
     const SlotCfg<T> C = slot_cfg(P);
     const int N = C.N;
+    const uint32_t fl = C.fl;
+    const int ex = (fl >> FL_EX_SHIFT) & 3, bd = (fl >> FL_BD_SHIFT) & 3, idm = (fl >> FL_ID_SHIFT) & 3;
+    const int ze = (fl >> FL_ZE_SHIFT) & 3;
     // ---- loads ----
     msk = gld(C.nbm, I);
     const T Ms = gld(C.mu0Ms, I);
     mx = gld(m, I);
     my = gld(m, N + I);
     mz = gld(m, 2 * N + I);
-    const bool stencil = C.ex || C.bd || C.idm;
+    const bool stencil = ex || bd || idm;
     int nb[6];
     {
         const int nx = C.nx, ny = C.ny, nz = C.nz, sxy = nx * ny;
         int t;
         t = i - 1;
-        if (C.px && t < 0) t += nx;
+        if ((fl & FL_PX) && t < 0) t += nx;
         nb[0] = (t >= 0) ? I - i + t : I;
         t = i + 1;
-        if (C.px && t >= nx) t -= nx;
+        if ((fl & FL_PX) && t >= nx) t -= nx;
         nb[1] = (t < nx) ? I - i + t : I;
         t = j - 1;
-        if (C.py && t < 0) t += ny;
+        if ((fl & FL_PY) && t < 0) t += ny;
         nb[2] = (t >= 0) ? I + (t - j) * nx : I;
         t = j + 1;
-        if (C.py && t >= ny) t -= ny;
+        if ((fl & FL_PY) && t >= ny) t -= ny;
         nb[3] = (t < ny) ? I + (t - j) * nx : I;
         t = k - 1;
-        if (C.pz && t < 0) t += nz;
+        if ((fl & FL_PZ) && t < 0) t += nz;
         nb[4] = (t >= 0) ? I + (t - k) * sxy : I;
         t = k + 1;
-        if (C.pz && t >= nz) t -= nz;
+        if ((fl & FL_PZ) && t >= nz) t -= nz;
         nb[5] = (t < nz) ? I + (t - k) * sxy : I;
     }
     T nmx[6], nmy[6], nmz[6];
@@ -604,64 +539,75 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
         }
     }
     T exAI = 0, exAn[6] = {0, 0, 0, 0, 0, 0};
-    if (C.ex == 2) {
-        exAI = gld(C.exA, I);
+    if (ex == 2) {
+        const T *A = P.t[C.se].a;
+        exAI = gld(A, I);
 #pragma unroll
-        for (int q = 0; q < 6; q++) exAn[q] = gld(C.exA, nb[q]);
+        for (int q = 0; q < 6; q++) exAn[q] = gld(A, nb[q]);
     }
     T bDI[3] = {0, 0, 0}, bDn[6] = {0, 0, 0, 0, 0, 0};
-    if (C.bd == 2) {
-        bDI[0] = gld(C.bDx, I);
-        bDI[1] = gld(C.bDy, I);
-        bDI[2] = gld(C.bDz, I);
+    if (bd == 2) {
+        const T *Dx = P.t[C.sb].a, *Dy = P.t[C.sb].b, *Dz = P.t[C.sb].c;
+        bDI[0] = gld(Dx, I);
+        bDI[1] = gld(Dy, I);
+        bDI[2] = gld(Dz, I);
 #pragma unroll
-        for (int q = 0; q < 6; q++) bDn[q] = gld(q < 2 ? C.bDx : (q < 4 ? C.bDy : C.bDz), nb[q]);
+        for (int q = 0; q < 6; q++) bDn[q] = gld(q < 2 ? Dx : (q < 4 ? Dy : Dz), nb[q]);
     }
     T iDI = C.iDu, iDn[4] = {C.iDu, C.iDu, C.iDu, C.iDu};
-    if (C.idm == 2) {
-        iDI = gld(C.iDa, I);
+    if (idm == 2) {
+        const T *D = P.t[C.si].a;
+        iDI = gld(D, I);
 #pragma unroll
-        for (int q = 0; q < 4; q++) iDn[q] = gld(C.iDa, nb[q]);
+        for (int q = 0; q < 4; q++) iDn[q] = gld(D, nb[q]);
     }
     T ax = C.ax, ay = C.ay, az = C.az, Ku = C.Ku;
-    if (C.an) {
-        if (C.anAx) {
-            ax = gld(C.axA, I);
-            ay = gld(C.axA, N + I);
-            az = gld(C.axA, 2 * N + I);
-        }
-        if (C.KuA) Ku = gld(C.KuA, I);
+    if (fl & FL_AN_AXIS) {
+        const T *A = P.t[C.sa].b;
+        ax = gld(A, I);
+        ay = gld(A, N + I);
+        az = gld(A, 2 * N + I);
     }
-    T Kc = C.Kc;
-    if (C.cu && C.KcA) Kc = gld(C.KcA, I);
+    if (fl & FL_AN_KU) Ku = gld(P.t[C.sa].a, I);
+    T Kc = 0;
+    if (fl & FL_CU) Kc = (fl & FL_CU_KC) ? gld(P.t[C.sc].a, I) : P.t[C.sc].s[9];
     T zhx = 0, zhy = 0, zhz = 0;
-    if (C.ze == 1) {
+    if (ze == 1) {
         zhx = C.zx;
         zhy = C.zy;
         zhz = C.zz;
-    } else if (C.ze == 2) {
-        zhx = gld(C.zA, I);
-        zhy = gld(C.zB, I);
-        zhz = gld(C.zC, I);
+    } else if (ze == 2) {
+        zhx = gld(P.t[C.sz].a, I);
+        zhy = gld(P.t[C.sz].b, I);
+        zhz = gld(P.t[C.sz].c, I);
     }
     T Tl = 0;
-    if (C.th) Tl = (C.TA ? gld(C.TA, I) : C.Tu) + C.T0;
+    if (fl & FL_TH) {
+        const MMX_CONST TermDev<T> &D = P.t[C.st];
+        Tl = (D.uniform ? D.s[1] : gld(D.a, I)) + D.s[0];
+    }
     // ---- arithmetic ----
-    const bool ex_on = C.ex && (Ms != T(0)) && (C.ex == 1 || exAI != T(0));
-    const bool bd_on = C.bd && (Ms != T(0));
-    const bool id_on = C.idm && (Ms != T(0)) && (iDI != T(0));
+    const bool ex_on = ex && (Ms != T(0)) && (ex == 1 || exAI != T(0));
+    const bool bd_on = bd && (Ms != T(0));
+    const bool id_on = idm && (Ms != T(0)) && (iDI != T(0));
     T fex = 0, fey = 0, fez = 0, fbx = 0, fby = 0, fbz = 0, fix = 0, fiy = 0, fiz = 0;
+    T bs0 = 0, bs1 = 0, bs2 = 0;
+    if (bd) {
+        bs0 = P.t[C.sb].s[0];
+        bs1 = P.t[C.sb].s[1];
+        bs2 = P.t[C.sb].s[2];
+    }
     if (stencil) {
 #pragma unroll
         for (int q = 0; q < 6; q++) {
             if (!((msk >> q) & 1u)) continue;
             const T ax_ = nmx[q], ay_ = nmy[q], az_ = nmz[q];
-            if (C.ex == 1) {  // kernels.jl:269-304
+            if (ex == 1) {  // kernels.jl:269-304
                 const T nab = (q < 2) ? C.nab0 : ((q < 4) ? C.nab1 : C.nab2);
                 fex += nab * (ax_ - mx);
                 fey += nab * (ay_ - my);
                 fez += nab * (az_ - mz);
-            } else if (C.ex == 2) {  // kernels.jl:226-266
+            } else if (ex == 2) {  // kernels.jl:226-266
                 const T nab = (q < 2) ? C.nab0 : ((q < 4) ? C.nab1 : C.nab2);
                 const T A_nb = exAn[q];
                 if (A_nb != T(0)) {
@@ -671,36 +617,36 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
                     fez += coeff * (az_ - mz);
                 }
             }
-            if (C.bd == 1) {  // bulkdmi_kernel!: kernels.jl:306-344
-                if (q == 0) { fby += C.bs0 * (-az_); fbz += C.bs0 * ay_; }
-                if (q == 1) { fby += C.bs0 * az_;    fbz += C.bs0 * (-ay_); }
-                if (q == 2) { fbx += C.bs1 * az_;    fbz += C.bs1 * (-ax_); }
-                if (q == 3) { fbx += C.bs1 * (-az_); fbz += C.bs1 * ax_; }
-                if (q == 4) { fbx += C.bs2 * (-ay_); fby += C.bs2 * ax_; }
-                if (q == 5) { fbx += C.bs2 * ay_;    fby += C.bs2 * (-ax_); }
-            } else if (C.bd == 2) {  // spatial_vector_bulkdmi_kernel!: kernels.jl:347-422
+            if (bd == 1) {  // bulkdmi_kernel!: kernels.jl:306-344
+                if (q == 0) { fby += bs0 * (-az_); fbz += bs0 * ay_; }
+                if (q == 1) { fby += bs0 * az_;    fbz += bs0 * (-ay_); }
+                if (q == 2) { fbx += bs1 * az_;    fbz += bs1 * (-ax_); }
+                if (q == 3) { fbx += bs1 * (-az_); fbz += bs1 * ax_; }
+                if (q == 4) { fbx += bs2 * (-ay_); fby += bs2 * ax_; }
+                if (q == 5) { fbx += bs2 * ay_;    fby += bs2 * (-ax_); }
+            } else if (bd == 2) {  // spatial_vector_bulkdmi_kernel!: kernels.jl:347-422
                 const T DI = bDI[q / 2], Dn = bDn[q];
                 if (DI != T(0) && Dn != T(0)) {
                     const T De = T(2) * DI * Dn / (DI + Dn);
                     if (q < 2) {
-                        const T a = q == 0 ? C.bs0 : -C.bs0;
+                        const T a = q == 0 ? bs0 : -bs0;
                         fby += De * (-a * az_);
                         fbz += De * (a * ay_);
                     } else if (q < 4) {
-                        const T a = q == 2 ? C.bs1 : -C.bs1;
+                        const T a = q == 2 ? bs1 : -bs1;
                         fbx += De * (a * az_);
                         fbz += De * (-a * ax_);
                     } else {
-                        const T a = q == 4 ? C.bs2 : -C.bs2;
+                        const T a = q == 4 ? bs2 : -bs2;
                         fbx += De * (-a * ay_);
                         fby += De * (a * ax_);
                     }
                 }
             }
-            if (C.idm && q < 4) {  // interfacial_dmi_kernel!: kernels.jl:426-466
+            if (idm && q < 4) {  // interfacial_dmi_kernel!: kernels.jl:426-466
                 const T Dn = iDn[q];
                 if (Dn != T(0)) {
-                    const T De = C.idm == 1 ? C.iDe : T(2) * iDI * Dn / (iDI + Dn);
+                    const T De = idm == 1 ? C.iDe : T(2) * iDI * Dn / (iDI + Dn);
                     const T c = De * (q < 2 ? C.id0 : C.id1);
                     if (q == 0) { fix += c * az_;    fiz += c * (-ax_); }
                     if (q == 1) { fix += c * (-az_); fiz += c * ax_; }
@@ -712,7 +658,7 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
     }
     const T Ms_inv = (Ms != T(0)) ? T(1.0) / Ms : T(0);
     T ahx = 0, ahy = 0, ahz = 0;  // uniaxial, kernels.jl:54-105 (2/Ms == 2 * (1/Ms) exactly)
-    if (C.an && Ms != T(0)) {
+    if ((fl & FL_AN) && Ms != T(0)) {
         const T Ms2 = T(2.0) * Ms_inv;
         const T sa = mx * ax + my * ay + mz * az;
         ahx = Ku * sa * ax * Ms2;
@@ -720,7 +666,7 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
         ahz = Ku * sa * az * Ms2;
     }
     T chx = 0, chy = 0, chz = 0;  // cubic, kernels.jl:110-137
-    if (C.cu && Ms != T(0)) {
+    if ((fl & FL_CU) && Ms != T(0)) {
         const MMX_CONST T *ca = P.t[C.sc].s;
         const T Ms4 = T(4.0) * Kc / Ms;
         const T mxp = ca[0] * mx + ca[1] * my + ca[2] * mz;
@@ -732,18 +678,19 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
         chz = Ms4 * (mxp3 * ca[2] + myp3 * ca[5] + mzp3 * ca[8]);
     }
     T thx = 0, thy = 0, thz = 0;  // thermal, kernels.jl:469-494
-    if (C.th && Ms > T(0)) {
+    if ((fl & FL_TH) && Ms > T(0)) {
         const T scale = (T)sqrt((T)dyn.thermal_factor * Tl / Ms);
         const uint64_t base = 3 * (uint64_t)I;
-        thx = (T)philox_normal(C.seed, dyn.noise_step, base) * scale;
-        thy = (T)philox_normal(C.seed, dyn.noise_step, base + 1) * scale;
-        thz = (T)philox_normal(C.seed, dyn.noise_step, base + 2) * scale;
+        thx = (T)philox_normal(P.seed, dyn.noise_step, base) * scale;
+        thy = (T)philox_normal(P.seed, dyn.noise_step, base + 1) * scale;
+        thz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
     }
     Hx = Hy = Hz = T(0);
+    const int nterms = (fl >> FL_NT_SHIFT) & 15;
 #pragma unroll
     for (int q = 0; q < MAX_TERMS; q++) {
-        if (q >= C.nterms) break;
-        const int sk = P.ord[q];
+        if (q >= nterms) break;
+        const int sk = (C.ordp >> (3 * q)) & 7;
         T hx, hy, hz;
         if (sk == SL_EXCH) { hx = ex_on ? fex * Ms_inv : T(0); hy = ex_on ? fey * Ms_inv : T(0); hz = ex_on ? fez * Ms_inv : T(0); }
         else if (sk == SL_DMIB) { hx = bd_on ? fbx * Ms_inv : T(0); hy = bd_on ? fby * Ms_inv : T(0); hz = bd_on ? fbz * Ms_inv : T(0); }
@@ -760,7 +707,7 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
 }
 
 // Epilogue of the specialised kernels: MODE is EPI_STAGE, EPI_RK4_FINAL or EPI_ERR.
-template <class T, int MODE>
+template <class T, int MODE, bool NOKP>
 __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I,
                                             int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
                                             double &red)
@@ -768,7 +715,7 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
 #pragma clang fp contract(off)
     const int N = (int)P.N;
     const T alpha_ = P.alpha, llg_a_ = P.llg_a;
-    const int prec_ = P.precession;
+    const bool prec_ = (P.flags & FL_PREC) != 0;
     // operands of the stage combination, loaded with the field's loads
     T y0v[3] = {0, 0, 0}, kpv[3][3];
     if constexpr (MODE != EPI_ERR) {
@@ -833,9 +780,11 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
                 y[c] = y0v[c] + (T)E.cself * s;
             } else {
                 T s = y0v[c];
+                if constexpr (!NOKP) {
 #pragma unroll
-                for (int q = 0; q < 6; q++)
-                    if (q < E.np) s = s + (T)E.cp[q] * gld(E.kp[q], c * N + I);
+                    for (int q = 0; q < 6; q++)
+                        if (q < E.np) s = s + (T)E.cp[q] * gld(E.kp[q], c * N + I);
+                }
                 y[c] = s + (T)E.cself * kk[c];
             }
         }

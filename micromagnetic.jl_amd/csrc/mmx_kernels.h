@@ -69,7 +69,7 @@ template <class T>
 void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *const *Ps, const BatchStage<T> *st,
                         const Dyn *dyns, int V, int cpt, int grid, hipStream_t s);
 void launch_batch_tick(Dyn *dyns, int M, hipStream_t s);
-int batch_variant(int mode, int fast);
+int batch_variant(int mode, int fast, int np);
 
 // nudged elastic band (per-image reductions over grids of nb x images)
 template <class T>

@@ -262,9 +262,12 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 // ---------------------------------------------------------------------------
 // V selects the epilogue: 0 = any mode with the generic field (cell_field),
 // or the slot-form specialisations 1 = EPI_STAGE, 2 = EPI_RK4_FINAL, 3 = EPI_ERR.
+// V: 1 EPI_STAGE, 2 EPI_RK4_FINAL, 3 EPI_ERR, 4 EPI_STAGE without earlier stages
+// in the combination (np == 0: RK4 stages, Heun's first)
 template <int V>
 struct VMode {
-    static constexpr int mode = V == 1 ? EPI_STAGE : (V == 2 ? EPI_RK4_FINAL : EPI_ERR);
+    static constexpr int mode = (V == 1 || V == 4) ? EPI_STAGE : (V == 2 ? EPI_RK4_FINAL : EPI_ERR);
+    static constexpr bool nokp = V == 4;
 };
 
 template <class T, int V>
@@ -272,7 +275,7 @@ __device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, cons
                                          int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
 {
     if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
-    else finish_fast<T, VMode<V>::mode>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
+    else finish_fast<T, VMode<V>::mode, VMode<V>::nokp>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
 }
 
 // at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
@@ -970,10 +973,10 @@ __global__ void k_maxabs(const T *__restrict__ v, int64_t n, double *__restrict_
 // launchers
 // ---------------------------------------------------------------------------
 // epilogue variant for a mode: the slot-form kernels take the integrator stages
-static int variant(int mode, int fast)
+static int variant(int mode, int fast, int np)
 {
     if (!fast) return 0;
-    if (mode == EPI_STAGE) return 1;
+    if (mode == EPI_STAGE) return np == 0 ? 4 : 1;
     if (mode == EPI_RK4_FINAL) return 2;
     if (mode == EPI_ERR) return 3;
     return 0;
@@ -1014,8 +1017,9 @@ int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T
                  hipStream_t s)
 {
     int g = grid_for(N, max_local_blocks());
-    const int v = variant(E.mode, fast);
-    if (v == 1) hipLaunchKernelGGL((k_local<T, 1>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    const int v = variant(E.mode, fast, E.np);
+    if (v == 4) hipLaunchKernelGGL((k_local<T, 4>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    else if (v == 1) hipLaunchKernelGGL((k_local<T, 1>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else if (v == 2) hipLaunchKernelGGL((k_local<T, 2>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else if (v == 3) hipLaunchKernelGGL((k_local<T, 3>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else hipLaunchKernelGGL((k_local<T, 0>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
@@ -1097,8 +1101,10 @@ int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, con
 #define P5_CALL(H)                                                                                   \
     {                                                                                                \
         g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;                         \
-        const int v = variant(E.mode, fast);                                                         \
-        if (v == 1)                                                                                  \
+        const int v = variant(E.mode, fast, E.np);                                                   \
+        if (v == 4)                                                                                  \
+            hipLaunchKernelGGL((k_p5<T, H, 4>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 1)                                                                             \
             hipLaunchKernelGGL((k_p5<T, H, 1>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
         else if (v == 2)                                                                             \
             hipLaunchKernelGGL((k_p5<T, H, 2>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
@@ -1186,7 +1192,8 @@ void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *con
     if (cpt == 1) BL_CALL(VV, 1);       \
     else if (cpt == 2) BL_CALL(VV, 2);  \
     else BL_CALL(VV, 4);
-    if (V == 1) { BL_CPT(1) }
+    if (V == 4) { BL_CPT(4) }
+    else if (V == 1) { BL_CPT(1) }
     else if (V == 2) { BL_CPT(2) }
     else { BL_CPT(0) }
 #undef BL_CPT
@@ -1198,7 +1205,7 @@ void launch_batch_tick(Dyn *dyns, int M, hipStream_t s)
     hipLaunchKernelGGL(k_batch_tick, dim3((M + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, dyns, M);
 }
 
-int batch_variant(int mode, int fast) { return variant(mode, fast); }
+int batch_variant(int mode, int fast, int np) { return variant(mode, fast, np); }
 
 template <class T>
 int launch_sum(const T *v, int64_t n, double *partial, hipStream_t s)

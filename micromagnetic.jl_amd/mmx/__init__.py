@@ -295,20 +295,26 @@ def add_dmi(sim, D, name="dmi", type="bulk"):
 
 
 def add_anis(sim, Ku, axis=(0, 0, 1), name="anis"):
+    """add_anis: src/micro/add_field.jl:430-465.  A number Ku stays a uniform
+    parameter (the same value in every cell, without a per-cell array)."""
     L = _lib.load()
-    Kua = init_scalar(sim.mesh, Ku)
+    uniform = np.ndim(Ku) == 0 and not callable(Ku)
+    Kua = None if uniform else init_scalar(sim.mesh, Ku)
+    Ku0 = float(Ku) if uniform else 0.0
     if isinstance(axis, tuple):
-        return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), 0.0, None, *map(float, axis)), name)
+        return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), Ku0, None, *map(float, axis)), name)
     ax = np.ascontiguousarray(init_vector(sim.mesh, axis))
-    return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), 0.0, dptr(ax), 0.0, 0.0, 0.0), name)
+    return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), Ku0, dptr(ax), 0.0, 0.0, 0.0), name)
 
 
 def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
+    """add_cubic_anis: src/micro/add_field.jl:541-574 (a number Kc stays uniform)."""
     L = _lib.load()
-    a = init_scalar(sim.mesh, Kc)
+    uniform = np.ndim(Kc) == 0 and not callable(Kc)
+    a = None if uniform else init_scalar(sim.mesh, Kc)
     a1 = np.array(axis1, dtype=np.float64)
     a2 = np.array(axis2, dtype=np.float64)
-    return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), 0.0, dptr(a1), dptr(a2)), name)
+    return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
 def add_zeeman(sim, H0, name="zeeman"):
