@@ -341,7 +341,23 @@ struct Engine : CtxBase {
             for (int i = 0; i < P.nterms; i++) o |= (uint32_t)P.ord[i] << (3 * i);
             P.ordp = o;
         }
-        // -gamma/(1+alpha^2) rounded as the kernel computes it (llg.jl:18)
+        // term signature: every term uniform, no thermal term, added in slot order
+        {
+            bool ok = P.fast != 0;
+            uint32_t sig = 0;
+            int last = -1;
+            for (int i = 0; i < P.nterms && ok; i++) {
+                const HostTerm &h = C->terms[i];
+                const bool uni = h.kind == K_EXCH || h.kind == K_DMI_BULK || h.kind == K_ZEEMAN_U ||
+                                 h.kind == K_DEMAG || (h.kind == K_DMI_INT && h.uniform) ||
+                                 (h.kind == K_ANIS && h.uniform) || (h.kind == K_CUBIC && h.uniform);
+                ok = uni && P.ord[i] > last;
+                last = P.ord[i];
+                sig |= 1u << P.ord[i];
+            }
+            P.sig = ok ? sig : 0u;
+        }
+                // -gamma/(1+alpha^2) rounded as the kernel computes it (llg.jl:18)
         volatile T al = (T)C->alpha;
         T den = T(1) + al * al;
         P.llg_a = -(T)C->gamma / den;
@@ -411,7 +427,7 @@ struct Engine : CtxBase {
         int nb;
         if (!demag) {
             mark(MMX_PASS_LOCAL, true);
-            nb = launch_local<T>(N, P, d, E, m, fast, C->stream);
+            nb = launch_local<T>(N, P, d, E, m, fast, Plast.sig, C->stream);
             mark(MMX_PASS_LOCAL, false);
         } else {
             const int nx = G.nx, ny = G.ny, nz = G.nz;
@@ -428,7 +444,7 @@ struct Engine : CtxBase {
             launch_coly<T>(G, Bbuf.p, Abuf.p, G.Py, G.Py, ny, ny, nz, 3, 1, tw.p, C->stream);
             mark(MMX_PASS_P4, false);
             mark(MMX_PASS_P5, true);
-            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, fast, tw.p, C->stream);
+            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, fast, Plast.sig, tw.p, C->stream);
             mark(MMX_PASS_P5, false);
         }
         HIPCHECK(hipGetLastError());

@@ -82,6 +82,7 @@ struct FieldParams {
     // the slot form's switches and insertion order packed into two words
     // (FL_* bits below; ordp holds ord[q] in bits 3q..3q+2)
     uint32_t flags, ordp;
+    uint32_t sig;  // term signature (0 when none applies)
 };
 
 enum SlotFlags : uint32_t {
@@ -97,7 +98,46 @@ enum SlotFlags : uint32_t {
     FL_NT_SHIFT = 20,  // 4 bits: number of terms
 };
 
+
+
 enum SlotKind : int { SL_EXCH = 0, SL_DMIB, SL_DMII, SL_ANIS, SL_CUBIC, SL_ZEE, SL_DEMAG, SL_THERM };
+
+// A term signature: the set of slots present (bit = SlotKind) when every term
+// has uniform parameters (no per-cell arrays), there is no thermal term and
+// the terms were added in slot order.  Kernels compiled for a signature know
+// the switches and the insertion order at compile time; signature 0 reads
+// them from the parameter block.
+constexpr uint32_t SG_EX = 1u << SL_EXCH, SG_BD = 1u << SL_DMIB, SG_ID = 1u << SL_DMII, SG_AN = 1u << SL_ANIS,
+                   SG_CU = 1u << SL_CUBIC, SG_ZE = 1u << SL_ZEE, SG_DM = 1u << SL_DEMAG;
+// the signatures the kernels are compiled for
+constexpr uint32_t SIG_CHIRAL = SG_EX | SG_ID | SG_AN | SG_ZE | SG_DM;    // film with iDMI, Ku, field, demag
+constexpr uint32_t SIG_SOFT = SG_EX | SG_DM;                              // exchange + demag
+constexpr uint32_t SIG_BULK = SG_EX | SG_BD | SG_CU | SG_DM;              // bulk DMI, cubic, demag
+constexpr uint32_t SIG_CHIRAL_LOCAL = SG_EX | SG_ID | SG_AN | SG_ZE;      // as SIG_CHIRAL without demag
+
+constexpr uint32_t sig_flags(uint32_t sig)
+{
+    uint32_t f = 0, n = 0;
+    for (int b = 0; b < 8; b++) n += (sig >> b) & 1u;
+    if (sig & SG_EX) f |= 1u << FL_EX_SHIFT;
+    if (sig & SG_BD) f |= 1u << FL_BD_SHIFT;
+    if (sig & SG_ID) f |= 1u << FL_ID_SHIFT;
+    if (sig & SG_AN) f |= FL_AN;
+    if (sig & SG_CU) f |= FL_CU;
+    if (sig & SG_ZE) f |= 1u << FL_ZE_SHIFT;
+    return f | (n << FL_NT_SHIFT);
+}
+
+constexpr uint32_t sig_ordp(uint32_t sig)
+{
+    uint32_t o = 0, q = 0;
+    for (uint32_t b = 0; b < 8; b++)
+        if ((sig >> b) & 1u) {
+            o |= b << (3 * q);
+            q++;
+        }
+    return o;
+}
 
 // Per-launch values (by value; the parameter block above lives in device
 // memory and is read with wave-uniform scalar loads).

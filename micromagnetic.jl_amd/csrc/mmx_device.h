@@ -486,7 +486,7 @@ __device__ __forceinline__ void gst(X *p, int64_t i, X v)
 
 // Field of cell I in slot form (see SlotCfg); returns m of the cell and its
 // neighbour / pin mask.
-template <class T>
+template <class T, uint32_t SIG>
 __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, int I, int i, int j,
                                                 int k, const T *__restrict__ m, T hdx, T hdy, T hdz, T &mx, T &my,
                                                 T &mz, T &Hx, T &Hy, T &Hz, uint32_t &msk)
@@ -494,7 +494,9 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
 #pragma clang fp contract(off)
     const SlotCfg<T> C = slot_cfg(P);
     const int N = C.N;
-    const uint32_t fl = C.fl;
+    constexpr uint32_t GEOM = FL_PX | FL_PY | FL_PZ | FL_PREC;
+    const uint32_t fl = SIG ? (sig_flags(SIG) | (C.fl & GEOM)) : C.fl;
+    const uint32_t ordp = SIG ? sig_ordp(SIG) : C.ordp;
     const int ex = (fl >> FL_EX_SHIFT) & 3, bd = (fl >> FL_BD_SHIFT) & 3, idm = (fl >> FL_ID_SHIFT) & 3;
     const int ze = (fl >> FL_ZE_SHIFT) & 3;
     // ---- loads ----
@@ -690,7 +692,7 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
 #pragma unroll
     for (int q = 0; q < MAX_TERMS; q++) {
         if (q >= nterms) break;
-        const int sk = (C.ordp >> (3 * q)) & 7;
+        const int sk = (ordp >> (3 * q)) & 7;
         T hx, hy, hz;
         if (sk == SL_EXCH) { hx = ex_on ? fex * Ms_inv : T(0); hy = ex_on ? fey * Ms_inv : T(0); hz = ex_on ? fez * Ms_inv : T(0); }
         else if (sk == SL_DMIB) { hx = bd_on ? fbx * Ms_inv : T(0); hy = bd_on ? fby * Ms_inv : T(0); hz = bd_on ? fbz * Ms_inv : T(0); }
@@ -707,7 +709,7 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
 }
 
 // Epilogue of the specialised kernels: MODE is EPI_STAGE, EPI_RK4_FINAL or EPI_ERR.
-template <class T, int MODE, bool NOKP>
+template <class T, int MODE, bool NOKP, uint32_t SIG>
 __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I,
                                             int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
                                             double &red)
@@ -730,7 +732,7 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
     }
     T mx, my, mz, Hx, Hy, Hz;
     uint32_t msk;
-    cell_field_fast(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk);
+    cell_field_fast<T, SIG>(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk);
     // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
     T kk[3];
     if ((msk >> 6) & 1u) {

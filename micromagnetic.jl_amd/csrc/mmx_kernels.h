@@ -18,7 +18,7 @@ void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s);
 // used (the number of partial-reduction slots written).
 template <class T>
 int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, int fast,
-                 hipStream_t s);
+                 uint32_t sig, hipStream_t s);
 
 // demag passes
 template <class T>
@@ -32,7 +32,14 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
                   int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s);
 template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
-              const T *m, int fast, const cplx<T> *tw, hipStream_t s);
+              const T *m, int fast, uint32_t sig, const cplx<T> *tw, hipStream_t s);
+// kernels compiled for a term signature (mmx_kernels_sig.hip); -1 when none matches
+template <class T>
+int launch_p5_sig(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
+                  const T *m, uint32_t sig, const cplx<T> *tw, hipStream_t s);
+template <class T>
+int launch_local_sig(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, uint32_t sig,
+                     int grid, hipStream_t s);
 
 // tensor set-up (FP64)
 void launch_tensor_pad(const DemagGeom &G, const double *rt, int comp, double *pad, hipStream_t s);

@@ -14,32 +14,12 @@
 #include "mmx_device.h"
 #include "mmx_fft.h"
 #include "mmx_kernels.h"
+#include "mmx_pass.h"
 
 namespace mmx {
 
 // Block shapes.  A pass works on batches of sequences; each sequence of length
 // L uses TPS(L) threads holding EPT(L) elements (mmx_fft.h).
-template <int L>
-struct Shape : FftShape<L> {
-    using FftShape<L>::TPS;
-    // y columns: TY consecutive kx columns per block (>= 8 for whole cache lines)
-    static constexpr int TY = (256 / TPS) >= 8 ? (256 / TPS) : ((1024 / TPS) < 8 ? (1024 / TPS) : 8);
-    static constexpr int BY = TY * TPS;
-    // z columns: TZ columns at ky and TZ at Py-ky (which share the folded
-    // tensor), all three components in one thread, 4 elements each
-    static constexpr int TPSZ = FftShape<L, 4>::TPS;
-    static constexpr int TZ = (128 / TPSZ) > 64 ? 64 : ((128 / TPSZ) < 1 ? 1 : (128 / TPSZ));
-    static constexpr int BZ = 2 * TZ * TPSZ;
-    // x rows: SX sequences per block
-    static constexpr int SX = (256 / TPS) < 1 ? 1 : (256 / TPS);
-    static constexpr int BX = SX * TPS;
-    // last pass: RX rows x 3 components per block; 3*RX*TPS is a whole number
-    // of 64-wide waves (the block reductions shuffle across full waves)
-    static constexpr int GX = TPS >= 64 ? 1 : 64 / TPS;
-    static constexpr int RX = (256 / (3 * TPS)) / GX * GX < GX ? GX : (256 / (3 * TPS)) / GX * GX;
-    static constexpr int B5 = 3 * RX * TPS;
-    static_assert(B5 % 64 == 0, "last-pass blocks must be whole waves");
-};
 
 // ---------------------------------------------------------------------------
 // P1: rows of n_in reals (zero-padded to Px = 2H) -> Kx = H+1 complex values
@@ -67,13 +47,30 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
     const int64_t rb = row * n_in;
     const T *src = in + c * cstride + rb;
     cplx<T> v[EPT];
+    // even rows whose start is a multiple of 2 elements: one 2-element load per
+    // pair (x[2k], x[2k+1]) for the input and for the scale
+    const bool pairs = (n_in % 2 == 0) && (cstride % 2 == 0);
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
         const int k = j + m * TPS;
         const int x0 = 2 * k, x1 = 2 * k + 1;
         T a = 0, b = 0;
-        if (active && x0 < n_in) a = scale ? src[x0] * scale[rb + x0] : src[x0];
-        if (active && x1 < n_in) b = scale ? src[x1] * scale[rb + x1] : src[x1];
+        if (pairs) {
+            if (active && x0 < n_in) {
+                const cplx<T> xv = *(const cplx<T> *)(src + x0);
+                if (scale) {
+                    const cplx<T> sv = *(const cplx<T> *)(scale + rb + x0);
+                    a = xv.x * sv.x;
+                    b = xv.y * sv.y;
+                } else {
+                    a = xv.x;
+                    b = xv.y;
+                }
+            }
+        } else {
+            if (active && x0 < n_in) a = scale ? src[x0] * scale[rb + x0] : src[x0];
+            if (active && x1 < n_in) b = scale ? src[x1] * scale[rb + x1] : src[x1];
+        }
         v[m] = {a, b};
     }
     cplx<T> *buf = lds + ls * HS;
@@ -162,7 +159,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
                                                          int nzb, int n_in, const cplx<T> *__restrict__ tw)
 {
     using S = Shape<L>;
-    constexpr int EPT = FftShape<L, 4>::EPT, TPS = FftShape<L, 4>::TPS, TZ = S::TZ;
+    constexpr int EPT = FftShape<L, MMX_P3_E>::EPT, TPS = FftShape<L, MMX_P3_E>::TPS, TZ = S::TZ;
     constexpr int LS = L + 1;
     constexpr int NC = FWD_ONLY ? 1 : 3;
     // one exchange buffer per component when the three transforms share barriers
@@ -191,7 +188,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
         }
     cplx<T> *buf = lds + (half * TZ + t) * LS;
     if constexpr (FWD_ONLY) {
-        fft_reg<T, L, false, 4>(v[0], j, buf, tw, G.lgLmax);
+        fft_reg<T, L, false, MMX_P3_E>(v[0], j, buf, tw, G.lgLmax);
         if (!ok) return;
 #pragma unroll
         for (int m = 0; m < EPT; m++) col[(int64_t)(j + m * TPS) * zs_b] = v[0][m];
@@ -223,10 +220,10 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
         };
         if constexpr (MMX_P3_PF) load_tensor();
         if constexpr (NB == 3) {
-            fft_reg_multi<T, L, false, 4, 3>(&v[0][0], j, buf, CS, tw, G.lgLmax);
+            fft_reg_multi<T, L, false, MMX_P3_E, 3>(&v[0][0], j, buf, CS, tw, G.lgLmax);
         } else {
 #pragma unroll
-            for (int c = 0; c < NC; c++) fft_reg<T, L, false, 4>(v[c], j, buf, tw, G.lgLmax);
+            for (int c = 0; c < NC; c++) fft_reg<T, L, false, MMX_P3_E>(v[c], j, buf, tw, G.lgLmax);
         }
         if constexpr (!MMX_P3_PF) load_tensor();
 #pragma unroll
@@ -238,10 +235,10 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
             v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
         }
         if constexpr (NB == 3) {
-            fft_reg_multi<T, L, true, 4, 3>(&v[0][0], j, buf, CS, tw, G.lgLmax);
+            fft_reg_multi<T, L, true, MMX_P3_E, 3>(&v[0][0], j, buf, CS, tw, G.lgLmax);
         } else {
 #pragma unroll
-            for (int c = 0; c < NC; c++) fft_reg<T, L, true, 4>(v[c], j, buf, tw, G.lgLmax);
+            for (int c = 0; c < NC; c++) fft_reg<T, L, true, MMX_P3_E>(v[c], j, buf, tw, G.lgLmax);
         }
         if (!ok) return;
         const int nz = G.nz;
@@ -262,124 +259,6 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 // ---------------------------------------------------------------------------
 // V selects the epilogue: 0 = any mode with the generic field (cell_field),
 // or the slot-form specialisations 1 = EPI_STAGE, 2 = EPI_RK4_FINAL, 3 = EPI_ERR.
-// V: 1 EPI_STAGE, 2 EPI_RK4_FINAL, 3 EPI_ERR, 4 EPI_STAGE without earlier stages
-// in the combination (np == 0: RK4 stages, Heun's first)
-template <int V>
-struct VMode {
-    static constexpr int mode = (V == 1 || V == 4) ? EPI_STAGE : (V == 2 ? EPI_RK4_FINAL : EPI_ERR);
-    static constexpr bool nokp = V == 4;
-};
-
-template <class T, int V>
-__device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
-                                         int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
-{
-    if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
-    else finish_fast<T, VMode<V>::mode, VMode<V>::nokp>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
-}
-
-// at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
-// latency-bound, and the register budget this sets costs no spills
-#ifndef MMX_P5_WPE
-#define MMX_P5_WPE 3
-#endif
-#define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu(MMX_P5_WPE)))
-template <class T, int H, int V>
-__global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
-                                                      const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
-                                                      const T *__restrict__ m, const cplx<T> *__restrict__ tw)
-{
-    using S = Shape<H>;
-    constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
-    constexpr int HS = H + 1;
-    __shared__ cplx<T> lds[3 * RX * HS];
-    const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const int nrows_tot = ny * nz;
-    // A block takes RX rows (x lines) at one y and consecutive z.  Groups are
-    // ordered y-major, z-minor, and each XCD (launch indices go round-robin
-    // over the 8 XCDs) gets one contiguous run of groups: the z+-1 rows a cell
-    // reads were just fetched by this or the previous block, the y+-1 rows a
-    // few blocks earlier, so both come from that XCD's L2.
-    const int gz = (nz + RX - 1) / RX;
-    const int ngroups = ny * gz;
-    const int per = (ngroups + 7) / 8;
-    const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (lb >= ngroups) {
-        if (threadIdx.x == 0 && (E.mode == EPI_ERR || E.mode == EPI_ENERGY)) E.partial[blockIdx.x] = 0.0;
-        return;
-    }
-    const int y0 = lb / gz, z0 = (lb % gz) * RX;
-    const int nrows = min(RX, nz - z0);
-    const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
-    const int c = ls / RX, r = ls % RX;
-    const bool active = r < nrows;
-    const cplx<T> *row = A + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
-    const int sh = G.lgLmax - (FftShape<H>::LG + 1);
-    cplx<T> v[EPT];
-#pragma unroll
-    for (int q = 0; q < EPT; q++) {
-        const int k = j + q * TPS;
-        cplx<T> Z = {0, 0};
-        if (active) {
-            cplx<T> Xk = row[k];
-            cplx<T> Xc = row[H - k];
-            Xc.y = -Xc.y;
-            cplx<T> Fe = cadd(Xk, Xc);
-            cplx<T> w = tw[k << sh];
-            w.y = -w.y;  // W_Px^{-k}
-            cplx<T> Fo = cmul(csub(Xk, Xc), w);
-            Z = {Fe.x - Fo.y, Fe.y + Fo.x};  // Fe + i Fo
-        }
-        v[q] = Z;
-    }
-    cplx<T> *buf = lds + (c * RX + r) * HS;
-    fft_reg<T, H, true>(v, j, buf, tw, G.lgLmax);
-#pragma unroll
-    for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
-    __syncthreads();
-    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
-    double red = 0;
-    // (row, x) of this thread's cell, advanced without dividing
-    int rr = (int)threadIdx.x / nx, x = (int)threadIdx.x - rr * nx;
-    for (int idx = threadIdx.x; idx < nrows * nx; idx += blockDim.x) {
-        const int y = y0, z = z0 + rr;
-        const int rw = z * ny + y;
-        const cplx<T> h0 = lds[(0 * RX + rr) * HS + (x >> 1)];
-        const cplx<T> h1 = lds[(1 * RX + rr) * HS + (x >> 1)];
-        const cplx<T> h2 = lds[(2 * RX + rr) * HS + (x >> 1)];
-        const T hx = (x & 1) ? h0.y : h0.x;
-        const T hy = (x & 1) ? h1.y : h1.x;
-        const T hz = (x & 1) ? h2.y : h2.x;
-        epilogue<T, V>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
-        x += blockDim.x;
-        while (x >= nx) {
-            x -= nx;
-            rr++;
-        }
-    }
-    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
-}
-
-// ---------------------------------------------------------------------------
-// Local-field kernel (meshes without demag): one thread per cell.
-// ---------------------------------------------------------------------------
-template <class T, int V>
-__global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
-                                                 const T *__restrict__ m)
-{
-    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
-    double red = 0;
-    const unsigned nx = P.nx, ny = P.ny;
-    const int N = (int)P.N;  // meshes hold fewer than 2^31 cells
-    for (int I = blockIdx.x * BLOCK + threadIdx.x; I < N; I += gridDim.x * BLOCK) {
-        const unsigned q = (unsigned)I / nx;
-        const int i = (int)((unsigned)I - q * nx);
-        const unsigned k = q / ny;
-        const int j = (int)(q - k * ny);
-        epilogue<T, V>(P, dyn, E, I, i, j, (int)k, m, T(0), T(0), T(0), red);
-    }
-    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
-}
 
 
 
@@ -972,15 +851,7 @@ __global__ void k_maxabs(const T *__restrict__ v, int64_t n, double *__restrict_
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// epilogue variant for a mode: the slot-form kernels take the integrator stages
-static int variant(int mode, int fast, int np)
-{
-    if (!fast) return 0;
-    if (mode == EPI_STAGE) return np == 0 ? 4 : 1;
-    if (mode == EPI_RK4_FINAL) return 2;
-    if (mode == EPI_ERR) return 3;
-    return 0;
-}
+
 
 static int grid_for(int64_t n, int cap = 8192)
 {
@@ -1014,9 +885,10 @@ void launch_to_double(const T *src, double *dst, int64_t n, hipStream_t s)
 
 template <class T>
 int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E, const T *m, int fast,
-                 hipStream_t s)
+                 uint32_t sig, hipStream_t s)
 {
     int g = grid_for(N, max_local_blocks());
+    if (fast && sig && launch_local_sig<T>(N, P, dyn, E, m, sig, g, s) >= 0) return g;
     const int v = variant(E.mode, fast, E.np);
     if (v == 4) hipLaunchKernelGGL((k_local<T, 4>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else if (v == 1) hipLaunchKernelGGL((k_local<T, 1>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
@@ -1095,8 +967,12 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
 
 template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
-              const T *m, int fast, const cplx<T> *tw, hipStream_t s)
+              const T *m, int fast, uint32_t sig, const cplx<T> *tw, hipStream_t s)
 {
+    if (fast && sig) {
+        const int gs = launch_p5_sig<T>(G, A, P, dyn, E, m, sig, tw, s);
+        if (gs >= 0) return gs;
+    }
     int g = 0;
 #define P5_CALL(H)                                                                                   \
     {                                                                                                \
@@ -1283,7 +1159,7 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
     template void launch_cast<T>(const double *, T *, int64_t, hipStream_t);                                 \
     template void launch_to_double<T>(const T *, double *, int64_t, hipStream_t);                            \
     template int launch_local<T>(int64_t, const FieldParams<T> *, const Dyn &, const Epi<T> &, const T *, int,  \
-                                 hipStream_t);                                                               \
+                                 uint32_t, hipStream_t);                                                               \
     template void launch_p1<T>(const DemagGeom &, const T *, int64_t, const T *, int, int, int, int, cplx<T> *, \
                                const cplx<T> *, hipStream_t);                                                \
     template void launch_coly<T>(const DemagGeom &, const cplx<T> *, cplx<T> *, int, int, int, int, int, int,  \
@@ -1291,7 +1167,7 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
     template void launch_convz<T>(const DemagGeom &, cplx<T> *, const T *, const T *, int, int, int, int,      \
                                   const cplx<T> *, hipStream_t);                                                              \
     template int launch_p5<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,        \
-                              const Epi<T> &, const T *, int, const cplx<T> *, hipStream_t);                 \
+                              const Epi<T> &, const T *, int, uint32_t, const cplx<T> *, hipStream_t);                 \
     template void launch_tensor_extract<T>(const DemagGeom &, const cplx<double> *, int, double, T *,         \
                                            hipStream_t);                                                     \
     template int launch_maxdiff<T>(const T *, const T *, int64_t, double *, hipStream_t);          \

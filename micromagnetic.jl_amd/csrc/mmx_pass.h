@@ -1,0 +1,172 @@
+// mmx_pass.h -- the tile shapes of the demag passes and the kernels that end an
+// RHS evaluation (the last demag pass k_p5 and the local-field kernel k_local),
+// shared by the translation units that instantiate them.
+#pragma once
+#include "mmx_common.h"
+#include "mmx_device.h"
+#include "mmx_fft.h"
+
+namespace mmx {
+
+// z pass: elements per thread of the length-Pz transform, and the number of
+// threads per block the column tile aims at
+#ifndef MMX_P3_E
+#define MMX_P3_E 4
+#endif
+#ifndef MMX_P3_THREADS
+#define MMX_P3_THREADS 256
+#endif
+
+template <int L>
+struct Shape : FftShape<L> {
+    using FftShape<L>::TPS;
+    // y columns: TY consecutive kx columns per block (>= 8 for whole cache lines)
+    static constexpr int TY = (256 / TPS) >= 8 ? (256 / TPS) : ((1024 / TPS) < 8 ? (1024 / TPS) : 8);
+    static constexpr int BY = TY * TPS;
+    // z columns: TZ columns at ky and TZ at Py-ky (which share the folded
+    // tensor), all three components in one thread, 4 elements each
+    static constexpr int TPSZ = FftShape<L, MMX_P3_E>::TPS;
+    static constexpr int TZ0 = MMX_P3_THREADS / 2 / TPSZ;
+    static constexpr int TZ = TZ0 > 64 ? 64 : (TZ0 < 1 ? 1 : TZ0);
+    static constexpr int BZ = 2 * TZ * TPSZ;
+    // x rows: SX sequences per block
+    static constexpr int SX = (256 / TPS) < 1 ? 1 : (256 / TPS);
+    static constexpr int BX = SX * TPS;
+    // last pass: RX rows x 3 components per block; 3*RX*TPS is a whole number
+    // of 64-wide waves (the block reductions shuffle across full waves)
+    static constexpr int GX = TPS >= 64 ? 1 : 64 / TPS;
+    static constexpr int RX = (256 / (3 * TPS)) / GX * GX < GX ? GX : (256 / (3 * TPS)) / GX * GX;
+    static constexpr int B5 = 3 * RX * TPS;
+    static_assert(B5 % 64 == 0, "last-pass blocks must be whole waves");
+};
+
+// V: 1 EPI_STAGE, 2 EPI_RK4_FINAL, 3 EPI_ERR, 4 EPI_STAGE without earlier stages
+// in the combination (np == 0: RK4 stages, Heun's first)
+template <int V>
+struct VMode {
+    static constexpr int mode = (V == 1 || V == 4) ? EPI_STAGE : (V == 2 ? EPI_RK4_FINAL : EPI_ERR);
+    static constexpr bool nokp = V == 4;
+};
+
+template <class T, int V, uint32_t SIG = 0>
+__device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
+                                         int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
+{
+    if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
+    else finish_fast<T, VMode<V>::mode, VMode<V>::nokp, SIG>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
+}
+
+// at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
+// latency-bound, and the register budget this sets costs no spills
+#ifndef MMX_P5_WPE
+#define MMX_P5_WPE 3
+#endif
+#define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu(MMX_P5_WPE)))
+template <class T, int H, int V, uint32_t SIG = 0>
+__global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
+                                                      const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
+                                                      const T *__restrict__ m, const cplx<T> *__restrict__ tw)
+{
+    using S = Shape<H>;
+    constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
+    constexpr int HS = H + 1;
+    __shared__ cplx<T> lds[3 * RX * HS];
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int nrows_tot = ny * nz;
+    // A block takes RX rows (x lines) at one y and consecutive z.  Groups are
+    // ordered y-major, z-minor, and each XCD (launch indices go round-robin
+    // over the 8 XCDs) gets one contiguous run of groups: the z+-1 rows a cell
+    // reads were just fetched by this or the previous block, the y+-1 rows a
+    // few blocks earlier, so both come from that XCD's L2.
+    const int gz = (nz + RX - 1) / RX;
+    const int ngroups = ny * gz;
+    const int per = (ngroups + 7) / 8;
+    const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (lb >= ngroups) {
+        if (threadIdx.x == 0 && (E.mode == EPI_ERR || E.mode == EPI_ENERGY)) E.partial[blockIdx.x] = 0.0;
+        return;
+    }
+    const int y0 = lb / gz, z0 = (lb % gz) * RX;
+    const int nrows = min(RX, nz - z0);
+    const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
+    const int c = ls / RX, r = ls % RX;
+    const bool active = r < nrows;
+    const cplx<T> *row = A + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
+    const int sh = G.lgLmax - (FftShape<H>::LG + 1);
+    cplx<T> v[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int k = j + q * TPS;
+        cplx<T> Z = {0, 0};
+        if (active) {
+            cplx<T> Xk = row[k];
+            cplx<T> Xc = row[H - k];
+            Xc.y = -Xc.y;
+            cplx<T> Fe = cadd(Xk, Xc);
+            cplx<T> w = tw[k << sh];
+            w.y = -w.y;  // W_Px^{-k}
+            cplx<T> Fo = cmul(csub(Xk, Xc), w);
+            Z = {Fe.x - Fo.y, Fe.y + Fo.x};  // Fe + i Fo
+        }
+        v[q] = Z;
+    }
+    cplx<T> *buf = lds + (c * RX + r) * HS;
+    fft_reg<T, H, true>(v, j, buf, tw, G.lgLmax);
+#pragma unroll
+    for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
+    __syncthreads();
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
+    double red = 0;
+    // (row, x) of this thread's cell, advanced without dividing
+    int rr = (int)threadIdx.x / nx, x = (int)threadIdx.x - rr * nx;
+    for (int idx = threadIdx.x; idx < nrows * nx; idx += blockDim.x) {
+        const int y = y0, z = z0 + rr;
+        const int rw = z * ny + y;
+        const cplx<T> h0 = lds[(0 * RX + rr) * HS + (x >> 1)];
+        const cplx<T> h1 = lds[(1 * RX + rr) * HS + (x >> 1)];
+        const cplx<T> h2 = lds[(2 * RX + rr) * HS + (x >> 1)];
+        const T hx = (x & 1) ? h0.y : h0.x;
+        const T hy = (x & 1) ? h1.y : h1.x;
+        const T hz = (x & 1) ? h2.y : h2.x;
+        epilogue<T, V, SIG>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
+        x += blockDim.x;
+        while (x >= nx) {
+            x -= nx;
+            rr++;
+        }
+    }
+    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Local-field kernel (meshes without demag): one thread per cell.
+// ---------------------------------------------------------------------------
+template <class T, int V, uint32_t SIG = 0>
+__global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
+                                                 const T *__restrict__ m)
+{
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
+    double red = 0;
+    const unsigned nx = P.nx, ny = P.ny;
+    const int N = (int)P.N;  // meshes hold fewer than 2^31 cells
+    for (int I = blockIdx.x * BLOCK + threadIdx.x; I < N; I += gridDim.x * BLOCK) {
+        const unsigned q = (unsigned)I / nx;
+        const int i = (int)((unsigned)I - q * nx);
+        const unsigned k = q / ny;
+        const int j = (int)(q - k * ny);
+        epilogue<T, V, SIG>(P, dyn, E, I, i, j, (int)k, m, T(0), T(0), T(0), red);
+    }
+    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
+}
+
+// epilogue variant for a mode: the slot-form kernels take the integrator stages
+inline int variant(int mode, int fast, int np)
+{
+    if (!fast) return 0;
+    if (mode == EPI_STAGE) return np == 0 ? 4 : 1;
+    if (mode == EPI_RK4_FINAL) return 2;
+    if (mode == EPI_ERR) return 3;
+    return 0;
+}
+
+}  // namespace mmx
