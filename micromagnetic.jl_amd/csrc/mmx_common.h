@@ -239,6 +239,7 @@ struct DemagGeom {
     int lgH, lgPy, lgPz;
     int Lmax, lgLmax;   // twiddle table length
     int Py2, Pz2;       // Py/2+1, Pz/2+1 (folded tensor extents)
+    int ktw;            // 1: folded tensor grouped by 4 kx per (kyf, kzf run), for the wave-form z pass
 };
 
 }  // namespace mmx

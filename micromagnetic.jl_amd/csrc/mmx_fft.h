@@ -83,13 +83,32 @@ __device__ __forceinline__ void dftR(cplx<T> *a)
     else if constexpr (R == 2) dft2<T, INV>(a);
 }
 
+// Orders the LDS accesses of one wave before and after this point.  A wave's
+// LDS instructions execute in issue order, so when every thread of a sequence
+// sits in the same wave the exchange needs no block barrier, only this
+// compiler-level fence (no instruction is emitted for wavefront scope).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync()
+{
+    if constexpr (WAVE) wave_lds_sync();
+    else __syncthreads();
+}
+
 // One Stockham stage of radix Rs at span Ns over NC sequences held by the same
 // threads (v[c*EPT + m]; sequence c exchanges through lds + c*cs).  Butterfly
 // jb = j + q*TPS (q < EPT/Rs) reads v[q + r*EPT/Rs] = x[jb + r*L/Rs], applies
 // W_{Ns Rs}^{r (jb mod Ns)}, and writes y[(jb/Ns) Ns Rs + jb mod Ns + r Ns].  The
 // last stage (Ns Rs = L) leaves its outputs in v at the input positions.  The
-// NC sequences share each exchange's barriers and twiddles.
-template <class T, int L, int E, int Rs, int Ns, bool INV, int NC>
+// NC sequences share each exchange's barriers and twiddles.  WAVE: all threads
+// of each sequence are lanes of one wave (exchange without block barriers).
+template <class T, int L, int E, int Rs, int Ns, bool INV, int NC, bool WAVE = false>
 __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
                                           int twsh)
 {
@@ -134,16 +153,16 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int c
 #pragma unroll
         for (int m = 0; m < NC * EPT; m++) v[m] = o[m];
     } else {
-        __syncthreads();
+        exchange_sync<WAVE>();
 #pragma unroll
         for (int c = 0; c < NC; c++)
 #pragma unroll
             for (int m = 0; m < EPT; m++) v[c * EPT + m] = lds[c * cs + j + m * TPS];
-        __syncthreads();
+        exchange_sync<WAVE>();
     }
 }
 
-template <class T, int L, int E, int Ns, bool INV, int NC>
+template <class T, int L, int E, int Ns, bool INV, int NC, bool WAVE = false>
 __device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
                                            int twsh)
 {
@@ -151,8 +170,8 @@ __device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int 
         constexpr int rem = L / Ns;
         constexpr int EPT = FftShape<L, E>::EPT;
         constexpr int Rs = rem >= EPT ? EPT : rem;
-        fft_stage<T, L, E, Rs, Ns, INV, NC>(v, j, lds, cs, tw, twsh);
-        fft_stages<T, L, E, Ns * Rs, INV, NC>(v, j, lds, cs, tw, twsh);
+        fft_stage<T, L, E, Rs, Ns, INV, NC, WAVE>(v, j, lds, cs, tw, twsh);
+        fft_stages<T, L, E, Ns * Rs, INV, NC, WAVE>(v, j, lds, cs, tw, twsh);
     }
 }
 
@@ -171,6 +190,16 @@ __device__ __forceinline__ void fft_reg_multi(cplx<T> *v, int j, cplx<T> *lds, i
                                               int lgLmax)
 {
     if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC>(v, j, lds, cs, tw, lgLmax - FftShape<L, E>::LG);
+}
+
+// NC sequences per thread whose threads all sit in one wave: the exchanges
+// synchronise the wave only.
+template <class T, int L, bool INV, int E, int NC>
+__device__ __forceinline__ void fft_wave_multi(cplx<T> *v, int j, cplx<T> *lds, int cs,
+                                               const cplx<T> *__restrict__ tw, int lgLmax)
+{
+    static_assert(FftShape<L, E>::TPS <= 64, "a sequence must fit in one wave");
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC, true>(v, j, lds, cs, tw, lgLmax - FftShape<L, E>::LG);
 }
 
 }  // namespace mmx

@@ -253,6 +253,121 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 }
 
 // ---------------------------------------------------------------------------
+// P3, wave form (conv mode, TPS = Pz/4 <= 64): every z sequence lies in one
+// wave, so the exchanges need no block barrier and the waves of a block run
+// independently.  A block is 4 waves: waves 0 and 1 take TW = 64/TPS
+// consecutive kx each at row ky0, waves 2 and 3 the same kx at the mirror row
+// Py-ky0 (same folded tensor entries); lane = j*TW + t.  The folded tensor is
+// stored in groups of 4 kx, (comp, kyf, kx/4, kzf, kx%4), so that the entries
+// of one wave for all kz form one contiguous run.
+// ---------------------------------------------------------------------------
+#ifndef MMX_P3_WAVE
+#define MMX_P3_WAVE 1
+#endif
+#ifndef MMX_P3_NBW
+#define MMX_P3_NBW 3
+#endif
+// exchange-buffer padding per sequence, in elements (LDS banks: 16-byte
+// elements are conflict-free on the reads at +2, 8-byte ones at +4)
+#ifndef MMX_P3_PAD
+#define MMX_P3_PAD (sizeof(T) == 8 ? 2 : 4)
+#endif
+#ifndef MMX_P3W_PF
+#define MMX_P3W_PF 1
+#endif
+template <class T, int L>
+__global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict__ B, const T *__restrict__ Kt,
+                                               const T *__restrict__ Lline, int n_in, int nzb,
+                                               const cplx<T> *__restrict__ tw)
+{
+    using F = FftShape<L, MMX_P3_E>;
+    constexpr int EPT = F::EPT, TPS = F::TPS;
+    static_assert(TPS <= 64, "one sequence per wave");
+    constexpr int TW = 64 / TPS;
+    constexpr int LS = L + MMX_P3_PAD;
+    constexpr int NB = MMX_P3_NBW;
+    constexpr int WS = TW * LS;
+    __shared__ cplx<T> lds[4 * NB * WS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = lane % TW, j = lane / TW;
+    const int half = wave >> 1;
+    const int Py = G.Py, KS = G.KS;
+    const int ky0 = blockIdx.y;
+    // the mirror row exists only for 0 < ky0 < Py/2 (a whole wave's decision)
+    if (half && !(ky0 >= 1 && ky0 < Py / 2)) return;
+    const int ky = half ? Py - ky0 : ky0;
+    const int kx = blockIdx.x * (2 * TW) + (wave & 1) * TW + t;
+    const bool ok = kx < G.Kx;
+    const int kxs = ok ? kx : 0;
+    const int64_t cs_b = (int64_t)nzb * Py * KS;
+    const int64_t zs_b = (int64_t)Py * KS;
+    cplx<T> *col = B + (int64_t)ky * KS + kxs;
+    cplx<T> v[3][EPT];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int z = j + m * TPS;
+            v[c][m] = (ok && z < n_in) ? col[c * cs_b + z * zs_b] : cplx<T>{0, 0};
+        }
+    const int kyf = (ky <= Py / 2) ? ky : Py - ky;
+    const T sy = (ky <= Py / 2) ? T(1) : T(-1);
+    const int Pz2 = G.Pz2;
+    const int64_t cs = (int64_t)G.Py2 * KS * Pz2;
+    const T *kb = Kt + (((int64_t)kyf * (KS >> 2) + (kxs >> 2)) * Pz2) * 4 + (kxs & 3);
+    T kxx[EPT], kyy[EPT], kzz[EPT], kxy[EPT], kxz[EPT], kyz[EPT];
+    auto load_tensor = [&]() {
+        const T lyz = Lline[Pz2 + G.Py2 + kxs];
+        const T lxz = Lline[Pz2 + kyf];
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int kz = j + m * TPS;
+            const int kzf = (kz <= L / 2) ? kz : L - kz;
+            const T sz = (kz <= L / 2) ? T(1) : T(-1);
+            const T *kp = kb + kzf * 4;
+            kxx[m] = kp[0];
+            kyy[m] = kp[cs];
+            kzz[m] = kp[2 * cs];
+            kxy[m] = sy * kp[3 * cs] + Lline[kzf];
+            kxz[m] = sz * kp[4 * cs] + lxz;
+            kyz[m] = sy * sz * kp[5 * cs] + lyz;
+        }
+    };
+    if constexpr (MMX_P3W_PF) load_tensor();
+    cplx<T> *buf = lds + wave * NB * WS + t * LS;
+    if constexpr (NB == 3) {
+        fft_wave_multi<T, L, false, MMX_P3_E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, false, MMX_P3_E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
+    }
+    if constexpr (!MMX_P3W_PF) load_tensor();
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const T xx = kxx[m], yy = kyy[m], zz = kzz[m], xy = kxy[m], xz = kxz[m], yz = kyz[m];
+        const cplx<T> Mx = v[0][m], My = v[1][m], Mz = v[2][m];
+        v[0][m] = {xx * Mx.x + xy * My.x + xz * Mz.x, xx * Mx.y + xy * My.y + xz * Mz.y};
+        v[1][m] = {xy * Mx.x + yy * My.x + yz * Mz.x, xy * Mx.y + yy * My.y + yz * Mz.y};
+        v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
+    }
+    if constexpr (NB == 3) {
+        fft_wave_multi<T, L, true, MMX_P3_E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, true, MMX_P3_E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
+    }
+    if (!ok) return;
+    const int nz = G.nz;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int z = j + m * TPS;
+            if (z < nz) col[c * cs_b + z * zs_b] = v[c][m];
+        }
+}
+
+// ---------------------------------------------------------------------------
 // P5: half-spectrum -> real along x for RX rows x 3 components, then the
 // fused epilogue for every cell of those rows.  A element (c, row, kx) at
 // (c*ny*nz + row)*KS + kx.
@@ -795,7 +910,9 @@ __global__ void k_tensor_extract(DemagGeom G, const cplx<double> *__restrict__ B
         int kyf = (int)(q % G.Py2);
         int kzf = (int)(q / G.Py2);
         double v = Bt[((int64_t)kzf * G.Py + kyf) * G.KS + kx].x * scale;
-        Kt[(((int64_t)comp * G.Pz2 + kzf) * G.Py2 + kyf) * G.KS + kx] = (T)v;
+        const int64_t o_kt = G.ktw ? ((((int64_t)comp * G.Py2 + kyf) * (G.KS >> 2) + (kx >> 2)) * G.Pz2 + kzf) * 4 + (kx & 3)
+                                   : (((int64_t)comp * G.Pz2 + kzf) * G.Py2 + kyf) * G.KS + kx;
+        Kt[o_kt] = (T)v;
     }
 }
 
@@ -948,6 +1065,12 @@ void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, 
 #undef CY_CALL
 }
 
+int convz_grouped_tensor(const DemagGeom &G)
+{
+    const int ept = G.Pz < MMX_P3_E ? G.Pz : MMX_P3_E;
+    return (MMX_P3_WAVE && G.Pz / ept <= 64) ? 1 : 0;
+}
+
 template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in, int ncomp,
                   int fwd_only, const cplx<T> *tw, hipStream_t s)
@@ -956,6 +1079,14 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
 #define CZ_CALL(L)                                                                                   \
     {                                                                                                \
         dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py / 2 + 1);                           \
+        if constexpr (FftShape<L, MMX_P3_E>::TPS <= 64) {                                            \
+            if (!fwd_only && G.ktw) {                                                                \
+                constexpr int TW = 64 / FftShape<L, MMX_P3_E>::TPS;                                  \
+                dim3 gw((G.Kx + 2 * TW - 1) / (2 * TW), G.Py / 2 + 1);                               \
+                hipLaunchKernelGGL((k_convw<T, L>), gw, dim3(256), 0, s, G, B, Kt, Lline, n_in, nzb, tw); \
+                break;                                                                               \
+            }                                                                                        \
+        }                                                                                            \
         if (fwd_only)                                                                                \
             hipLaunchKernelGGL((k_convz<T, L, true>), grid, dim3(Shape<L>::BZ), 0, s, G, B, Kt, Lline, nzb, n_in, tw); \
         else                                                                                         \
