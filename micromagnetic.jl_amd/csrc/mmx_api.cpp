@@ -418,12 +418,24 @@ struct Engine : CtxBase {
         evs.clear();
     }
 
+    // Stage chaining: a caller sets nx_req right before an evaluation whose
+    // yout is the input of the very next evaluation; the last pass then also
+    // leaves the x-spectrum of mu0Ms*yout in Abuf (a_ready = yout), and that
+    // next evaluation skips its first pass.  Both are cleared by every eval
+    // and at every entry into the engine.
+    bool nx_req = false;
+    const T *a_ready = nullptr;
+
     // ---- one RHS evaluation with a fused epilogue; returns partial blocks --
     int eval(const T *m, const Epi<T> &E, bool generic = false)
     {
         const FieldParams<T> *P = params_dev();
         const Dyn d = dyn();
         const int fast = generic ? 0 : Plast.fast;
+        const bool have_a = demag && a_ready == m;
+        const bool want_nx = nx_req && demag && fast && (E.mode == EPI_STAGE || E.mode == EPI_RK4_FINAL);
+        a_ready = nullptr;
+        nx_req = false;
         int nb;
         if (!demag) {
             mark(MMX_PASS_LOCAL, true);
@@ -431,9 +443,11 @@ struct Engine : CtxBase {
             mark(MMX_PASS_LOCAL, false);
         } else {
             const int nx = G.nx, ny = G.ny, nz = G.nz;
-            mark(MMX_PASS_P1, true);
-            launch_p1<T>(G, m, N, mu0Ms.p, nx, ny, nz, 3, Abuf.p, tw.p, C->stream);
-            mark(MMX_PASS_P1, false);
+            if (!have_a) {
+                mark(MMX_PASS_P1, true);
+                launch_p1<T>(G, m, N, mu0Ms.p, nx, ny, nz, 3, Abuf.p, tw.p, C->stream);
+                mark(MMX_PASS_P1, false);
+            }
             mark(MMX_PASS_P2, true);
             launch_coly<T>(G, Abuf.p, Bbuf.p, ny, ny, G.Py, G.Py, nz, 3, 0, tw.p, C->stream);
             mark(MMX_PASS_P2, false);
@@ -444,7 +458,14 @@ struct Engine : CtxBase {
             launch_coly<T>(G, Bbuf.p, Abuf.p, G.Py, G.Py, ny, ny, nz, 3, 1, tw.p, C->stream);
             mark(MMX_PASS_P4, false);
             mark(MMX_PASS_P5, true);
-            nb = launch_p5<T>(G, Abuf.p, P, d, E, m, fast, Plast.sig, tw.p, C->stream);
+            if (want_nx) {
+                Epi<T> En = E;
+                En.anext = Abuf.p;
+                nb = launch_p5<T>(G, Abuf.p, P, d, En, m, fast, Plast.sig, tw.p, C->stream);
+                a_ready = E.yout;
+            } else {
+                nb = launch_p5<T>(G, Abuf.p, P, d, E, m, fast, Plast.sig, tw.p, C->stream);
+            }
             mark(MMX_PASS_P5, false);
         }
         HIPCHECK(hipGetLastError());
@@ -597,7 +618,8 @@ struct Engine : CtxBase {
 
     // RungeKutta.advance_step: src/integrator/rk.jl:23-52.  rk4_into writes the
     // new state of y0 to out (out may be stg[1] but not stg[0] or y0).
-    void rk4_into(const T *y0, T *out, double h)
+    // chain_final: the next evaluation after this step is at `out`.
+    void rk4_into(const T *y0, T *out, double h, bool chain_final = false)
     {
         ensure_k();
         T *a = stg[0], *b = stg[1];
@@ -606,13 +628,16 @@ struct Engine : CtxBase {
         E.kstore = k[0];
         E.cself = 0.5 * h;
         E.yout = a;
+        nx_req = true;
         eval(y0, E);
         E.kstore = k[1];
         E.yout = b;
+        nx_req = true;
         eval(a, E);
         E.kstore = k[2];
         E.cself = h;
         E.yout = a;
+        nx_req = true;
         eval(b, E);
         Epi<T> F = epi(EPI_RK4_FINAL);
         F.y0 = y0;
@@ -623,13 +648,14 @@ struct Engine : CtxBase {
         F.cself = 1.0 / 6 * h;
         F.yout = out;
         F.normalize = 1;
+        nx_req = chain_final;
         eval(a, F);
     }
 
-    void rk4_step()
+    void rk4_step(bool more = false)
     {
         const double h = C->step;
-        rk4_into(y, stg[1], h);
+        rk4_into(y, stg[1], h, more);
         swap_y(stg[1]);  // new state in stg[1]'s buffer; the previous state stays in stg[1] (prespin)
         C->nfevals += 4;
         C->nsteps += 1;
@@ -637,7 +663,7 @@ struct Engine : CtxBase {
     }
 
     // ModifiedEuler.advance_step: src/integrator/heun.jl:28-49
-    void heun_step()
+    void heun_step(bool more = false)
     {
         ensure_k();
         const double h = C->step;
@@ -647,6 +673,7 @@ struct Engine : CtxBase {
         E.kstore = k[0];
         E.cself = h;
         E.yout = a;
+        nx_req = true;
         eval(y, E);
         Epi<T> F = epi(EPI_STAGE);
         F.y0 = y;
@@ -656,6 +683,7 @@ struct Engine : CtxBase {
         F.cself = 0.5 * h;
         F.yout = b;
         F.normalize = 1;
+        nx_req = more;
         eval(a, F);
         swap_y(stg[1]);
         C->nfevals += 2;
@@ -684,6 +712,7 @@ struct Engine : CtxBase {
         E.np = 0;
         E.cself = a1 * step;
         E.yout = A;
+        nx_req = true;
         eval(y, E);
         // k2
         E.kstore = k[1];
@@ -692,6 +721,7 @@ struct Engine : CtxBase {
         E.np = 1;
         E.cself = a2[1] * step;
         E.yout = B;
+        nx_req = true;
         eval(A, E);
         // k3
         E.kstore = k[2];
@@ -701,6 +731,7 @@ struct Engine : CtxBase {
         E.np = 2;
         E.cself = a3[2] * step;
         E.yout = A;
+        nx_req = true;
         eval(B, E);
         // k4
         E.kstore = k[3];
@@ -709,6 +740,7 @@ struct Engine : CtxBase {
         E.np = 3;
         E.cself = a4[3] * step;
         E.yout = B;
+        nx_req = true;
         eval(A, E);
         // k5
         E.kstore = k[4];
@@ -717,6 +749,7 @@ struct Engine : CtxBase {
         E.np = 4;
         E.cself = a5[4] * step;
         E.yout = A;
+        nx_req = true;
         eval(B, E);
         // k6 -> y_next = y + a61 k1 + a63 k3 + a64 k4 + a65 k5 + a66 k6, normalised
         E.kstore = k[5];
@@ -732,6 +765,7 @@ struct Engine : CtxBase {
         E.cself = a6[5] * step;
         E.yout = B;
         E.normalize = 1;
+        nx_req = true;
         eval(A, E);
         // k7 at y_next and the error combination (vector_add6b)
         Epi<T> R = epi(EPI_ERR);
@@ -768,6 +802,7 @@ struct Engine : CtxBase {
             E.kstore = k[0];
             E.cself = step * 0.5;
             E.yout = A;
+            nx_req = true;
             eval(y, E);
             C->nfevals += 1;
         }
@@ -776,6 +811,7 @@ struct Engine : CtxBase {
         E.np = 0;
         E.cself = step * 0.75;
         E.yout = B;
+        nx_req = true;
         eval(A, E);
         // k3 -> y_next = y + step (2/9 k1 + 1/3 k2 + 4/9 k3), normalised
         Epi<T> S = epi(EPI_STAGE_SCALED);
@@ -908,8 +944,8 @@ struct Engine : CtxBase {
     int advance(int64_t n)
     {
         for (int64_t i = 0; i < n; i++) {
-            if (C->ikind == MMX_RK4) rk4_step();
-            else if (C->ikind == MMX_HEUN) heun_step();
+            if (C->ikind == MMX_RK4) rk4_step(i + 1 < n);
+            else if (C->ikind == MMX_HEUN) heun_step(i + 1 < n);
             else if (C->ikind == MMX_SD) sd_step();
             else {
                 int rc = adaptive_step();
@@ -1550,8 +1586,16 @@ static int with_engine(mmx_ctx *c, F &&f)
 {
     if (!c) return fail(MMX_EINVAL, "null context");
     try {
-        if (c->mesh.precision == MMX_F64) return f(*static_cast<Engine<double> *>(c->impl.get()));
-        return f(*static_cast<Engine<float> *>(c->impl.get()));
+        if (c->mesh.precision == MMX_F64) {
+            auto &e = *static_cast<Engine<double> *>(c->impl.get());
+            e.nx_req = false;
+            e.a_ready = nullptr;
+            return f(e);
+        }
+        auto &e = *static_cast<Engine<float> *>(c->impl.get());
+        e.nx_req = false;
+        e.a_ready = nullptr;
+        return f(e);
     } catch (const HipError &e) {
         return e.code;
     } catch (const std::bad_alloc &) {

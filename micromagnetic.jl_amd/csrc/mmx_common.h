@@ -178,6 +178,9 @@ struct AdjIO {
 };
 
 template <class T>
+struct cplx;
+
+template <class T>
 struct Epi {
     int mode;
     uint32_t term_mask;  // which terms enter the field (bit per term index)
@@ -194,6 +197,7 @@ struct Epi {
     int normalize;
     double *partial;     // one slot per block (ERR: max; ENERGY: nterms slots per block)
     AdjIO<T> adj;
+    cplx<T> *anext;      // last demag pass, STAGE / RK4_FINAL: x-spectrum of mu0Ms*yout (nullable)
 };
 
 // One stage of one member of a batch (many independent systems advanced by

@@ -709,10 +709,11 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
 }
 
 // Epilogue of the specialised kernels: MODE is EPI_STAGE, EPI_RK4_FINAL or EPI_ERR.
+// ynext (nullable): receives the stored new state of the cell (STAGE / RK4_FINAL).
 template <class T, int MODE, bool NOKP, uint32_t SIG>
 __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I,
                                             int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
-                                            double &red)
+                                            double &red, T *ynext = nullptr)
 {
 #pragma clang fp contract(off)
     const int N = (int)P.N;
@@ -802,6 +803,11 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
         gst(E.yout, I, y[0]);
         gst(E.yout, N + I, y[1]);
         gst(E.yout, 2 * N + I, y[2]);
+        if (ynext) {
+            ynext[0] = y[0];
+            ynext[1] = y[1];
+            ynext[2] = y[2];
+        }
     }
 }
 

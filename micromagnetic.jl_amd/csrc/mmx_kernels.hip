@@ -75,24 +75,7 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
     }
     cplx<T> *buf = lds + ls * HS;
     fft_reg<T, H, false>(v, j, buf, tw, G.lgLmax);
-#pragma unroll
-    for (int m = 0; m < EPT; m++) buf[j + m * TPS] = v[m];
-    __syncthreads();
-    if (!active) return;
-    const int sh = G.lgLmax - (FftShape<H>::LG + 1);  // W_Px^k = tw[k << sh]
-    cplx<T> *dst = A + ((int64_t)c * nrows + row) * G.KS;
-#pragma unroll
-    for (int m = 0; m < EPT; m++) {
-        const int k = j + m * TPS;
-        cplx<T> Zk = v[m];
-        cplx<T> Zc = buf[(H - k) & (H - 1)];
-        Zc.y = -Zc.y;
-        cplx<T> Fe = {(Zk.x + Zc.x) * T(0.5), (Zk.y + Zc.y) * T(0.5)};
-        cplx<T> d = csub(Zk, Zc);
-        cplx<T> Fo = {d.y * T(0.5), -d.x * T(0.5)};
-        dst[k] = cadd(Fe, cmul(tw[k << sh], Fo));
-        if (k == 0) dst[H] = csub(Fe, Fo);  // W_Px^H = -1
-    }
+    r2c_split_store<T, H>(v, j, buf, active, A + ((int64_t)c * nrows + row) * G.KS, tw, G.lgLmax);
 }
 
 // ---------------------------------------------------------------------------

@@ -50,10 +50,44 @@ struct VMode {
 
 template <class T, int V, uint32_t SIG = 0>
 __device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
-                                         int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red)
+                                         int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red,
+                                         T *ynext = nullptr)
 {
     if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
-    else finish_fast<T, VMode<V>::mode, VMode<V>::nokp, SIG>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red);
+    else finish_fast<T, VMode<V>::mode, VMode<V>::nokp, SIG>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red, ynext);
+}
+
+// Real-to-complex x transform tail shared by the first and the last pass: v
+// holds this thread's outputs Z[j + q*TPS] of the half-length complex FFT of
+// one row packed as z[n] = (x[2n], x[2n+1]); buf is that sequence's exchange
+// buffer (all of its threads arrive here together).  Writes the Px/2+1
+// spectrum values X[k] = Fe + W_Px^k Fo (X[H] = Fe(0) - Fo(0)) to dst.
+template <class T, int H>
+__device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T> *buf, bool active,
+                                                cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int lgLmax)
+{
+    constexpr int EPT = FftShape<H>::EPT, TPS = FftShape<H>::TPS;
+#pragma unroll
+    for (int m = 0; m < EPT; m++) buf[j + m * TPS] = v[m];
+    __syncthreads();
+    if (!active) return;
+    // the twiddle pointer passes through an empty asm statement after the
+    // barrier so that these loads are not hoisted over the transform (where
+    // their registers would add to its peak)
+    asm volatile("" : "+s"(tw));
+    const int sh = lgLmax - (FftShape<H>::LG + 1);  // W_Px^k = tw[k << sh]
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int k = j + m * TPS;
+        cplx<T> Zk = v[m];
+        cplx<T> Zc = buf[(H - k) & (H - 1)];
+        Zc.y = -Zc.y;
+        cplx<T> Fe = {(Zk.x + Zc.x) * T(0.5), (Zk.y + Zc.y) * T(0.5)};
+        cplx<T> d = csub(Zk, Zc);
+        cplx<T> Fo = {d.y * T(0.5), -d.x * T(0.5)};
+        dst[k] = cadd(Fe, cmul(tw[k << sh], Fo));
+        if (k == 0) dst[H] = csub(Fe, Fo);  // W_Px^H = -1
+    }
 }
 
 // at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
@@ -117,6 +151,12 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
     __syncthreads();
     const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     double red = 0;
+    // next spectrum (STAGE / RK4_FINAL of the specialised epilogues): the cell
+    // replaces its demag value in LDS by mu0Ms * (new state), and the rows are
+    // transformed as the first pass would transform them for the next stage
+    constexpr bool NXOK = V == 1 || V == 2 || V == 4;
+    const bool nx_on = NXOK && E.anext != nullptr;
+    T *ldsr = reinterpret_cast<T *>(lds);
     // (row, x) of this thread's cell, advanced without dividing
     int rr = (int)threadIdx.x / nx, x = (int)threadIdx.x - rr * nx;
     for (int idx = threadIdx.x; idx < nrows * nx; idx += blockDim.x) {
@@ -128,14 +168,53 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         const T hx = (x & 1) ? h0.y : h0.x;
         const T hy = (x & 1) ? h1.y : h1.x;
         const T hz = (x & 1) ? h2.y : h2.x;
-        epilogue<T, V, SIG>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
+        if constexpr (NXOK) {
+            T yn[3];
+            const int I = rw * nx + x;
+            epilogue<T, V, SIG>(P, dyn, E, (int64_t)I, x, y, z, m, hx, hy, hz, red, yn);
+            if (nx_on) {
+                const T ms = gld(P.mu0Ms, I);
+#pragma unroll
+                for (int c = 0; c < 3; c++) ldsr[2 * (c * RX + rr) * HS + x] = yn[c] * ms;
+            }
+        } else {
+            epilogue<T, V, SIG>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
+        }
         x += blockDim.x;
         while (x >= nx) {
             x -= nx;
             rr++;
         }
     }
-    if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
+    if constexpr (!NXOK) {
+        if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
+    }
+    if (NXOK && nx_on) {
+        __syncthreads();
+        // the thread's sequence index and buffer pass through an empty asm
+        // statement: otherwise the compiler keeps the first transform's LDS
+        // addresses live through the cell loop to reuse them here, at the
+        // cost of occupancy
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int jt = tid % TPS;
+        cplx<T> *bt = lds + (tid / TPS) * HS;  // (c*RX + r)*HS
+        // row r of component c: z[n] = (M[2n], M[2n+1]) for 2n < nx, zero beyond
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            const int k = jt + q * TPS;
+            cplx<T> Z = {0, 0};
+            if (active && 2 * k < nx) {
+                Z = bt[k];
+                if (2 * k + 1 >= nx) Z.y = 0;
+            }
+            v[q] = Z;
+        }
+        __syncthreads();
+        fft_reg<T, H, false>(v, jt, bt, tw, G.lgLmax);
+        cplx<T> *dst = E.anext + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
+        r2c_split_store<T, H>(v, jt, bt, active, dst, tw, G.lgLmax);
+    }
 }
 
 // ---------------------------------------------------------------------------
