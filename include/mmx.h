@@ -93,6 +93,26 @@ int mmx_add_demag(mmx_ctx *ctx, int Nx, int Ny, int Nz);
 /* add_thermal_noise(sim, T; T0): src/micro/add_field.jl:734-771.  T per cell (or NULL
  * with T_uniform).  The noise is Philox4x32-10 keyed by (seed, integrator step, index). */
 int mmx_add_thermal_noise(mmx_ctx *ctx, const double *T, double T_uniform, double T0, uint64_t seed);
+/* Spin torques enter H_eff as further terms in insertion order and carry no
+ * energy (compute_system_energy skips them, src/micro/field.jl:520-528).
+ * add_sot(sim, aj, bj, p): src/micro/add_field.jl:951-966, df_torque_kernel!
+ * src/micro/kernels.jl:619-633 -- aj per cell (or NULL with aj0), p not normalised. */
+int mmx_add_sot(mmx_ctx *ctx, const double *aj, double aj0, double bj, double px, double py, double pz);
+/* add_stt(sim; model=:slonczewski, ...): src/micro/add_field.jl:1195-1227,
+ * slonczewski_torque_kernel! src/micro/kernels.jl:657-674.  J per cell (or NULL with
+ * J0); beta = hbar/(mu0 Ms e tf) as the reference computes it. */
+int mmx_add_stt_slonczewski(mmx_ctx *ctx, const double *J, double J0, double beta, double Lambda, double xi,
+                            double P, double px, double py, double pz);
+/* add_stt(sim; model=:zhang_li, ...): src/micro/add_field.jl:1150-1192,
+ * zhangli_torque_kernel! src/micro/kernels.jl:680-739.  J = 3N AoS current density (or
+ * NULL with (Jx,Jy,Jz)), b the coefficient (b, or P mu_B/(e Ms)); xi per cell (or NULL
+ * with xi0); xi_factor = 1 applies 1/(1 + xi^2) as the (P, Ms) form does. */
+int mmx_add_stt_zhangli(mmx_ctx *ctx, const double *J, double Jx, double Jy, double Jz, double b,
+                        const double *xi, double xi0, int xi_factor);
+/* The ft / ufun time profile of a Slonczewski or Zhang-Li term (field.jl:335,351):
+ * called on the host at every evaluation time (stage times inside the steppers);
+ * fn = NULL restores the constant 1. */
+int mmx_set_term_time_fn(mmx_ctx *ctx, int term, double (*fn)(double t, void *user), void *user);
 int mmx_num_terms(mmx_ctx *ctx);
 
 /* effective_field(sim, spin, t): src/micro/field.jl:496-505 -- sim.field (3N) */

@@ -93,6 +93,8 @@ struct HostTerm {
     double thermal_T0 = 0;
     uint64_t seed = 0;
     std::vector<double> rtensor;  // demag: compact real-space tensor (6N)
+    double (*ufun)(double, void *) = nullptr;  // torque amplitude ufun(t) (null: 1)
+    void *uuser = nullptr;
 };
 
 struct CtxBase {
@@ -307,7 +309,13 @@ struct Engine : CtxBase {
             case K_CUBIC: sk = SL_CUBIC; break;
             case K_ZEEMAN: case K_ZEEMAN_U: sk = SL_ZEE; break;
             case K_DEMAG: sk = SL_DEMAG; break;
-            default: sk = SL_THERM; break;
+            case K_THERMAL: sk = SL_THERM; break;
+            default: sk = -1; break;  // torque terms: generic kernels only
+            }
+            if (sk < 0) {
+                P.fast = 0;
+                P.ord[i] = 0;
+                continue;
             }
             if (P.slot[sk] >= 0) P.fast = 0;
             P.slot[sk] = i;
@@ -364,12 +372,22 @@ struct Engine : CtxBase {
         return P;
     }
 
+    // time of the next evaluation (a stage time inside the steppers); enters
+    // only through the torque terms' ufun(t)
+    double t_eval = 0;
+
     Dyn dyn() const
     {
         Dyn d{};
         // stochastic field factor, src/micro/field.jl:261 (dt = integrator.step)
         d.thermal_factor = (C->step > 0) ? 2 * C->alpha * KB / (C->volume * C->gamma * C->step) : 0.0;
         d.noise_step = (uint64_t)C->nsteps;
+        for (size_t i = 0; i < C->terms.size(); i++) {
+            const HostTerm &h = C->terms[i];
+            const double u = h.ufun ? h.ufun(t_eval, h.uuser) : 1.0;
+            if (h.kind == K_STT_SL) d.tfac[i] = u * h.s[7];    // ft = ufun(t)*beta, field.jl:351
+            if (h.kind == K_STT_ZL) d.tfac[i] = u / C->gamma;  // ut = ufun(t)/gamma, field.jl:335
+        }
         return d;
     }
 
@@ -623,16 +641,19 @@ struct Engine : CtxBase {
     {
         ensure_k();
         T *a = stg[0], *b = stg[1];
+        const double t0 = C->t;  // rk.jl:31-43: t, t + h/2, t + h/2, t + h
         Epi<T> E = epi(EPI_STAGE);
         E.y0 = y0;
         E.kstore = k[0];
         E.cself = 0.5 * h;
         E.yout = a;
         nx_req = true;
+        t_eval = t0;
         eval(y0, E);
         E.kstore = k[1];
         E.yout = b;
         nx_req = true;
+        t_eval = t0 + 0.5 * h;
         eval(a, E);
         E.kstore = k[2];
         E.cself = h;
@@ -649,6 +670,7 @@ struct Engine : CtxBase {
         F.yout = out;
         F.normalize = 1;
         nx_req = chain_final;
+        t_eval = t0 + h;
         eval(a, F);
     }
 
@@ -674,6 +696,7 @@ struct Engine : CtxBase {
         E.cself = h;
         E.yout = a;
         nx_req = true;
+        t_eval = C->t;  // heun.jl:34-38: t, t + h
         eval(y, E);
         Epi<T> F = epi(EPI_STAGE);
         F.y0 = y;
@@ -684,6 +707,7 @@ struct Engine : CtxBase {
         F.yout = b;
         F.normalize = 1;
         nx_req = more;
+        t_eval = C->t + h;
         eval(a, F);
         swap_y(stg[1]);
         C->nfevals += 2;
@@ -703,6 +727,9 @@ struct Engine : CtxBase {
         static const double a6[6] = {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84};
         static const double w[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525,
                                     -1.0 / 40};
+        // stage times t + c_i h (adaptive_rk_steps.jl:50,67-96)
+        static const double c[6] = {1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
+        const double t0 = C->t;
         ensure_k();
         T *A = stg[0], *B = stg[1];
         Epi<T> E = epi(EPI_STAGE);
@@ -713,6 +740,7 @@ struct Engine : CtxBase {
         E.cself = a1 * step;
         E.yout = A;
         nx_req = true;
+        t_eval = t0;
         eval(y, E);
         // k2
         E.kstore = k[1];
@@ -722,6 +750,7 @@ struct Engine : CtxBase {
         E.cself = a2[1] * step;
         E.yout = B;
         nx_req = true;
+        t_eval = t0 + c[0] * step;
         eval(A, E);
         // k3
         E.kstore = k[2];
@@ -732,6 +761,7 @@ struct Engine : CtxBase {
         E.cself = a3[2] * step;
         E.yout = A;
         nx_req = true;
+        t_eval = t0 + c[1] * step;
         eval(B, E);
         // k4
         E.kstore = k[3];
@@ -741,6 +771,7 @@ struct Engine : CtxBase {
         E.cself = a4[3] * step;
         E.yout = B;
         nx_req = true;
+        t_eval = t0 + c[2] * step;
         eval(A, E);
         // k5
         E.kstore = k[4];
@@ -750,6 +781,7 @@ struct Engine : CtxBase {
         E.cself = a5[4] * step;
         E.yout = A;
         nx_req = true;
+        t_eval = t0 + c[3] * step;
         eval(B, E);
         // k6 -> y_next = y + a61 k1 + a63 k3 + a64 k4 + a65 k5 + a66 k6, normalised
         E.kstore = k[5];
@@ -766,6 +798,7 @@ struct Engine : CtxBase {
         E.yout = B;
         E.normalize = 1;
         nx_req = true;
+        t_eval = t0 + step;
         eval(A, E);
         // k7 at y_next and the error combination (vector_add6b)
         Epi<T> R = epi(EPI_ERR);
@@ -782,6 +815,7 @@ struct Engine : CtxBase {
         R.cp[4] = w[5];
         R.np = 5;
         R.cself = w[6];
+        t_eval = t0 + step;
         int nb = eval(B, R);
         C->nfevals += 7;
         return reduce_scalar(nb, true) * step / C->tol;
@@ -803,6 +837,7 @@ struct Engine : CtxBase {
             E.cself = step * 0.5;
             E.yout = A;
             nx_req = true;
+            t_eval = t;  // adaptive_rk_steps.jl:22-37: t, t + step/2, t + 3 step/4, t + step
             eval(y, E);
             C->nfevals += 1;
         }
@@ -812,6 +847,7 @@ struct Engine : CtxBase {
         E.cself = step * 0.75;
         E.yout = B;
         nx_req = true;
+        t_eval = t + (1.0 / 2) * step;
         eval(A, E);
         // k3 -> y_next = y + step (2/9 k1 + 1/3 k2 + 4/9 k3), normalised
         Epi<T> S = epi(EPI_STAGE_SCALED);
@@ -826,6 +862,7 @@ struct Engine : CtxBase {
         S.outer = step;
         S.yout = A;
         S.normalize = 1;
+        t_eval = t + (3.0 / 4) * step;
         eval(B, S);
         // k4 at y_next and the error combination
         Epi<T> R = epi(EPI_ERR);
@@ -838,6 +875,7 @@ struct Engine : CtxBase {
         R.cp[2] = 1.0 / 9;
         R.np = 3;
         R.cself = -1.0 / 8;
+        t_eval = t + step;
         int nb = eval(A, R);
         C->nfevals += 3;
         // keep the candidate in stg[1] as the DOPRI path does
@@ -855,6 +893,7 @@ struct Engine : CtxBase {
         C->step = 1e-15;
         Epi<T> E = epi(EPI_RHS);
         E.out = k[6];
+        t_eval = C->t;
         eval(y, E);
         double mx = maxabs(k[6]);
         double r_step = mx / (0.824 * std::pow(C->tol, 1.0 / order_main));
@@ -920,6 +959,7 @@ struct Engine : CtxBase {
         E.out = h;
         const double save_step = C->step;
         C->step = 0;  // the stochastic field factor needs a time step; SD has none
+        t_eval = 0;
         eval(y, E);
         C->step = save_step;
         if (C->sd_steps == 0) {  // compute_gk_kernel_1!, tau = min_tau
@@ -978,6 +1018,9 @@ struct Engine : CtxBase {
     {
         if (C->ikind != MMX_RK4) return fail(MMX_ESTATE, "the adjoint needs the RungeKutta (RK4) integrator");
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
+        for (const auto &h : C->terms)
+            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL)
+                return fail(MMX_EINVAL, "the adjoint covers field terms without spin torques");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
         const int64_t nck = (nsteps + every - 1) / every;
         adj.recorded = false;
@@ -1989,15 +2032,113 @@ int mmx_add_thermal_noise(mmx_ctx *c, const double *T, double T_uniform, double 
     });
 }
 
+int mmx_add_sot(mmx_ctx *c, const double *aj, double aj0, double bj, double px, double py, double pz)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (aj && !finite_all(aj, c->N)) return fail(MMX_EINVAL, "NaN in aj");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // DFTorqueField(T(p), aj array, T(bj)): add_field.jl:951-966
+        h.kind = K_SOT;
+        h.s[0] = (double)(TT)px;
+        h.s[1] = (double)(TT)py;
+        h.s[2] = (double)(TT)pz;
+        h.s[4] = (double)(TT)bj;
+        const TT *a = nullptr;
+        if (aj) a = e.new_array(aj, c->N, false);
+        else {
+            h.uniform = 1;
+            h.s[3] = (double)(TT)aj0;
+        }
+        return add_term(c, e, h, a, nullptr, nullptr);
+    });
+}
+
+int mmx_add_stt_slonczewski(mmx_ctx *c, const double *J, double J0, double beta, double Lambda, double xi,
+                            double P, double px, double py, double pz)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (J && !finite_all(J, c->N)) return fail(MMX_EINVAL, "NaN in J");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // SlonczewskiTorque: add_field.jl:1195-1227; lambda_sq = T(Lambda^2), field.jl:349
+        h.kind = K_STT_SL;
+        h.s[0] = (double)(TT)px;
+        h.s[1] = (double)(TT)py;
+        h.s[2] = (double)(TT)pz;
+        h.s[4] = (double)(TT)((double)(TT)Lambda * (double)(TT)Lambda);
+        h.s[5] = (double)(TT)P;
+        h.s[6] = (double)(TT)xi;
+        h.s[7] = (double)(TT)beta;
+        const TT *a = nullptr;
+        if (J) a = e.new_array(J, c->N, false);
+        else {
+            h.uniform = 1;
+            h.s[3] = (double)(TT)J0;
+        }
+        return add_term(c, e, h, a, nullptr, nullptr);
+    });
+}
+
+int mmx_add_stt_zhangli(mmx_ctx *c, const double *J, double Jx, double Jy, double Jz, double b, const double *xi,
+                        double xi0, int xi_factor)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    const int64_t N = c->N;
+    if (J && !finite_all(J, 3 * N)) return fail(MMX_EINVAL, "NaN in J");
+    if (xi && !finite_all(xi, N)) return fail(MMX_EINVAL, "NaN in xi");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        // ZhangLiTorque: add_field.jl:1150-1192.  bJ = T(J) .* b in T, then, for
+        // (P, Ms) input, times 1/(1 + xi^2) evaluated in T
+        std::vector<double> bJ(3 * N);
+        for (int64_t i = 0; i < N; i++) {
+            const TT x = (TT)(xi ? xi[i] : xi0);
+            const TT f = TT(1) / (TT(1) + x * x);
+            for (int q = 0; q < 3; q++) {
+                const TT Jv = (TT)(J ? J[3 * i + q] : (q == 0 ? Jx : (q == 1 ? Jy : Jz)));
+                TT v = (TT)((double)Jv * b);
+                if (xi_factor) v = v * f;
+                bJ[3 * i + q] = (double)v;
+            }
+        }
+        HostTerm h;
+        h.kind = K_STT_ZL;
+        h.s[0] = (double)(TT)c->mesh.dx;
+        h.s[1] = (double)(TT)c->mesh.dy;
+        h.s[2] = (double)(TT)c->mesh.dz;
+        const TT *xa = nullptr;
+        if (xi) xa = e.new_array(xi, N, false);
+        else {
+            h.uniform = 1;
+            h.s[3] = (double)(TT)xi0;
+        }
+        const TT *bb = e.new_array(bJ.data(), 3 * N, true);
+        return add_term(c, e, h, xa, bb, nullptr);
+    });
+}
+
+int mmx_set_term_time_fn(mmx_ctx *c, int term, double (*fn)(double t, void *user), void *user)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (term < 0 || term >= (int)c->terms.size()) return fail(MMX_EINVAL, "no term %d", term);
+    HostTerm &h = c->terms[term];
+    if (h.kind != K_STT_SL && h.kind != K_STT_ZL)
+        return fail(MMX_EINVAL, "term %d has no time-dependent amplitude", term);
+    h.ufun = fn;
+    h.uuser = user;
+    return MMX_OK;
+}
+
 int mmx_num_terms(mmx_ctx *c) { return c ? (int)c->terms.size() : fail(MMX_EINVAL, "null context"); }
 
 int mmx_effective_field(mmx_ctx *c, double t, double *H)
 {
-    (void)t;
     return with_engine(c, [&](auto &e) {
         ensure_out(e);
         auto E = e.epi(EPI_FIELD);
         E.out = e.k[6];
+        e.t_eval = t;
         e.eval(e.y, E);
         if (H) e.download_aos(e.k[6], H);
         else HIPCHECK(hipStreamSynchronize(c->stream));
@@ -2007,7 +2148,6 @@ int mmx_effective_field(mmx_ctx *c, double t, double *H)
 
 int mmx_term_field(mmx_ctx *c, int term, double t, double *H, double *Eo)
 {
-    (void)t;
     if (!c) return fail(MMX_EINVAL, "null context");
     if (term < 0 || term >= (int)c->terms.size()) return fail(MMX_EINVAL, "no such term");
     return with_engine(c, [&](auto &e) {
@@ -2016,6 +2156,7 @@ int mmx_term_field(mmx_ctx *c, int term, double t, double *H, double *Eo)
         E.term_mask = 1u << term;
         E.out = e.k[6];
         E.e_out = e.k[5];
+        e.t_eval = t;
         e.eval(e.y, E);
         if (H) e.download_aos(e.k[6], H);
         if (Eo) e.download_scalar(e.k[5], Eo, c->N);
@@ -2026,14 +2167,14 @@ int mmx_term_field(mmx_ctx *c, int term, double t, double *H, double *Eo)
 
 int mmx_compute_energies(mmx_ctx *c, double t, double *Eo)
 {
-    (void)t;
     if (!c || !Eo) return fail(MMX_EINVAL, "null argument");
     return with_engine(c, [&](auto &e) {
         // one masked evaluation per term; each block writes its FP64 partial sum
-        for (int t = 0; t < (int)c->terms.size(); t++) {
+        e.t_eval = t;
+        for (int q = 0; q < (int)c->terms.size(); q++) {
             auto E = e.epi(EPI_ENERGY);
-            E.term_mask = 1u << t;
-            Eo[t] = e.reduce_scalar(e.eval(e.y, E), false);
+            E.term_mask = 1u << q;
+            Eo[q] = e.reduce_scalar(e.eval(e.y, E), false);
         }
         return MMX_OK;
     });
@@ -2050,7 +2191,6 @@ int mmx_set_llg(mmx_ctx *c, double alpha, double gamma, int precession)
 
 int mmx_llg_rhs(mmx_ctx *c, double t, const double *m, double *dmdt)
 {
-    (void)t;
     if (!c || !dmdt) return fail(MMX_EINVAL, "null argument");
     return with_engine(c, [&](auto &e) {
         ensure_out(e);
@@ -2061,6 +2201,7 @@ int mmx_llg_rhs(mmx_ctx *c, double t, const double *m, double *dmdt)
         }
         auto E = e.epi(EPI_RHS);
         E.out = e.k[6];
+        e.t_eval = t;
         e.eval(src, E);
         e.download_aos(e.k[6], dmdt);
         return MMX_OK;
@@ -2261,6 +2402,8 @@ int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n)
             return fail(MMX_EINVAL, "member %d: batches run the fixed-step RungeKutta or Heun integrators", i);
         if (c->ikind != members[0]->ikind) return fail(MMX_EINVAL, "members must use the same integrator");
         if (!(c->step > 0)) return fail(MMX_EINVAL, "member %d has no step size", i);
+        for (const auto &h : c->terms)
+            if (h.ufun) return fail(MMX_EINVAL, "member %d has a time-dependent torque; batches take constant ones", i);
         for (int j = 0; j < i; j++)
             if (members[j] == c) return fail(MMX_EINVAL, "member %d appears twice", i);
     }

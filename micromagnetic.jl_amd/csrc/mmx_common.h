@@ -44,6 +44,9 @@ enum TermKind : int {
     K_CUBIC,        // cubic, Kc (a) or uniform s[9]; axes s[0..8]
     K_DEMAG,        // FFT demag; its field comes from the convolution passes
     K_THERMAL,      // stochastic field, T (a) or uniform s[1]; s[0] = T0
+    K_SOT,          // damping/field-like torque, aj (a) or uniform s[3]; p s[0..2], bj s[4]
+    K_STT_SL,       // Slonczewski torque, J (a) or uniform s[3]; p s[0..2], Lambda^2 s[4], P s[5], xi s[6]
+    K_STT_ZL,       // Zhang-Li torque, b*J planes (b), xi (a) or uniform s[3]; dx, dy, dz s[0..2]
 };
 
 template <class T>
@@ -144,6 +147,8 @@ constexpr uint32_t sig_ordp(uint32_t sig)
 struct Dyn {
     double thermal_factor;  // 2 alpha kB / (V gamma dt), src/micro/field.jl:261
     uint64_t noise_step;    // integrator step count keying the noise stream
+    double tfac[MAX_TERMS]; // time factor of a torque term at the evaluation time
+                            // (Slonczewski: ufun(t)*beta; Zhang-Li: ufun(t)/gamma)
 };
 
 // What the last pass does with the field of a cell.

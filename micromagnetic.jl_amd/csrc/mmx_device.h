@@ -325,6 +325,48 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             hz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
             e = (double)(-Ms * V * (mx * hx + my * hy + mz * hz));
         } break;
+        case K_SOT: {  // df_torque_kernel!: kernels.jl:619-633 (no energy, no Ms rule)
+            const T a = (D.uniform ? D.s[3] : D.a[I]) / P.gamma;
+            const T b = D.s[4] / P.gamma;
+            const T px = D.s[0], py = D.s[1], pz = D.s[2];
+            hx = a * (-mz * py + my * pz) + b * px;
+            hy = a * (mz * px - mx * pz) + b * py;
+            hz = a * (-my * px + mx * py) + b * pz;
+        } break;
+        case K_STT_SL: {  // slonczewski_torque_kernel!: kernels.jl:657-674
+            const T px = D.s[0], py = D.s[1], pz = D.s[2];
+            const T lsq = D.s[4], Pp = D.s[5], xi = D.s[6];
+            const T mp = mx * px + my * py + mz * pz;
+            const T eps = Pp * lsq / (lsq + T(1) + (lsq - T(1)) * mp);
+            const T a = (D.uniform ? D.s[3] : D.a[I]) * (T)dyn.tfac[tt];
+            hx = a * (eps * (-mz * py + my * pz) + xi * px);
+            hy = a * (eps * (mz * px - mx * pz) + xi * py);
+            hz = a * (eps * (-my * px + mx * py) + xi * pz);
+        } break;
+        case K_STT_ZL: {  // zhangli_torque_kernel!: kernels.jl:680-739 (mesh neighbours only)
+            T gx = 0, gy = 0, gz = 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int64_t i1 = nb[2 * d], i2 = nb[2 * d + 1];
+                const T dd = D.s[d];
+                const T factor = (i1 >= 0 && i2 >= 0) ? T(1) / (T(2) * dd) : T(1) / dd;
+                const T m1x = i1 >= 0 ? nmx[2 * d] : mx, m1y = i1 >= 0 ? nmy[2 * d] : my, m1z = i1 >= 0 ? nmz[2 * d] : mz;
+                const T m2x = i2 >= 0 ? nmx[2 * d + 1] : mx, m2y = i2 >= 0 ? nmy[2 * d + 1] : my,
+                        m2z = i2 >= 0 ? nmz[2 * d + 1] : mz;
+                const T u = D.b[d * N + I] * factor;
+                gx += u * (m2x - m1x);
+                gy += u * (m2y - m1y);
+                gz += u * (m2z - m1z);
+            }
+            const T ut = (T)dyn.tfac[tt];
+            gx = ut * gx;
+            gy = ut * gy;
+            gz = ut * gz;
+            const T xi = D.uniform ? D.s[3] : D.a[I];
+            hx = (-mz * gy + my * gz) + xi * gx;
+            hy = (mz * gx - mx * gz) + xi * gy;
+            hz = (-my * gx + mx * gy) + xi * gz;
+        } break;
         default: break;
         }
         Hx = Hx + hx;

@@ -12,12 +12,33 @@
 
 namespace mmx {
 
+// Exchange-buffer index map: element i of a sequence sits at i + (i >> SW)
+// (one gap every 2^SW elements), which spreads the strided writes of the
+// early stages over the LDS banks; SW = 0 disables the gaps.  SW8 applies to
+// the 8-element-per-thread transforms, SW4 to the 4-element ones.
+#ifndef MMX_FFT_SW8
+#define MMX_FFT_SW8 0
+#endif
+#ifndef MMX_FFT_SW4
+#define MMX_FFT_SW4 0
+#endif
+
 template <int L, int E = 8>
 struct FftShape {
     static constexpr int EPT = L >= E ? E : L;
     static constexpr int TPS = L / EPT;
     static constexpr int LG = L <= 1 ? 0 : (L <= 2 ? 1 : (L <= 4 ? 2 : (L <= 8 ? 3 : (L <= 16 ? 4 : (L <= 32 ? 5 : (L <= 64 ? 6 : (L <= 128 ? 7 : (L <= 256 ? 8 : (L <= 512 ? 9 : (L <= 1024 ? 10 : (L <= 2048 ? 11 : 12)))))))))));
+    static constexpr int SW = E >= 8 ? MMX_FFT_SW8 : MMX_FFT_SW4;
+    // elements one sequence's exchange buffer spans
+    static constexpr int XS = SW ? L + (L >> SW) : L;
 };
+
+template <int SW>
+__device__ __forceinline__ int xsw(int i)
+{
+    if constexpr (SW > 0) return i + (i >> SW);
+    else return i;
+}
 
 template <class T>
 __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a, bool inv)  // a * (-i) forward, a * (+i) inverse
@@ -145,7 +166,7 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int c
             } else {
                 const int base = (jb / Ns) * Ns * Rs + kk;
 #pragma unroll
-                for (int r = 0; r < Rs; r++) lds[c * cs + base + r * Ns] = a[r];
+                for (int r = 0; r < Rs; r++) lds[c * cs + xsw<FftShape<L, E>::SW>(base + r * Ns)] = a[r];
             }
         }
     }
@@ -157,7 +178,7 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int c
 #pragma unroll
         for (int c = 0; c < NC; c++)
 #pragma unroll
-            for (int m = 0; m < EPT; m++) v[c * EPT + m] = lds[c * cs + j + m * TPS];
+            for (int m = 0; m < EPT; m++) v[c * EPT + m] = lds[c * cs + xsw<FftShape<L, E>::SW>(j + m * TPS)];
         exchange_sync<WAVE>();
     }
 }

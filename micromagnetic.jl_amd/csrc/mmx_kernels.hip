@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
 {
     using S = Shape<H>;
     constexpr int EPT = S::EPT, TPS = S::TPS, SX = S::SX;
-    constexpr int HS = H + 1;
+    constexpr int HS = FftShape<H>::XS + 1;
     __shared__ cplx<T> lds[SX * HS];
     const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
     const int64_t seq = (int64_t)blockIdx.x * SX + ls;
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(Shape<L>::BY) k_coly(DemagGeom G, const cplx<T
 {
     using S = Shape<L>;
     constexpr int EPT = S::EPT, TPS = S::TPS, TY = S::TY;
-    constexpr int LS = L + 1;
+    constexpr int LS = FftShape<L>::XS + 1;
     __shared__ cplx<T> lds[TY * LS];
     const int t = threadIdx.x % TY, j = threadIdx.x / TY;
     const int kx = blockIdx.x * TY + t;
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 {
     using S = Shape<L>;
     constexpr int EPT = FftShape<L, MMX_P3_E>::EPT, TPS = FftShape<L, MMX_P3_E>::TPS, TZ = S::TZ;
-    constexpr int LS = L + 1;
+    constexpr int LS = FftShape<L, MMX_P3_E>::XS + 1;
     constexpr int NC = FWD_ONLY ? 1 : 3;
     // one exchange buffer per component when the three transforms share barriers
     constexpr int CS = 2 * TZ * LS;
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     constexpr int EPT = F::EPT, TPS = F::TPS;
     static_assert(TPS <= 64, "one sequence per wave");
     constexpr int TW = 64 / TPS;
-    constexpr int LS = L + MMX_P3_PAD;
+    constexpr int LS = FftShape<L, MMX_P3_E>::XS + MMX_P3_PAD;
     constexpr int NB = MMX_P3_NBW;
     constexpr int WS = TW * LS;
     __shared__ cplx<T> lds[4 * NB * WS];

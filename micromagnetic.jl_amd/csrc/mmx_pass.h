@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
 {
     using S = Shape<H>;
     constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
-    constexpr int HS = H + 1;
+    constexpr int HS = FftShape<H>::XS + 1;
     __shared__ cplx<T> lds[3 * RX * HS];
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int nrows_tot = ny * nz;

@@ -19,7 +19,10 @@ from . import _lib
 from ._lib import BS23, DOPRI54, F32, F64, HEUN, RK4, MmxError, check, dptr
 
 mu_0 = 4 * math.pi * 1e-7       # src/const.jl:1
+mu_B = 9.27400949e-24           # src/const.jl:2
 k_B = 1.3806505e-23             # src/const.jl:3
+c_e = 1.602176565e-19           # src/const.jl:4
+h_bar = 1.05457172647e-34       # src/const.jl:9
 mT = 0.001 / mu_0               # src/const.jl:15
 
 _INTEGRATORS = {"RungeKutta": RK4, "Heun": HEUN, "DormandPrince": DOPRI54, "DOPRI54": DOPRI54,
@@ -215,11 +218,11 @@ class Sim:
         d = self.driver
         check(_lib.load().mmx_set_llg(self._h, float(d.alpha), float(d.gamma), int(bool(d.precession))))
 
-    def _term_field(self, index):
+    def _term_field(self, index, t=0.0):
         N = self.n_total
         f = np.empty(3 * N)
         e = np.empty(N)
-        check(_lib.load().mmx_term_field(self._h, index, 0.0, dptr(f), dptr(e)))
+        check(_lib.load().mmx_term_field(self._h, index, float(t), dptr(f), dptr(e)))
         return f, e
 
     def _add(self, index, name):
@@ -350,6 +353,87 @@ def add_thermal_noise(sim, Temp, name="thermal", T0=0.0, seed=0):
         return sim._add(L.mmx_add_thermal_noise(sim._h, None, float(Temp), float(T0), int(seed)), name)
     a = init_scalar(sim.mesh, Temp)
     return sim._add(L.mmx_add_thermal_noise(sim._h, dptr(a), 0.0, float(T0), int(seed)), name)
+
+
+_TIMEFN = None
+
+
+def _set_time_fn(sim, term, ft):
+    """ft(t) -> float as the term's ufun (called at every evaluation time)."""
+    global _TIMEFN
+    import ctypes as C
+
+    if _TIMEFN is None:
+        _TIMEFN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+    cb = _TIMEFN(lambda t, user: float(ft(t)))
+    term._ft = cb  # the library keeps only the pointer
+    check(_lib.load().mmx_set_term_time_fn(sim._h, term.index, C.cast(cb, C.c_void_p), None))
+
+
+def add_sot(sim, aj, bj, p, name="sot"):
+    """add_sot(sim, aj, bj, p; name): src/micro/add_field.jl:951-966.  The torque
+    enters as the field H = (1/gamma)(aj m x p + bj p) (kernels.jl:619-633)."""
+    L = _lib.load()
+    uniform = np.ndim(aj) == 0 and not callable(aj)
+    a = None if uniform else init_scalar(sim.mesh, aj)
+    return sim._add(L.mmx_add_sot(sim._h, dptr(a), float(aj) if uniform else 0.0, float(bj),
+                                  *map(float, p)), name)
+
+
+def add_stt(sim, model="slonczewski", **params):
+    """add_stt(sim; model, kwargs...): src/micro/add_field.jl:1134-1147 with the
+    Zhang-Li (:1150-1192) and Slonczewski (:1195-1227) models."""
+    model = str(model).lstrip(":")
+    if "J" not in params:
+        raise ValueError("Current density parameter `J` is required.")
+    if model == "zhang_li":
+        return _add_zhang_li(sim, params.pop("name", "zhangli"), params)
+    if model == "slonczewski":
+        return _add_slonczewski(sim, params.pop("name", "slonczewski"), params)
+    raise ValueError(f"Unsupported STT model: {model}. Supported models are: [:zhang_li, :slonczewski]")
+
+
+def _add_zhang_li(sim, name, params):
+    has_b = "b" in params
+    has_P = "P" in params and "Ms" in params
+    if has_b:
+        b = params["b"]
+    elif has_P:
+        b = params["P"] * mu_B / (c_e * params["Ms"])
+    else:
+        raise ValueError("Zhang-Li model requires either `b` or `(P, Ms)` parameters")
+    L = _lib.load()
+    xi = params.get("xi", 0)
+    xi_uniform = np.ndim(xi) == 0 and not callable(xi)
+    xa = None if xi_uniform else init_scalar(sim.mesh, xi)
+    J = params["J"]
+    if isinstance(J, tuple) and len(J) == 3:
+        Ja, J3 = None, tuple(map(float, J))
+    else:
+        Ja, J3 = np.ascontiguousarray(init_vector(sim.mesh, J)), (0.0, 0.0, 0.0)
+    t = sim._add(L.mmx_add_stt_zhangli(sim._h, dptr(Ja), *J3, float(b), dptr(xa),
+                                       float(xi) if xi_uniform else 0.0, int(has_P and not has_b)), name)
+    if "ft" in params:
+        _set_time_fn(sim, t, params["ft"])
+    return t
+
+
+def _add_slonczewski(sim, name, params):
+    for k in ("tf", "Ms", "p", "P"):
+        if k not in params:
+            raise ValueError(f"Parameter {k} is not provided")
+    xi = params.get("xi", 0)
+    Lambda = params.get("Lambda", 2.0)
+    beta = h_bar / (mu_0 * params["Ms"] * c_e * params["tf"])
+    L = _lib.load()
+    J = params["J"]
+    uniform = np.ndim(J) == 0 and not callable(J)
+    Ja = None if uniform else init_scalar(sim.mesh, J)
+    t = sim._add(L.mmx_add_stt_slonczewski(sim._h, dptr(Ja), float(J) if uniform else 0.0, beta, float(Lambda),
+                                           float(xi), float(params["P"]), *map(float, params["p"])), name)
+    if "ft" in params:
+        _set_time_fn(sim, t, params["ft"])
+    return t
 
 
 def effective_field(sim, spin=None, t=0.0):

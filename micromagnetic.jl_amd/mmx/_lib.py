@@ -53,6 +53,10 @@ SIGNATURES = {
     "mmx_update_zeeman": [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double],
     "mmx_add_demag": [_vp, C.c_int, C.c_int, C.c_int],
     "mmx_add_thermal_noise": [_vp, _dp, C.c_double, C.c_double, C.c_uint64],
+    "mmx_add_sot": [_vp, _dp] + [C.c_double] * 5,
+    "mmx_add_stt_slonczewski": [_vp, _dp] + [C.c_double] * 8,
+    "mmx_add_stt_zhangli": [_vp, _dp] + [C.c_double] * 4 + [_dp, C.c_double, C.c_int],
+    "mmx_set_term_time_fn": [_vp, C.c_int, C.c_void_p, C.c_void_p],
     "mmx_num_terms": [_vp],
     "mmx_effective_field": [_vp, C.c_double, _dp],
     "mmx_term_field": [_vp, C.c_int, C.c_double, _dp, _dp],

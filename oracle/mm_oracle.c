@@ -36,7 +36,10 @@ enum {
     T_CUBIC,
     T_DEMAG,
     T_DEMAG_DIRECT,
-    T_THERMAL
+    T_THERMAL,
+    T_SOT,
+    T_STT_SL,
+    T_STT_ZL
 };
 
 typedef struct {
@@ -54,6 +57,9 @@ typedef struct {
     uint64_t seed;
     int64_t eta_step;
     double *eta;
+    /* torque amplitude ufun(t) (NULL: 1) */
+    double (*ufun)(double, void *);
+    void *uuser;
 } term_t;
 
 enum { I_RK4 = 0, I_HEUN = 1, I_DOPRI54 = 2, I_BS23 = 3 };
@@ -78,6 +84,7 @@ struct orc_sim {
     int succeed;
     double *k[7];
     double *y_current, *y_next, *y_temp;
+    double t_eval; /* time of the evaluation in progress (torque ufun) */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1199,6 +1206,141 @@ static void field_thermal(orc_sim *s, term_t *t, const double *m)
 }
 
 /* ------------------------------------------------------------------------ */
+/* spin torques as effective fields (no energy): src/micro/field.jl:317-356 */
+/* ------------------------------------------------------------------------ */
+/* cross_x/y/z: src/util.jl:6-16 */
+static inline double crx(double x1, double x2, double x3, double y1, double y2, double y3) { return -x3 * y2 + x2 * y3; }
+static inline double cry(double x1, double x2, double x3, double y1, double y2, double y3) { return x3 * y1 - x1 * y3; }
+static inline double crz(double x1, double x2, double x3, double y1, double y2, double y3) { return -x2 * y1 + x1 * y2; }
+
+/* add_sot: src/micro/add_field.jl:951-966 (aj per cell, p as given, not normalised) */
+int orc_add_sot(orc_sim *s, const double *aj, double bj, double px, double py, double pz)
+{
+    term_t *t = new_term(s, T_SOT);
+    t->a = dup_array(aj, s->N);
+    t->p[0] = px;
+    t->p[1] = py;
+    t->p[2] = pz;
+    t->p[4] = bj;
+    return s->nterms - 1;
+}
+
+/* df_torque_kernel!: src/micro/kernels.jl:619-633 -- H = (1/gamma)(aj m x p + bj p) */
+static void field_sot(orc_sim *s, term_t *t, const double *m)
+{
+    const double gamma = s->gamma, px = t->p[0], py = t->p[1], pz = t->p[2];
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        double a = t->a[I] / gamma;
+        double b = t->p[4] / gamma;
+        double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        h[j] = a * crx(mx, my, mz, px, py, pz) + b * px;
+        h[j + 1] = a * cry(mx, my, mz, px, py, pz) + b * py;
+        h[j + 2] = a * crz(mx, my, mz, px, py, pz) + b * pz;
+        t->energy[I] = 0;
+    }
+}
+
+/* add_stt(model=:slonczewski): src/micro/add_field.jl:1195-1227; beta as computed
+ * there (h_bar/(mu_0*Ms*c_e*tf)) is an argument */
+int orc_add_stt_slonczewski(orc_sim *s, const double *J, double beta, double Lambda, double xi, double P,
+                            double px, double py, double pz)
+{
+    term_t *t = new_term(s, T_STT_SL);
+    t->a = dup_array(J, s->N);
+    t->p[0] = px;
+    t->p[1] = py;
+    t->p[2] = pz;
+    t->p[4] = Lambda * Lambda; /* lambda_sq = T(torque.Lambda^2), field.jl:349 */
+    t->p[5] = P;
+    t->p[6] = xi;
+    t->p[7] = beta;
+    return s->nterms - 1;
+}
+
+/* slonczewski_torque_kernel!: src/micro/kernels.jl:657-674; ft = ufun(t)*beta (field.jl:350) */
+static void field_stt_sl(orc_sim *s, term_t *t, const double *m)
+{
+    const double px = t->p[0], py = t->p[1], pz = t->p[2];
+    const double lambda_sq = t->p[4], P = t->p[5], xi = t->p[6];
+    const double ft = (t->ufun ? t->ufun(s->t_eval, t->uuser) : 1.0) * t->p[7];
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        double mp = mx * px + my * py + mz * pz;
+        double epsilon = P * lambda_sq / (lambda_sq + 1 + (lambda_sq - 1) * mp);
+        double a = t->a[I] * ft;
+        h[j] = a * (epsilon * crx(mx, my, mz, px, py, pz) + xi * px);
+        h[j + 1] = a * (epsilon * cry(mx, my, mz, px, py, pz) + xi * py);
+        h[j + 2] = a * (epsilon * crz(mx, my, mz, px, py, pz) + xi * pz);
+        t->energy[I] = 0;
+    }
+}
+
+/* add_stt(model=:zhang_li): src/micro/add_field.jl:1150-1192.  bJ = J .* b, and for
+ * the (P, Ms) form (xi_factor = 1) each cell's vector times 1/(1 + xi^2) */
+int orc_add_stt_zhangli(orc_sim *s, const double *J /* 3N */, double b, const double *xi, int xi_factor)
+{
+    term_t *t = new_term(s, T_STT_ZL);
+    t->a = dup_array(xi, s->N);
+    t->b = malloc(3 * (size_t)s->N * sizeof(double));
+    for (int I = 0; I < s->N; I++) {
+        double f = 1 / (1 + xi[I] * xi[I]);
+        for (int q = 0; q < 3; q++) {
+            double v = J[3 * (size_t)I + q] * b;
+            t->b[3 * (size_t)I + q] = xi_factor ? v * f : v;
+        }
+    }
+    return s->nterms - 1;
+}
+
+/* zhangli_torque_kernel!: src/micro/kernels.jl:680-739; ut = ufun(t)/gamma (field.jl:334).
+ * Mesh neighbours (ngbs) only -- no Ms test; a missing side falls back to the cell
+ * itself with a one-sided 1/d factor. */
+static void field_stt_zl(orc_sim *s, term_t *t, const double *m)
+{
+    const double ut = (t->ufun ? t->ufun(s->t_eval, t->uuser) : 1.0) / s->gamma;
+    const double d[3] = {s->dx, s->dy, s->dz};
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        double fx = 0, fy = 0, fz = 0;
+        for (int q = 0; q < 3; q++) {
+            int64_t i1 = s->ngbs[6 * (size_t)I + 2 * q], i2 = s->ngbs[6 * (size_t)I + 2 * q + 1];
+            double factor = (i1 > 0 && i2 > 0) ? 1 / (2 * d[q]) : 1 / d[q];
+            if (i1 < 0) i1 = I + 1;
+            if (i2 < 0) i2 = I + 1;
+            size_t j1 = 3 * (size_t)(i1 - 1), j2 = 3 * (size_t)(i2 - 1);
+            double u = t->b[j + q] * factor;
+            fx += u * (m[j2] - m[j1]);
+            fy += u * (m[j2 + 1] - m[j1 + 1]);
+            fz += u * (m[j2 + 2] - m[j1 + 2]);
+        }
+        fx = ut * fx;
+        fy = ut * fy;
+        fz = ut * fz;
+        double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        h[j] = crx(mx, my, mz, fx, fy, fz) + t->a[I] * fx;
+        h[j + 1] = cry(mx, my, mz, fx, fy, fz) + t->a[I] * fy;
+        h[j + 2] = crz(mx, my, mz, fx, fy, fz) + t->a[I] * fz;
+        t->energy[I] = 0;
+    }
+}
+
+int orc_set_term_time_fn(orc_sim *s, int term, double (*fn)(double, void *), void *user)
+{
+    if (term < 0 || term >= s->nterms) return -1;
+    s->terms[term].ufun = fn;
+    s->terms[term].uuser = user;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* dispatch and aggregation: src/micro/field.jl:496-505, :520-536           */
 /* ------------------------------------------------------------------------ */
 static void term_eval(orc_sim *s, term_t *t, const double *m)
@@ -1216,12 +1358,15 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_DEMAG: field_demag(s, t, m); break;
     case T_DEMAG_DIRECT: field_demag_direct(s, t, m); break;
     case T_THERMAL: field_thermal(s, t, m); break;
+    case T_SOT: field_sot(s, t, m); break;
+    case T_STT_SL: field_stt_sl(s, t, m); break;
+    case T_STT_ZL: field_stt_zl(s, t, m); break;
     }
 }
 
 void orc_effective_field(orc_sim *s, const double *spin, double t, double *field_out)
 {
-    (void)t;
+    s->t_eval = t;
     size_t n3 = 3 * (size_t)s->N;
     memset(s->field, 0, n3 * sizeof(double));
     for (int i = 0; i < s->nterms; i++) {
@@ -1238,7 +1383,7 @@ void orc_effective_field(orc_sim *s, const double *spin, double t, double *field
 void orc_term_field(orc_sim *s, int term, const double *spin, double t, double *field_out,
                     double *energy_out)
 {
-    (void)t;
+    s->t_eval = t;
     term_t *tm = &s->terms[term];
     term_eval(s, tm, spin);
     if (field_out) memcpy(field_out, tm->field, 3 * (size_t)s->N * sizeof(double));
@@ -1260,7 +1405,7 @@ static double psum(const double *a, size_t n)
 /* compute_system_energy: field.jl:520-536; returns the sum of each term's energy */
 void orc_term_energies(orc_sim *s, const double *spin, double t, double *E_out)
 {
-    (void)t;
+    s->t_eval = t;
     for (int i = 0; i < s->nterms; i++) {
         term_eval(s, &s->terms[i], spin);
         E_out[i] = psum(s->terms[i].energy, s->N);

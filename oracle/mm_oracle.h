@@ -48,6 +48,11 @@ int orc_add_zeeman(orc_sim *s, const double *H /* 3N */);
 int orc_add_demag(orc_sim *s, int Nx, int Ny, int Nz);
 int orc_add_demag_direct(orc_sim *s, int Nx, int Ny, int Nz);
 int orc_add_thermal(orc_sim *s, const double *Temp, double T0, uint64_t seed);
+int orc_add_sot(orc_sim *s, const double *aj, double bj, double px, double py, double pz);
+int orc_add_stt_slonczewski(orc_sim *s, const double *J, double beta, double Lambda, double xi, double P,
+                            double px, double py, double pz);
+int orc_add_stt_zhangli(orc_sim *s, const double *J /* 3N */, double b, const double *xi, int xi_factor);
+int orc_set_term_time_fn(orc_sim *s, int term, double (*fn)(double, void *), void *user);
 int orc_num_terms(const orc_sim *s);
 
 /* the compact real-space tensor of a demag term: 6*N doubles, order xx,yy,zz,xy,xz,yz */

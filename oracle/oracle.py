@@ -69,6 +69,10 @@ def lib():
             "orc_add_demag": [C.c_void_p, C.c_int, C.c_int, C.c_int],
             "orc_add_demag_direct": [C.c_void_p, C.c_int, C.c_int, C.c_int],
             "orc_add_thermal": [C.c_void_p, _dp, C.c_double, C.c_uint64],
+            "orc_add_sot": [C.c_void_p, _dp] + [C.c_double] * 4,
+            "orc_add_stt_slonczewski": [C.c_void_p, _dp] + [C.c_double] * 7,
+            "orc_add_stt_zhangli": [C.c_void_p, _dp, C.c_double, _dp, C.c_int],
+            "orc_set_term_time_fn": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
             "orc_advance_step": [C.c_void_p],
@@ -79,6 +83,9 @@ def lib():
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
+
+
+_TIMEFN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
 
 
 def _p(a):
@@ -204,6 +211,32 @@ class OracleSim:
 
     def add_thermal(self, T, T0=0.0, seed=0):
         return lib().orc_add_thermal(self.h, _p(_f64(T, self.N)), float(T0), int(seed))
+
+    def add_sot(self, aj, bj, p):
+        """add_sot(sim, aj, bj, p): src/micro/add_field.jl:951-966."""
+        return lib().orc_add_sot(self.h, _p(_f64(aj, self.N)), float(bj), *map(float, p))
+
+    def add_stt_slonczewski(self, J, beta, Lambda, xi, P, p):
+        """add_stt(model=:slonczewski): src/micro/add_field.jl:1195-1227 (beta precomputed)."""
+        return lib().orc_add_stt_slonczewski(self.h, _p(_f64(J, self.N)), float(beta), float(Lambda),
+                                             float(xi), float(P), *map(float, p))
+
+    def add_stt_zhangli(self, J, b, xi, xi_factor):
+        """add_stt(model=:zhang_li): src/micro/add_field.jl:1150-1192; J 3-vector or 3N AoS."""
+        J = np.asarray(J, dtype=np.float64)
+        if J.size == 3:
+            J = np.tile(J, self.N)
+        return lib().orc_add_stt_zhangli(self.h, _p(_f64(J.reshape(-1), 3 * self.N)), float(b),
+                                         _p(_f64(xi, self.N)), int(bool(xi_factor)))
+
+    def set_term_time_fn(self, term, fn):
+        """ufun(t) of a torque term; fn(t) -> float, or None for 1."""
+        if fn is None:
+            lib().orc_set_term_time_fn(self.h, term, None, None)
+            return
+        cb = _TIMEFN(lambda t, user: float(fn(t)))
+        self._keep = getattr(self, "_keep", []) + [cb]
+        lib().orc_set_term_time_fn(self.h, term, C.cast(cb, C.c_void_p), None)
 
     def demag_tensor(self, term):
         out = np.empty(6 * self.N)
