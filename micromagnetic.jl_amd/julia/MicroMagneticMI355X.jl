@@ -123,7 +123,38 @@ end
 function add_term!(h, t::MicroMagnetic.Demag)
     error("attach! needs the macro-PBC image counts of the demag term: call attach_demag!(sim, Nx, Ny, Nz)")
 end
+function add_term!(h, t::MicroMagnetic.DFTorqueField)
+    ccall((:mmx_add_sot, libmmx), Cint,
+          (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble), h, host(t.aj), 0, t.bj,
+          t.px, t.py, t.pz) |> check
+end
+function add_term!(h, t::MicroMagnetic.SlonczewskiTorque)
+    idx = ccall((:mmx_add_stt_slonczewski, libmmx), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble),
+                h, host(t.J), 0, t.beta, t.Lambda, t.xi, t.P, t.px, t.py, t.pz) |> check
+    set_time_fn!(h, idx, t.ufun)
+    return idx
+end
+function add_term!(h, t::MicroMagnetic.ZhangLiTorque)
+    # the struct already holds bJ = b*J (with the 1/(1+xi^2) factor): pass it as J with b = 1
+    idx = ccall((:mmx_add_stt_zhangli, libmmx), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Float64}, Cdouble, Cint),
+                h, host(t.bJ), 0, 0, 0, 1.0, host(t.xi), 0, 0) |> check
+    set_time_fn!(h, idx, t.ufun)
+    return idx
+end
 add_term!(h, t) = error("term $(typeof(t)) is not on the MI355X path")
+
+# ufun(t) of a torque term, called by libmmx at every evaluation time; the
+# C-callable closures are kept alive here for as long as the context lives
+const timefns = IdDict{Any,Any}()
+function set_time_fn!(h, idx, ufun)
+    f = (t::Cdouble, user::Ptr{Cvoid}) -> Cdouble(ufun(t))
+    cf = @cfunction($f, Cdouble, (Cdouble, Ptr{Cvoid}))
+    timefns[(h, idx)] = cf
+    ccall((:mmx_set_term_time_fn, libmmx), Cint, (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}), h, idx, cf,
+          C_NULL) |> check
+end
 
 """
     set_integrator!(sim)
