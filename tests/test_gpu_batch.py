@@ -118,3 +118,36 @@ def test_batch_rejects_bad_members():
         mmx.Batch([s1, s3])
     with pytest.raises(mmx.MmxError):
         mmx.Batch([s1, s1])
+
+
+def test_batch_with_constant_spin_torques():
+    """Members carrying SOT / Slonczewski / Zhang-Li terms (constant currents)
+    take the term-list kernels in the batch and still land exactly where each
+    member advanced alone lands; a time profile is refused."""
+
+    def members(seed):
+        rng = np.random.default_rng(seed)
+        out = []
+        for q in range(4):
+            s, info = _member(rng, "RungeKutta")
+            N = s.n_total
+            if q % 3 == 0:
+                mmx.add_sot(s, rng.uniform(1e9, 2e9, N), 3e8, (0, 1, 0))
+            elif q % 3 == 1:
+                mmx.add_stt(s, model="slonczewski", J=1e12, tf=2e-9, Ms=8e5, p=(0, 0, 1), P=0.6, xi=0.1)
+            else:
+                mmx.add_stt(s, model="zhang_li", b=5.0, J=(1.0, 0.5, 0), xi=0.05)
+            out.append(s)
+        return out
+
+    bm, alone = members(11), members(11)
+    b = mmx.Batch(bm)
+    b.advance(5)
+    for s in alone:
+        mmx.run_step(s, n=5)
+    for sb, sa in zip(bm, alone):
+        assert np.array_equal(sb.spin, sa.spin)
+    s = members(12)[2]
+    mmx._set_time_fn(s, s.interactions[-1], lambda t: 2.0)
+    with pytest.raises(mmx.MmxError):
+        mmx.Batch([s])
