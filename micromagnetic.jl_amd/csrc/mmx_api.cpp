@@ -542,7 +542,7 @@ struct Engine : CtxBase {
         G.lgLmax = ilog2(G.Lmax);
         G.Py2 = G.Py / 2 + 1;
         G.Pz2 = G.Pz / 2 + 1;
-        G.ktw = convz_grouped_tensor(G);
+        G.ktw = convz_grouped_tensor(G, (int)sizeof(T));
         if (G.H > 2048 || G.Py > 2048 || G.Pz > 2048) {
             fail(MMX_EINVAL, "mesh %dx%dx%d exceeds the demag transform limits of this build", nx, ny, nz);
             throw HipError{MMX_EINVAL};

@@ -28,7 +28,7 @@ template <class T>
 void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, int Rin, int n_out,
                  int Rout, int nplanes, int ncomp, int inverse, const cplx<T> *tw, hipStream_t s);
 // 1 when the z pass takes the wave form, whose folded tensor layout groups 4 kx
-int convz_grouped_tensor(const DemagGeom &G);
+int convz_grouped_tensor(const DemagGeom &G, int elsize);
 template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in,
                   int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s);

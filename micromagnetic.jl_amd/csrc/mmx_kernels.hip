@@ -83,13 +83,27 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
 // Input column (plane p, comp c): in[((c*nplanes + p)*Rin + y)*KS + kx], y < n_in.
 // Output: out[((c*nplanes + p)*Rout + y)*KS + kx] for y < n_out.
 // ---------------------------------------------------------------------------
+// FP32 columns are half as wide in bytes: MMX_COLY_F32X times more of them
+// per block (up to 1024 threads) keep whole cache lines per row
+#ifndef MMX_COLY_F32X
+#define MMX_COLY_F32X 2
+#endif
+template <class T, int L>
+struct ColyShape {
+    static constexpr int TPS = Shape<L>::TPS;
+    static constexpr int TY0 = sizeof(T) == 4 ? Shape<L>::TY * MMX_COLY_F32X : Shape<L>::TY;
+    static constexpr int TY = TY0 * TPS > 1024 ? (1024 / TPS < 1 ? 1 : 1024 / TPS) : TY0;
+    static constexpr int BY = TY * TPS;
+};
+
 template <class T, int L, bool INV>
-__global__ void __launch_bounds__(Shape<L>::BY) k_coly(DemagGeom G, const cplx<T> *__restrict__ in,
-                                                        cplx<T> *__restrict__ out, int n_in, int Rin, int n_out,
-                                                        int Rout, int nplanes, const cplx<T> *__restrict__ tw)
+__global__ void __launch_bounds__((ColyShape<T, L>::BY)) k_coly(DemagGeom G, const cplx<T> *__restrict__ in,
+                                                               cplx<T> *__restrict__ out, int n_in, int Rin,
+                                                               int n_out, int Rout, int nplanes,
+                                                               const cplx<T> *__restrict__ tw)
 {
     using S = Shape<L>;
-    constexpr int EPT = S::EPT, TPS = S::TPS, TY = S::TY;
+    constexpr int EPT = S::EPT, TPS = S::TPS, TY = ColyShape<T, L>::TY;
     constexpr int LS = FftShape<L>::XS + 1;
     __shared__ cplx<T> lds[TY * LS];
     const int t = threadIdx.x % TY, j = threadIdx.x / TY;
@@ -247,6 +261,14 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 #ifndef MMX_P3_WAVE
 #define MMX_P3_WAVE 1
 #endif
+// elements per thread of the wave-form z transform for FP32 (FP64: MMX_P3_E)
+#ifndef MMX_P3_E32
+#define MMX_P3_E32 8
+#endif
+template <class T>
+struct P3W {
+    static constexpr int E = sizeof(T) == 8 ? MMX_P3_E : MMX_P3_E32;
+};
 #ifndef MMX_P3_NBW
 #define MMX_P3_NBW 3
 #endif
@@ -263,11 +285,11 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
                                                const T *__restrict__ Lline, int n_in, int nzb,
                                                const cplx<T> *__restrict__ tw)
 {
-    using F = FftShape<L, MMX_P3_E>;
+    using F = FftShape<L, P3W<T>::E>;
     constexpr int EPT = F::EPT, TPS = F::TPS;
     static_assert(TPS <= 64, "one sequence per wave");
     constexpr int TW = 64 / TPS;
-    constexpr int LS = FftShape<L, MMX_P3_E>::XS + MMX_P3_PAD;
+    constexpr int LS = F::XS + MMX_P3_PAD;
     constexpr int NB = MMX_P3_NBW;
     constexpr int WS = TW * LS;
     __shared__ cplx<T> lds[4 * NB * WS];
@@ -319,10 +341,10 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     if constexpr (MMX_P3W_PF) load_tensor();
     cplx<T> *buf = lds + wave * NB * WS + t * LS;
     if constexpr (NB == 3) {
-        fft_wave_multi<T, L, false, MMX_P3_E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
+        fft_wave_multi<T, L, false, P3W<T>::E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
     } else {
 #pragma unroll
-        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, false, MMX_P3_E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
+        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, false, P3W<T>::E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
     }
     if constexpr (!MMX_P3W_PF) load_tensor();
 #pragma unroll
@@ -334,10 +356,10 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
         v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
     }
     if constexpr (NB == 3) {
-        fft_wave_multi<T, L, true, MMX_P3_E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
+        fft_wave_multi<T, L, true, P3W<T>::E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
     } else {
 #pragma unroll
-        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, true, MMX_P3_E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
+        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, true, P3W<T>::E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
     }
     if (!ok) return;
     const int nz = G.nz;
@@ -1036,21 +1058,23 @@ void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, 
 {
 #define CY_CALL(L)                                                                                   \
     {                                                                                                \
-        dim3 grid((G.Kx + Shape<L>::TY - 1) / Shape<L>::TY, nplanes, ncomp);                         \
+        constexpr int TY = ColyShape<T, L>::TY, BY = ColyShape<T, L>::BY;                           \
+        dim3 grid((G.Kx + TY - 1) / TY, nplanes, ncomp);                                             \
         if (inverse)                                                                                 \
-            hipLaunchKernelGGL((k_coly<T, L, true>), grid, dim3(Shape<L>::BY), 0, s, G, in, out, n_in, Rin, n_out, \
+            hipLaunchKernelGGL((k_coly<T, L, true>), grid, dim3(BY), 0, s, G, in, out, n_in, Rin, n_out,     \
                                Rout, nplanes, tw);                                                   \
         else                                                                                         \
-            hipLaunchKernelGGL((k_coly<T, L, false>), grid, dim3(Shape<L>::BY), 0, s, G, in, out, n_in, Rin, n_out, \
+            hipLaunchKernelGGL((k_coly<T, L, false>), grid, dim3(BY), 0, s, G, in, out, n_in, Rin, n_out,    \
                                Rout, nplanes, tw);                                                   \
     }
     MMX_LG_SWITCH(G.lgPy, CY_CALL)
 #undef CY_CALL
 }
 
-int convz_grouped_tensor(const DemagGeom &G)
+int convz_grouped_tensor(const DemagGeom &G, int elsize)
 {
-    const int ept = G.Pz < MMX_P3_E ? G.Pz : MMX_P3_E;
+    const int e = elsize == 8 ? P3W<double>::E : P3W<float>::E;
+    const int ept = G.Pz < e ? G.Pz : e;
     return (MMX_P3_WAVE && G.Pz / ept <= 64) ? 1 : 0;
 }
 
@@ -1062,9 +1086,9 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
 #define CZ_CALL(L)                                                                                   \
     {                                                                                                \
         dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py / 2 + 1);                           \
-        if constexpr (FftShape<L, MMX_P3_E>::TPS <= 64) {                                            \
+        if constexpr (FftShape<L, P3W<T>::E>::TPS <= 64) {                                           \
             if (!fwd_only && G.ktw) {                                                                \
-                constexpr int TW = 64 / FftShape<L, MMX_P3_E>::TPS;                                  \
+                constexpr int TW = 64 / FftShape<L, P3W<T>::E>::TPS;                                 \
                 dim3 gw((G.Kx + 2 * TW - 1) / (2 * TW), G.Py / 2 + 1);                               \
                 hipLaunchKernelGGL((k_convw<T, L>), gw, dim3(256), 0, s, G, B, Kt, Lline, n_in, nzb, tw); \
                 break;                                                                               \
