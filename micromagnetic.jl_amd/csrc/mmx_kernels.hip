@@ -88,10 +88,13 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
 #ifndef MMX_COLY_F32X
 #define MMX_COLY_F32X 2
 #endif
+#ifndef MMX_COLY_F64X
+#define MMX_COLY_F64X 1
+#endif
 template <class T, int L>
 struct ColyShape {
     static constexpr int TPS = Shape<L>::TPS;
-    static constexpr int TY0 = sizeof(T) == 4 ? Shape<L>::TY * MMX_COLY_F32X : Shape<L>::TY;
+    static constexpr int TY0 = Shape<L>::TY * (sizeof(T) == 4 ? MMX_COLY_F32X : MMX_COLY_F64X);
     static constexpr int TY = TY0 * TPS > 1024 ? (1024 / TPS < 1 ? 1 : 1024 / TPS) : TY0;
     static constexpr int BY = TY * TPS;
 };
