@@ -199,10 +199,10 @@ __device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int 
 // Transform of one sequence.  Threads of the same block must all call this
 // together (the exchanges use __syncthreads); `lds` is this sequence's buffer
 // of L elements.  `active` threads hold data; inactive ones only join barriers.
-template <class T, int L, bool INV, int E = 8>
+template <class T, int L, bool INV, int E = 8, bool WAVE = false>
 __device__ __forceinline__ void fft_reg(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int lgLmax)
 {
-    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, 1>(v, j, lds, 0, tw, lgLmax - FftShape<L, E>::LG);
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, 1, WAVE>(v, j, lds, 0, tw, lgLmax - FftShape<L, E>::LG);
 }
 
 // NC sequences per thread (v[c*EPT + m]), exchanging through lds + c*cs.

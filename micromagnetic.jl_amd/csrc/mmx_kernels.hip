@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
         v[m] = {a, b};
     }
     cplx<T> *buf = lds + ls * HS;
-    fft_reg<T, H, false>(v, j, buf, tw, G.lgLmax);
-    r2c_split_store<T, H>(v, j, buf, active, A + ((int64_t)c * nrows + row) * G.KS, tw, G.lgLmax);
+    fft_reg<T, H, false, 8, RowWave<H>::on>(v, j, buf, tw, G.lgLmax);
+    r2c_split_store<T, H, RowWave<H>::on>(v, j, buf, active, A + ((int64_t)c * nrows + row) * G.KS, tw,
+                                          G.lgLmax);
 }
 
 // ---------------------------------------------------------------------------

@@ -62,14 +62,14 @@ __device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, cons
 // one row packed as z[n] = (x[2n], x[2n+1]); buf is that sequence's exchange
 // buffer (all of its threads arrive here together).  Writes the Px/2+1
 // spectrum values X[k] = Fe + W_Px^k Fo (X[H] = Fe(0) - Fo(0)) to dst.
-template <class T, int H>
+template <class T, int H, bool WAVE = false>
 __device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T> *buf, bool active,
                                                 cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int lgLmax)
 {
     constexpr int EPT = FftShape<H>::EPT, TPS = FftShape<H>::TPS;
 #pragma unroll
     for (int m = 0; m < EPT; m++) buf[j + m * TPS] = v[m];
-    __syncthreads();
+    exchange_sync<WAVE>();
     if (!active) return;
     // the twiddle pointer passes through an empty asm statement after the
     // barrier so that these loads are not hoisted over the transform (where
@@ -89,6 +89,17 @@ __device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T>
         if (k == 0) dst[H] = csub(Fe, Fo);  // W_Px^H = -1
     }
 }
+
+// Row transforms (first and last pass): thread j of a sequence is lane
+// (tid % TPS), so a sequence of TPS <= 64 threads lies inside one wave and
+// its exchanges need only a wave-level fence, not a block barrier.
+#ifndef MMX_ROWWAVE
+#define MMX_ROWWAVE 1
+#endif
+template <int H>
+struct RowWave {
+    static constexpr bool on = MMX_ROWWAVE && FftShape<H>::TPS <= 64;
+};
 
 // at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
 // latency-bound, and the register budget this sets costs no spills
@@ -145,7 +156,7 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         v[q] = Z;
     }
     cplx<T> *buf = lds + (c * RX + r) * HS;
-    fft_reg<T, H, true>(v, j, buf, tw, G.lgLmax);
+    fft_reg<T, H, true, 8, RowWave<H>::on>(v, j, buf, tw, G.lgLmax);
 #pragma unroll
     for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
     __syncthreads();
@@ -210,10 +221,10 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
             }
             v[q] = Z;
         }
-        __syncthreads();
-        fft_reg<T, H, false>(v, jt, bt, tw, G.lgLmax);
+        exchange_sync<RowWave<H>::on>();
+        fft_reg<T, H, false, 8, RowWave<H>::on>(v, jt, bt, tw, G.lgLmax);
         cplx<T> *dst = E.anext + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
-        r2c_split_store<T, H>(v, jt, bt, active, dst, tw, G.lgLmax);
+        r2c_split_store<T, H, RowWave<H>::on>(v, jt, bt, active, dst, tw, G.lgLmax);
     }
 }
 
