@@ -96,6 +96,9 @@ __device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T>
 #ifndef MMX_ROWWAVE
 #define MMX_ROWWAVE 1
 #endif
+#ifndef MMX_P5_MIRROR_LDS
+#define MMX_P5_MIRROR_LDS 1
+#endif
 template <int H>
 struct RowWave {
     static constexpr bool on = MMX_ROWWAVE && FftShape<H>::TPS <= 64;
@@ -139,13 +142,25 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
     const cplx<T> *row = A + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
     const int sh = G.lgLmax - (FftShape<H>::LG + 1);
     cplx<T> v[EPT];
+    cplx<T> *buf = lds + (c * RX + r) * HS;
+    // with wave-local rows the mirror values X[H-k] come from LDS: each
+    // spectrum value is read from HBM once
+    constexpr bool MIR = RowWave<H>::on && MMX_P5_MIRROR_LDS;
+    if constexpr (MIR) {
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < EPT; q++) buf[j + q * TPS] = row[j + q * TPS];
+            if (j == 0) buf[H] = row[H];
+        }
+        exchange_sync<true>();
+    }
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
         const int k = j + q * TPS;
         cplx<T> Z = {0, 0};
         if (active) {
-            cplx<T> Xk = row[k];
-            cplx<T> Xc = row[H - k];
+            cplx<T> Xk = MIR ? buf[k] : row[k];
+            cplx<T> Xc = MIR ? buf[H - k] : row[H - k];
             Xc.y = -Xc.y;
             cplx<T> Fe = cadd(Xk, Xc);
             cplx<T> w = tw[k << sh];
@@ -155,7 +170,7 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         }
         v[q] = Z;
     }
-    cplx<T> *buf = lds + (c * RX + r) * HS;
+    if constexpr (MIR) exchange_sync<true>();
     fft_reg<T, H, true, 8, RowWave<H>::on>(v, j, buf, tw, G.lgLmax);
 #pragma unroll
     for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
