@@ -367,6 +367,25 @@ def test_thermal_field_parity(oracle):
     assert np.max(np.abs(g.spin - o.spin)) < 1e-11
 
 
+@pytest.mark.parametrize("integrator", ["Heun", "RungeKutta"])
+def test_thermal_demag_chained_trajectory(oracle, integrator):
+    """sLLG with demag: noise regenerated per step while the stages hand their
+    x-spectra over (the chained last pass)."""
+    import mmx
+
+    note(f"thermal + demag {integrator}")
+    g, o = build(oracle, 10, 8, 3, pbc="x", terms=[("exch", dict(A=1e-11)), ("demag", {}),
+                                                   ("thermal", dict(T=300.0, seed=91))],
+                 llg=(0.1, 2.21e5, True), seed=6)
+    kind = {"Heun": 1, "RungeKutta": 0}[integrator]
+    mmx.set_driver(g, integrator=integrator, step=1e-14)
+    o.set_integrator(kind, 1e-14, 0)
+    mmx.run_step(g, n=6)
+    for _ in range(6):
+        o.advance_step()
+    assert np.max(np.abs(g.spin - o.spin)) < 1e-11
+
+
 def test_fp32_error_vs_fp64_oracle(oracle):
     """Precision form: FP32 stencils/LLG/FFT; error against the FP64 oracle is stated."""
     import mmx
