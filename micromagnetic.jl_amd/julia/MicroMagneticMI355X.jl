@@ -11,8 +11,8 @@
 # From then on the reference's own entry points -- effective_field(sim, spin, t),
 # llg_call_back, advance_step(sim, integrator), run_until, run_sim, relax -- run
 # on the GPU; sim.spin is kept in sync at the boundary.  The C ABI it calls is
-# declared in include/mmx.h; the mapping of each reference method to an entry
-# point is listed in INTEGRATION.md.
+# declared in include/mmx.h, where each entry point names the reference method
+# it replaces.
 module MicroMagneticMI355X
 
 using MicroMagnetic
