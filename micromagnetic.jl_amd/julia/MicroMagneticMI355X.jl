@@ -131,7 +131,8 @@ end
 function add_term!(h, t::MicroMagnetic.SlonczewskiTorque)
     idx = ccall((:mmx_add_stt_slonczewski, libmmx), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble),
-                h, host(t.J), 0, t.beta, t.Lambda, t.xi, t.P, t.px, t.py, t.pz) |> check
+                h, host(t.J), 0, t.beta, t.Lambda, t.xi, t.P, t.px, t.py, t.pz)
+    check(idx)  # the return code is the term index
     set_time_fn!(h, idx, t.ufun)
     return idx
 end
@@ -139,7 +140,8 @@ function add_term!(h, t::MicroMagnetic.ZhangLiTorque)
     # the struct already holds bJ = b*J (with the 1/(1+xi^2) factor): pass it as J with b = 1
     idx = ccall((:mmx_add_stt_zhangli, libmmx), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Float64}, Cdouble, Cint),
-                h, host(t.bJ), 0, 0, 0, 1.0, host(t.xi), 0, 0) |> check
+                h, host(t.bJ), 0, 0, 0, 1.0, host(t.xi), 0, 0)
+    check(idx)
     set_time_fn!(h, idx, t.ufun)
     return idx
 end
