@@ -269,9 +269,22 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 #ifndef MMX_P3_E32
 #define MMX_P3_E32 8
 #endif
+// Waves of a block along kx: 2 (the other two waves take the mirror row
+// Py-ky, which reads the same folded tensor entries) or 4 (the block covers 4
+// wave widths of one row and the grid every ky).  Long z transforms leave a
+// wave few kx columns (TW = 64/TPS); from TW <= MMX_P3W_WX4 on the wider
+// tile measured faster (Pz = 128: 1634 -> 1462 us), below it slower.
+#ifndef MMX_P3W_WX4
+#define MMX_P3W_WX4 2
+#endif
 template <class T>
 struct P3W {
     static constexpr int E = sizeof(T) == 8 ? MMX_P3_E : MMX_P3_E32;
+    template <int L>
+    static constexpr int wx()
+    {
+        return 64 / FftShape<L, E>::TPS <= MMX_P3W_WX4 ? 4 : 2;
+    }
 };
 #ifndef MMX_P3_NBW
 #define MMX_P3_NBW 3
@@ -299,13 +312,14 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     __shared__ cplx<T> lds[4 * NB * WS];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int t = lane % TW, j = lane / TW;
-    const int half = wave >> 1;
+    constexpr int WX = P3W<T>::template wx<L>();
+    const int half = WX == 2 ? wave >> 1 : 0;
     const int Py = G.Py, KS = G.KS;
     const int ky0 = blockIdx.y;
     // the mirror row exists only for 0 < ky0 < Py/2 (a whole wave's decision)
     if (half && !(ky0 >= 1 && ky0 < Py / 2)) return;
     const int ky = half ? Py - ky0 : ky0;
-    const int kx = blockIdx.x * (2 * TW) + (wave & 1) * TW + t;
+    const int kx = blockIdx.x * (WX * TW) + (wave % WX) * TW + t;
     const bool ok = kx < G.Kx;
     const int kxs = ok ? kx : 0;
     const int64_t cs_b = (int64_t)nzb * Py * KS;
@@ -1092,8 +1106,8 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
         dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py / 2 + 1);                           \
         if constexpr (FftShape<L, P3W<T>::E>::TPS <= 64) {                                           \
             if (!fwd_only && G.ktw) {                                                                \
-                constexpr int TW = 64 / FftShape<L, P3W<T>::E>::TPS;                                 \
-                dim3 gw((G.Kx + 2 * TW - 1) / (2 * TW), G.Py / 2 + 1);                               \
+                constexpr int TW = 64 / FftShape<L, P3W<T>::E>::TPS, WX = P3W<T>::template wx<L>();  \
+                dim3 gw((G.Kx + WX * TW - 1) / (WX * TW), WX == 2 ? G.Py / 2 + 1 : G.Py);           \
                 hipLaunchKernelGGL((k_convw<T, L>), gw, dim3(256), 0, s, G, B, Kt, Lline, n_in, nzb, tw); \
                 break;                                                                               \
             }                                                                                        \
