@@ -284,6 +284,24 @@ def test_chained_stage_spectra(oracle, integrator, precision):
     assert err < (1e-12 if precision == 64 else 1e-5), err
 
 
+@pytest.mark.parametrize("mesh", [(1, 1, 1), (3, 1, 1), (1, 3, 2), (2, 2, 1), (5, 3, 3)])
+def test_tiny_mesh_chained_trajectory(oracle, mesh):
+    """Chained stages on degenerate transform sizes (Px = 2, Py or Pz = 1)."""
+    import mmx
+
+    nx, ny, nz = mesh
+    note(f"tiny mesh chained {mesh}")
+    N = nx * ny * nz
+    terms = [("exch", dict(A=1.3e-11)), ("anis", dict(Ku=2e5)), ("demag", {})]
+    g, o = build(oracle, nx, ny, nz, Ms=np.full(N, 7e5), terms=terms, seed=12)
+    mmx.set_driver(g, integrator="RungeKutta", step=5e-14)
+    o.set_integrator(0, 5e-14, 0)
+    mmx.run_step(g, n=8)
+    for _ in range(8):
+        o.advance_step()
+    assert np.max(np.abs(g.spin - o.spin)) < 1e-12
+
+
 @pytest.mark.parametrize("integrator", ["DormandPrince", "BS23"])
 def test_adaptive_trajectory(oracle, integrator):
     import mmx
