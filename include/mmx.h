@@ -124,6 +124,11 @@ int mmx_compute_energies(mmx_ctx *ctx, double t, double *E);
 
 /* LLG driver fields alpha, gamma, precession: src/driver.jl:15-21 */
 int mmx_set_llg(mmx_ctx *ctx, double alpha, double gamma, int precession);
+/* Sim(mesh; driver="SpatialLLG") + set_alpha(sim, alpha): src/driver.jl:32-38,149-152,
+ * src/sim.jl:213-222, spatial_llg_rhs_kernel! src/llg/llg.jl:62-97 -- alpha per cell
+ * (N values); NULL returns to the uniform alpha of mmx_set_llg.  The stochastic
+ * field and the adjoint need the uniform alpha. */
+int mmx_set_alpha(mmx_ctx *ctx, const double *alpha);
 /* llg_call_back(sim, dm_dt, spin, t): src/llg/llg.jl:45-58; m == NULL uses the current state */
 int mmx_llg_rhs(mmx_ctx *ctx, double t, const double *m, double *dmdt);
 

@@ -144,6 +144,7 @@ struct Engine : CtxBase {
     mmx_ctx *C;
     int64_t N;
     DevBuf<T> mu0Ms;
+    DevBuf<T> alphas;  // SpatialLLG damping per cell (empty: uniform alpha)
     DevBuf<uint8_t> pins;
     bool have_pins = false;
     DevBuf<uint8_t> nbm;  // neighbour / pin mask, refreshed when Ms or the pins change
@@ -292,6 +293,7 @@ struct Engine : CtxBase {
         }
         P.alpha = (T)C->alpha;
         P.gamma = (T)C->gamma;
+        P.alphas = alphas.p;
         P.precession = C->precession;
         P.seed = 0;
         for (const auto &h : C->terms)
@@ -321,6 +323,7 @@ struct Engine : CtxBase {
             P.slot[sk] = i;
             P.ord[i] = sk;
         }
+        if (alphas.p) P.fast = 0;  // per-cell damping: term-list kernels
         // packed switches of the slot form
         {
             uint32_t f = (P.px ? FL_PX : 0u) | (P.py ? FL_PY : 0u) | (P.pz ? FL_PZ : 0u);
@@ -1017,6 +1020,7 @@ struct Engine : CtxBase {
     int adj_record(int64_t nsteps, int64_t every)
     {
         if (C->ikind != MMX_RK4) return fail(MMX_ESTATE, "the adjoint needs the RungeKutta (RK4) integrator");
+        if (alphas.p) return fail(MMX_ESTATE, "the adjoint takes a uniform alpha");
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
             if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL)
@@ -2018,6 +2022,7 @@ int mmx_add_thermal_noise(mmx_ctx *c, const double *T, double T_uniform, double 
         if (h.kind == K_THERMAL) return fail(MMX_EINVAL, "only one thermal term per simulation");
     return with_engine(c, [&](auto &e) {
         using TT = std::remove_pointer_t<decltype(e.y)>;
+        if (e.alphas.p) return fail(MMX_EINVAL, "the stochastic field needs a uniform alpha (field.jl:257-261)");
         HostTerm h;
         h.kind = K_THERMAL;
         h.s[0] = (double)(TT)T0;
@@ -2176,6 +2181,27 @@ int mmx_compute_energies(mmx_ctx *c, double t, double *Eo)
             E.term_mask = 1u << q;
             Eo[q] = e.reduce_scalar(e.eval(e.y, E), false);
         }
+        return MMX_OK;
+    });
+}
+
+int mmx_set_alpha(mmx_ctx *c, const double *alpha)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    if (alpha && !finite_all(alpha, c->N)) return fail(MMX_EINVAL, "NaN in alpha");
+    if (alpha)
+        for (const auto &h : c->terms)
+            if (h.kind == K_THERMAL)
+                return fail(MMX_EINVAL, "the stochastic field needs a uniform alpha (field.jl:257-261)");
+    return with_engine(c, [&](auto &e) {
+        if (!alpha) {
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            e.alphas.release();
+            return MMX_OK;
+        }
+        if (!e.alphas.p) e.alphas.alloc(c->N);
+        e.upload_scalar(alpha, e.alphas.p, c->N);
+        HIPCHECK(hipStreamSynchronize(c->stream));
         return MMX_OK;
     });
 }

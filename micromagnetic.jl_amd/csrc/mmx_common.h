@@ -71,8 +71,10 @@ struct FieldParams {
     const uint8_t *nbm;
     int nterms;
     TermDev<T> t[MAX_TERMS];
-    // LLG driver (src/llg/llg.jl:6-40)
+    // LLG driver (src/llg/llg.jl:6-40); alphas: per-cell damping of the
+    // SpatialLLG driver (src/llg/llg.jl:62-97), null for the uniform alpha
     T alpha, gamma;
+    const T *alphas;
     int precession;
     uint64_t seed;  // thermal noise stream
     // Slot form used by the specialised kernels when every kind of term

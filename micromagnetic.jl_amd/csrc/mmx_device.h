@@ -863,7 +863,8 @@ __device__ __forceinline__ void llg_rhs(const MMX_CONST FieldParams<T> &P, int64
         dx = dy = dz = T(0);
         return;
     }
-    const T alpha = P.alpha;
+    // spatial_llg_rhs_kernel! (llg.jl:62-97) reads alpha per cell
+    const T alpha = P.alphas ? P.alphas[I] : P.alpha;
     const T a = -P.gamma / (T(1) + alpha * alpha);
     T mm = mx * mx + my * my + mz * mz;
     T mh = mx * hx + my * hy + mz * hz;

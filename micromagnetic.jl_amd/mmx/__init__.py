@@ -186,8 +186,8 @@ class Sim:
     """Sim(mesh; driver="LLG", integrator="DormandPrince"): src/sim.jl:48-107."""
 
     def __init__(self, mesh, driver="LLG", name="dyn", integrator="DormandPrince", precision=F64, device=0):
-        if driver not in ("LLG", "SD"):
-            raise ValueError(f"driver {driver!r} is not available on this path (LLG and SD)")
+        if driver not in ("LLG", "SD", "SpatialLLG"):
+            raise ValueError(f"driver {driver!r} is not available on this path (LLG, SpatialLLG and SD)")
         if integrator not in _INTEGRATORS:
             raise ValueError("Supported integrators: " + " ".join(_INTEGRATORS))
         self.mesh, self.name = mesh, name
@@ -207,6 +207,8 @@ class Sim:
         self._integrator = Integrator(self, kind, _DEFAULT_STEP.get(kind, 0.0), 1e-6)
         check(L.mmx_set_integrator(self._h, kind, _DEFAULT_STEP.get(kind, 0.0), 1e-6))
         self._push_llg()
+        if driver == "SpatialLLG":  # alpha .= 0.1 per cell, src/driver.jl:149-152
+            set_alpha(self, 0.1)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -216,7 +218,12 @@ class Sim:
 
     def _push_llg(self):
         d = self.driver
-        check(_lib.load().mmx_set_llg(self._h, float(d.alpha), float(d.gamma), int(bool(d.precession))))
+        if np.ndim(d.alpha) == 0:
+            check(_lib.load().mmx_set_llg(self._h, float(d.alpha), float(d.gamma), int(bool(d.precession))))
+        else:  # SpatialLLG: the per-cell array travels through mmx_set_alpha
+            check(_lib.load().mmx_set_llg(self._h, 0.1, float(d.gamma), int(bool(d.precession))))
+            a = np.ascontiguousarray(d.alpha, dtype=np.float64)
+            check(_lib.load().mmx_set_alpha(self._h, dptr(a)))
 
     def _term_field(self, index, t=0.0):
         N = self.n_total
@@ -370,6 +377,19 @@ def _set_time_fn(sim, term, ft):
     check(_lib.load().mmx_set_term_time_fn(sim._h, term.index, C.cast(cb, C.c_void_p), None))
 
 
+def set_alpha(sim, alpha):
+    """set_alpha(sim, alpha): src/sim.jl:213-222 (SpatialLLG driver only; a
+    number, an array or a function of (i,j,k,dx,dy,dz) / (x,y,z))."""
+    if sim.driver_name != "SpatialLLG":
+        import warnings
+
+        warnings.warn("set_alpha only works for the SpatialLLG driver")
+        return None
+    a = init_scalar(sim.mesh, alpha)
+    sim.driver.alpha = a
+    return True
+
+
 def add_sot(sim, aj, bj, p, name="sot"):
     """add_sot(sim, aj, bj, p; name): src/micro/add_field.jl:951-966.  The torque
     enters as the field H = (1/gamma)(aj m x p + bj p) (kernels.jl:619-633)."""
@@ -461,11 +481,19 @@ def llg_rhs(sim, spin=None, t=0.0):
 
 
 def set_driver(sim, driver="LLG", integrator="DormandPrince", step=None, tol=1e-6):
-    """set_driver: src/sim.jl:589-620 ("LLG" with an integrator, or "SD")."""
-    if driver not in ("LLG", "SD"):
-        raise ValueError("Supported drivers: LLG SD")
+    """set_driver: src/sim.jl:589-620 ("LLG" or "SpatialLLG" with an integrator,
+    or "SD").  A change of driver creates a new one (create_driver): the
+    SpatialLLG damping starts at 0.1 per cell, the LLG one is uniform again."""
+    if driver not in ("LLG", "SD", "SpatialLLG"):
+        raise ValueError("Supported drivers: LLG SpatialLLG SD")
     kind = _lib.SD if driver == "SD" else _INTEGRATORS[integrator]
+    changed = sim.driver_name != driver
     sim.driver_name = driver
+    if changed and driver == "SpatialLLG":
+        set_alpha(sim, 0.1)
+    elif changed and np.ndim(sim.driver.alpha) != 0:
+        check(_lib.load().mmx_set_alpha(sim._h, None))
+        sim.driver.alpha = 0.1
     st = _DEFAULT_STEP.get(kind, 0.0) if step is None else float(step)
     sim._integrator = Integrator(sim, kind, st, tol)
     check(_lib.load().mmx_set_integrator(sim._h, kind, st, float(tol)))

@@ -85,6 +85,7 @@ struct orc_sim {
     double *k[7];
     double *y_current, *y_next, *y_temp;
     double t_eval; /* time of the evaluation in progress (torque ufun) */
+    double *alphas; /* SpatialLLG per-cell damping, NULL: uniform alpha */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -176,6 +177,7 @@ void orc_destroy(orc_sim *s)
     free(s->pins);
     free(s->spin);
     free(s->prespin);
+    free(s->alphas);
     free(s->field);
     free(s);
 }
@@ -1422,6 +1424,13 @@ void orc_set_llg(orc_sim *s, double alpha, double gamma, int precession)
     s->precession = precession;
 }
 
+/* set_alpha with the SpatialLLG driver: src/sim.jl:213-222 (NULL: uniform again) */
+void orc_set_alpha(orc_sim *s, const double *alpha)
+{
+    free(s->alphas);
+    s->alphas = alpha ? dup_array(alpha, s->N) : NULL;
+}
+
 void orc_llg_rhs(orc_sim *s, const double *m, double t, double *dm_dt)
 {
     orc_effective_field(s, m, t, NULL);
@@ -1435,7 +1444,9 @@ void orc_llg_rhs(orc_sim *s, const double *m, double t, double *dm_dt)
             dm_dt[j] = dm_dt[j + 1] = dm_dt[j + 2] = 0;
             continue;
         }
-        double a = -gamma / (1 + alpha * alpha);
+        /* spatial_llg_rhs_kernel! (llg.jl:62-97) takes alpha per cell */
+        const double al = s->alphas ? s->alphas[I] : alpha;
+        double a = -gamma / (1 + al * al);
         double mx = m[j], my = m[j + 1], mz = m[j + 2];
         double mm = mx * mx + my * my + mz * mz;
         double mh = mx * h[j] + my * h[j + 1] + mz * h[j + 2];
@@ -1448,9 +1459,9 @@ void orc_llg_rhs(orc_sim *s, const double *m, double t, double *dm_dt)
             f2 = cross_y(mx, my, mz, h1, h2, h3);
             f3 = cross_z(mx, my, mz, h1, h2, h3);
         }
-        dm_dt[j] = a * (f1 - h1 * alpha);
-        dm_dt[j + 1] = a * (f2 - h2 * alpha);
-        dm_dt[j + 2] = a * (f3 - h3 * alpha);
+        dm_dt[j] = a * (f1 - h1 * al);
+        dm_dt[j + 1] = a * (f2 - h2 * al);
+        dm_dt[j + 2] = a * (f3 - h3 * al);
     }
 }
 

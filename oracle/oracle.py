@@ -52,6 +52,7 @@ def lib():
             "orc_integrator_state": [C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_void_p],
             "orc_fft": [_dp, C.c_int, C.c_int],
             "orc_philox_normal": [C.c_uint64, C.c_uint64, C.c_uint64, _dp],
+            "orc_set_alpha": [C.c_void_p, _dp],
         }.items():
             getattr(L, name).argtypes = args
             getattr(L, name).restype = None
@@ -266,6 +267,11 @@ class OracleSim:
 
     def set_llg(self, alpha=0.1, gamma=2.21e5, precession=True):
         lib().orc_set_llg(self.h, alpha, gamma, int(bool(precession)))
+
+    def set_alpha(self, alpha):
+        """Per-cell damping of the SpatialLLG driver (None: uniform again)."""
+        self._alpha = None if alpha is None else _f64(alpha, self.N).copy()
+        lib().orc_set_alpha(self.h, None if alpha is None else _p(self._alpha))
 
     def llg_rhs(self, spin=None, t=0.0):
         spin = self.spin if spin is None else _f64(spin, 3 * self.N)

@@ -469,3 +469,58 @@ def test_signature_kernels_trajectory(oracle, case):
     err = np.max(np.abs(g.spin - o.spin))
     print(f"{name}: max |dm| {err:.2e}")
     assert err < 1e-12
+
+
+def test_spatial_llg_driver_set_alpha():
+    """test/test_spatial_llg.jl:15-37 on the HIP path."""
+    import mmx
+
+    note("SpatialLLG set_alpha")
+    mesh = mmx.FDMesh(nx=100, ny=1, nz=1, dx=2e-9, dy=2e-9, dz=1e-9)
+    sim = mmx.Sim(mesh, name="test_alpha", driver="SpatialLLG")
+    mmx.set_Ms(sim, 8.6e5)
+    mmx.set_alpha(sim, lambda i, j, k, dx, dy, dz: 0.1)
+    alpha = np.asarray(sim.driver.alpha)
+    assert alpha.max() == alpha.min()
+    mmx.set_alpha(sim, np.random.default_rng(1).uniform(size=100) * 0.2 + 0.01)
+    alpha = np.asarray(sim.driver.alpha)
+    assert alpha.max() <= 0.21 and alpha.min() >= 0.01
+    mmx.set_alpha(sim, lambda i, j, k, dx, dy, dz: 0.01 if i < 10 else 0.99)
+    alpha = np.asarray(sim.driver.alpha)
+    assert alpha[8] == 0.01 and alpha[9] == 0.99
+    with pytest.raises(mmx.MmxError):
+        mmx.add_thermal_noise(sim, 300.0)
+
+
+@pytest.mark.parametrize("integrator", ["RungeKutta", "DormandPrince"])
+def test_spatial_alpha_trajectory(oracle, integrator):
+    """Per-cell damping (spatial_llg_rhs_kernel!, llg.jl:62-97) through the
+    term-list kernels, with demag, against the oracle."""
+    import mmx
+
+    note(f"spatial alpha {integrator}")
+    nx, ny, nz = 12, 9, 3
+    N = nx * ny * nz
+    terms = [("exch", dict(A=1.3e-11)), ("dmi", dict(D=2e-3, type="interfacial")), ("demag", {})]
+    g, o = build(oracle, nx, ny, nz, pbc="y", Ms=np.full(N, 6e5), terms=terms, seed=21)
+    alphas = np.random.default_rng(22).uniform(0.01, 0.8, N)
+    if integrator == "RungeKutta":
+        mmx.set_driver(g, driver="SpatialLLG", integrator=integrator, step=2e-14)
+    else:
+        mmx.set_driver(g, driver="SpatialLLG", integrator=integrator, tol=1e-6)
+    mmx.set_alpha(g, alphas)
+    o.set_alpha(alphas)
+    assert relerr(mmx.llg_rhs(g), o.llg_rhs()) <= TOL
+    if integrator == "RungeKutta":
+        o.set_integrator(0, 2e-14, 0)
+        mmx.run_step(g, n=15)
+        for _ in range(15):
+            o.advance_step()
+        assert np.max(np.abs(g.spin - o.spin)) < 1e-12
+    else:
+        o.set_integrator(2, 0, 1e-6)
+        for i in range(1, 4):
+            mmx.run_until(g, i * 3e-13)
+            o.run_until(i * 3e-13)
+        assert g._integrator.nsteps == o.state()["nsteps"]
+        assert np.max(np.abs(g.spin - o.spin)) < 1e-10

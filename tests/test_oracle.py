@@ -336,6 +336,36 @@ def test_llg_fixed_macrospin(oracle, kind):
     assert abs(s.state()["t"] - 5e-11) < 1e-24
 
 
+def test_spatial_alpha_macrospins(oracle):
+    """SpatialLLG (src/llg/llg.jl:62-97): uncoupled cells with their own damping
+    each follow the damped-precession closed form with their own alpha, and a
+    uniform per-cell array reproduces the scalar-alpha right-hand side exactly."""
+    alphas = np.array([0.02, 0.05, 0.2, 0.6])
+    s = oracle.OracleSim(4, 1, 1)
+    s.set_Ms(np.full(4, 8e5))
+    s.set_llg(0.1, 2.21e5, True)
+    s.add_zeeman([0, 0, 1e5])
+    s.init_m0(np.tile([1.0, 0, 0], 4))
+    s.set_alpha(alphas)
+    s.set_integrator(2, 0, 1e-8)
+    for i in range(1, 6):
+        assert s.run_until(1e-10 * i) == 0
+    m = s.spin.reshape(4, 3)
+    for q, a in enumerate(alphas):
+        assert np.all(np.abs(m[q] - _analytical(a, 2.21e5, 1e5, 5e-10)) < 8e-7), (q, a)
+    rng = np.random.default_rng(5)
+    u = oracle.OracleSim(6, 5, 2, 2e-9, 2e-9, 2e-9)
+    u.set_Ms(np.full(60, 8e5))
+    u.set_llg(0.37, 2.21e5, True)
+    u.add_exch(1.3e-11)
+    u.init_m0(rng.standard_normal(180))
+    r0 = u.llg_rhs()
+    u.set_alpha(np.full(60, 0.37))
+    assert np.array_equal(u.llg_rhs(), r0)
+    u.set_alpha(None)
+    assert np.array_equal(u.llg_rhs(), r0)
+
+
 def test_llg_rhs_invariants(oracle):
     """dm/dt is perpendicular to m; pinned cells get zero; precession off gives pure damping."""
     rng = np.random.default_rng(3)
