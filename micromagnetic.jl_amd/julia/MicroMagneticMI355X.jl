@@ -70,8 +70,14 @@ function attach!(sim::MicroMagnetic.MicroSim; device::Integer=0)
         add_term!(h, term)
     end
     drv = sim.driver
-    check(ccall((:mmx_set_llg, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint), h,
-                Float64(drv.alpha), Float64(drv.gamma), Cint(drv.precession)))
+    if drv isa MicroMagnetic.SpatialLLG  # per-cell damping, src/llg/llg.jl:62-97
+        check(ccall((:mmx_set_llg, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint), h,
+                    0.1, Float64(drv.gamma), Cint(drv.precession)))
+        check(ccall((:mmx_set_alpha, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}), h, host(drv.alpha)))
+    else
+        check(ccall((:mmx_set_llg, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint), h,
+                    Float64(drv.alpha), Float64(drv.gamma), Cint(drv.precession)))
+    end
     set_integrator!(sim)
     return h
 end
