@@ -265,6 +265,14 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
 #ifndef MMX_P3_WAVE
 #define MMX_P3_WAVE 1
 #endif
+// Transforms of zero-padded data: the nonzero inputs and the wanted outputs
+// of a length-L convolution transform lie in its first half (n <= L/2, since
+// L >= 2n - 1), i.e. in elements m < EPT/2 of every thread; with MMX_PRUNE the
+// others are compile-time zeros / dead values, which prunes the first forward
+// and the last inverse stage.
+#ifndef MMX_PRUNE
+#define MMX_PRUNE 1
+#endif
 // elements per thread of the wave-form z transform for FP32 (FP64: MMX_P3_E)
 #ifndef MMX_P3_E32
 #define MMX_P3_E32 8
@@ -305,6 +313,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     using F = FftShape<L, P3W<T>::E>;
     constexpr int EPT = F::EPT, TPS = F::TPS;
     static_assert(TPS <= 64, "one sequence per wave");
+    constexpr bool PR = MMX_PRUNE && L >= 2;  // Pz = 1 (nz = 1) is the one unpadded length
     constexpr int TW = 64 / TPS;
     constexpr int LS = F::XS + MMX_P3_PAD;
     constexpr int NB = MMX_P3_NBW;
@@ -331,7 +340,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
             const int z = j + m * TPS;
-            v[c][m] = (ok && z < n_in) ? col[c * cs_b + z * zs_b] : cplx<T>{0, 0};
+            v[c][m] = (ok && z < n_in && (!PR || m < EPT / 2)) ? col[c * cs_b + z * zs_b] : cplx<T>{0, 0};
         }
     const int kyf = (ky <= Py / 2) ? ky : Py - ky;
     const T sy = (ky <= Py / 2) ? T(1) : T(-1);
@@ -386,7 +395,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
             const int z = j + m * TPS;
-            if (z < nz) col[c * cs_b + z * zs_b] = v[c][m];
+            if ((!PR || m < EPT / 2) && z < nz) col[c * cs_b + z * zs_b] = v[c][m];
         }
 }
 
