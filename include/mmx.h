@@ -87,6 +87,12 @@ int mmx_add_cubic_anis(mmx_ctx *ctx, const double *Kc, double Kc0, const double 
 int mmx_add_zeeman(mmx_ctx *ctx, const double *H, double Hx, double Hy, double Hz);
 /* update_zeeman(sim, H0): src/micro/add_field.jl:60-100 */
 int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double Hy, double Hz);
+/* add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200, TimeZeeman field.jl:12-33,
+ * time_zeeman_kernel! src/micro/kernels.jl:39-49 -- H = H0 .* (fx, fy, fz)(t); fn writes the
+ * three factors (a scalar time_fun gives three equal ones) and runs on the host at every
+ * evaluation time.  3N per-cell H0 or NULL with (Hx,Hy,Hz). */
+int mmx_add_zeeman_time(mmx_ctx *ctx, const double *H, double Hx, double Hy, double Hz,
+                        void (*fn)(double t, double *f, void *user), void *user);
 /* add_demag(sim; Nx, Ny, Nz): src/micro/add_field.jl:403-419 -> init_demag, src/micro/demag.jl:30-100.
  * The FP64 Newell tensor is built on the host; its transform runs on the GPU in FP64. */
 int mmx_add_demag(mmx_ctx *ctx, int Nx, int Ny, int Nz);

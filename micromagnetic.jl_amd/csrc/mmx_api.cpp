@@ -95,6 +95,7 @@ struct HostTerm {
     std::vector<double> rtensor;  // demag: compact real-space tensor (6N)
     double (*ufun)(double, void *) = nullptr;  // torque amplitude ufun(t) (null: 1)
     void *uuser = nullptr;
+    void (*ft3)(double, double *, void *) = nullptr;  // time Zeeman (fx, fy, fz)(t)
 };
 
 struct CtxBase {
@@ -390,6 +391,13 @@ struct Engine : CtxBase {
             const double u = h.ufun ? h.ufun(t_eval, h.uuser) : 1.0;
             if (h.kind == K_STT_SL) d.tfac[i] = u * h.s[7];    // ft = ufun(t)*beta, field.jl:351
             if (h.kind == K_STT_ZL) d.tfac[i] = u / C->gamma;  // ut = ufun(t)/gamma, field.jl:335
+            if (h.kind == K_ZEEMAN_T) {                         // time_fun(t), field.jl:16-26
+                double f[3] = {1, 1, 1};
+                if (h.ft3) h.ft3(t_eval, f, h.uuser);
+                d.tfac[i] = f[0];
+                d.tfy[i] = f[1];
+                d.tfz[i] = f[2];
+            }
         }
         return d;
     }
@@ -1023,8 +1031,8 @@ struct Engine : CtxBase {
         if (alphas.p) return fail(MMX_ESTATE, "the adjoint takes a uniform alpha");
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
-            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL)
-                return fail(MMX_EINVAL, "the adjoint covers field terms without spin torques");
+            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T)
+                return fail(MMX_EINVAL, "the adjoint covers time-independent field terms without spin torques");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
         const int64_t nck = (nsteps + every - 1) / every;
         adj.recorded = false;
@@ -2123,6 +2131,29 @@ int mmx_add_stt_zhangli(mmx_ctx *c, const double *J, double Jx, double Jy, doubl
     });
 }
 
+int mmx_add_zeeman_time(mmx_ctx *c, const double *H, double Hx, double Hy, double Hz,
+                        void (*fn)(double t, double *f, void *user), void *user)
+{
+    if (!c || !fn) return fail(MMX_EINVAL, "null argument");
+    if (H && !finite_all(H, 3 * c->N)) return fail(MMX_EINVAL, "NaN in H");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // TimeZeeman: add_field.jl:161-200 (init_field in T)
+        h.kind = K_ZEEMAN_T;
+        h.ft3 = fn;
+        h.uuser = user;
+        if (!H) {
+            h.uniform = 1;
+            h.s[0] = (double)(TT)Hx;
+            h.s[1] = (double)(TT)Hy;
+            h.s[2] = (double)(TT)Hz;
+            return add_term(c, e, h, nullptr, nullptr, nullptr);
+        }
+        const TT *p = e.new_array(H, 3 * c->N, true);
+        return add_term(c, e, h, p, p + c->N, p + 2 * c->N);
+    });
+}
+
 int mmx_set_term_time_fn(mmx_ctx *c, int term, double (*fn)(double t, void *user), void *user)
 {
     if (!c) return fail(MMX_EINVAL, "null context");
@@ -2429,7 +2460,8 @@ int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n)
         if (c->ikind != members[0]->ikind) return fail(MMX_EINVAL, "members must use the same integrator");
         if (!(c->step > 0)) return fail(MMX_EINVAL, "member %d has no step size", i);
         for (const auto &h : c->terms)
-            if (h.ufun) return fail(MMX_EINVAL, "member %d has a time-dependent torque; batches take constant ones", i);
+            if (h.ufun || h.ft3)
+                return fail(MMX_EINVAL, "member %d has a time-dependent term; batches take constant ones", i);
         for (int j = 0; j < i; j++)
             if (members[j] == c) return fail(MMX_EINVAL, "member %d appears twice", i);
     }

@@ -47,6 +47,7 @@ enum TermKind : int {
     K_SOT,          // damping/field-like torque, aj (a) or uniform s[3]; p s[0..2], bj s[4]
     K_STT_SL,       // Slonczewski torque, J (a) or uniform s[3]; p s[0..2], Lambda^2 s[4], P s[5], xi s[6]
     K_STT_ZL,       // Zhang-Li torque, b*J planes (b), xi (a) or uniform s[3]; dx, dy, dz s[0..2]
+    K_ZEEMAN_T,     // time-dependent Zeeman: static planes (a,b,c) or uniform s[0..2], times ft(t)
 };
 
 template <class T>
@@ -149,8 +150,9 @@ constexpr uint32_t sig_ordp(uint32_t sig)
 struct Dyn {
     double thermal_factor;  // 2 alpha kB / (V gamma dt), src/micro/field.jl:261
     uint64_t noise_step;    // integrator step count keying the noise stream
-    double tfac[MAX_TERMS]; // time factor of a torque term at the evaluation time
-                            // (Slonczewski: ufun(t)*beta; Zhang-Li: ufun(t)/gamma)
+    double tfac[MAX_TERMS]; // time factor of a term at the evaluation time (Slonczewski:
+                            // ufun(t)*beta; Zhang-Li: ufun(t)/gamma; time Zeeman: fx(t))
+    double tfy[MAX_TERMS], tfz[MAX_TERMS];  // time Zeeman: fy(t), fz(t)
 };
 
 // What the last pass does with the field of a cell.

@@ -325,6 +325,20 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             hz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
             e = (double)(-Ms * V * (mx * hx + my * hy + mz * hz));
         } break;
+        case K_ZEEMAN_T: {  // time_zeeman_kernel!: kernels.jl:39-49, factors from field.jl:12-33
+            const T fx = (T)dyn.tfac[tt], fy = (T)dyn.tfy[tt], fz = (T)dyn.tfz[tt];
+            if (D.uniform) {
+                hx = D.s[0] * fx;
+                hy = D.s[1] * fy;
+                hz = D.s[2] * fz;
+            } else {
+                hx = D.a[I] * fx;
+                hy = D.b[I] * fy;
+                hz = D.c[I] * fz;
+            }
+            T mh = mx * hx + my * hy + mz * hz;
+            e = (double)(-V * Ms * mh);
+        } break;
         case K_SOT: {  // df_torque_kernel!: kernels.jl:619-633 (no energy, no Ms rule)
             const T a = (D.uniform ? D.s[3] : D.a[I]) / P.gamma;
             const T b = D.s[4] / P.gamma;

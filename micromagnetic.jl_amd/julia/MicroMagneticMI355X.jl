@@ -126,6 +126,25 @@ function add_term!(h, t::MicroMagnetic.Zeeman)
     ccall((:mmx_add_zeeman, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h,
           host(t.field), 0, 0, 0) |> check
 end
+function add_term!(h, t::MicroMagnetic.TimeZeeman)
+    # time_fun(t) returns one factor or (fx, fy, fz): src/micro/field.jl:16-26
+    fun, scalar = t.time_fun, t.is_scalar
+    f = (s::Cdouble, out::Ptr{Cdouble}, user::Ptr{Cvoid}) -> begin
+        v = fun(s)
+        fx, fy, fz = scalar ? (v, v, v) : v
+        unsafe_store!(out, fx, 1)
+        unsafe_store!(out, fy, 2)
+        unsafe_store!(out, fz, 3)
+        nothing
+    end
+    cf = @cfunction($f, Cvoid, (Cdouble, Ptr{Cdouble}, Ptr{Cvoid}))
+    idx = ccall((:mmx_add_zeeman_time, libmmx), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}), h,
+                host(t.init_field), 0, 0, 0, cf, C_NULL)
+    check(idx)
+    timefns[(h, idx)] = cf
+    return idx
+end
 function add_term!(h, t::MicroMagnetic.Demag)
     error("attach! needs the macro-PBC image counts of the demag term: call attach_demag!(sim, Nx, Ny, Nz)")
 end

@@ -128,6 +128,10 @@ class Term:
     def energy(self):
         return self.sim._term_field(self.index)[1]
 
+    def effective_field(self, t=0.0):
+        """effective_field(term, sim, sim.spin, t): the term's (field, energy) at time t."""
+        return self.sim._term_field(self.index, t)
+
 
 class LLGDriver:
     """The LLG driver fields (src/driver.jl:15-21); writes go to the library."""
@@ -327,8 +331,14 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
-def add_zeeman(sim, H0, name="zeeman"):
+def add_zeeman(sim, H0, ft=None, name=None):
+    """add_zeeman(sim, H0; name="zeeman"): src/micro/add_field.jl:11-45, and with a
+    time function add_zeeman(sim, H0, ft; name="timezeeman"): add_field.jl:161-200;
+    ft(t) returns a number or a 3-tuple scaling (Hx, Hy, Hz)."""
     L = _lib.load()
+    if ft is not None:
+        return _add_time_zeeman(sim, H0, ft, name or "timezeeman")
+    name = name or "zeeman"
     if isinstance(H0, tuple) and len(H0) == 3:
         return sim._add(L.mmx_add_zeeman(sim._h, None, *map(float, H0)), name)
     H = np.ascontiguousarray(init_vector(sim.mesh, H0))
@@ -363,6 +373,37 @@ def add_thermal_noise(sim, Temp, name="thermal", T0=0.0, seed=0):
 
 
 _TIMEFN = None
+_TIMEFN3 = None
+
+
+def _add_time_zeeman(sim, H0, ft, name):
+    global _TIMEFN3
+    import ctypes as C
+
+    f0 = ft(0.0)
+    n = 1 if np.ndim(f0) == 0 else len(f0)
+    if n not in (1, 3):
+        raise ValueError("The time_fun function should return either 1 value (scalar) or 3 values (as a tuple).")
+    if _TIMEFN3 is None:
+        _TIMEFN3 = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+    def factors(t, f, user):
+        v = ft(t)
+        if n == 1:
+            f[0] = f[1] = f[2] = float(v)
+        else:
+            f[0], f[1], f[2] = map(float, v)
+
+    cb = _TIMEFN3(factors)
+    L = _lib.load()
+    if isinstance(H0, tuple) and len(H0) == 3:
+        idx = L.mmx_add_zeeman_time(sim._h, None, *map(float, H0), C.cast(cb, C.c_void_p), None)
+    else:
+        H = np.ascontiguousarray(init_vector(sim.mesh, H0))
+        idx = L.mmx_add_zeeman_time(sim._h, dptr(H), 0.0, 0.0, 0.0, C.cast(cb, C.c_void_p), None)
+    t = sim._add(idx, name)
+    t._ft = cb  # the library keeps only the pointer
+    return t
 
 
 def _set_time_fn(sim, term, ft):

@@ -39,7 +39,8 @@ enum {
     T_THERMAL,
     T_SOT,
     T_STT_SL,
-    T_STT_ZL
+    T_STT_ZL,
+    T_ZEEMAN_T
 };
 
 typedef struct {
@@ -60,6 +61,8 @@ typedef struct {
     /* torque amplitude ufun(t) (NULL: 1) */
     double (*ufun)(double, void *);
     void *uuser;
+    /* time Zeeman factors (fx, fy, fz)(t) */
+    void (*ft3)(double, double *, void *);
 } term_t;
 
 enum { I_RK4 = 0, I_HEUN = 1, I_DOPRI54 = 2, I_BS23 = 3 };
@@ -1334,6 +1337,34 @@ static void field_stt_zl(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200 (H0 3N) */
+int orc_add_zeeman_time(orc_sim *s, const double *H, void (*fn)(double, double *, void *), void *user)
+{
+    term_t *t = new_term(s, T_ZEEMAN_T);
+    t->a = dup_array(H, 3 * (size_t)s->N);
+    t->ft3 = fn;
+    t->uuser = user;
+    return s->nterms - 1;
+}
+
+/* TimeZeeman: src/micro/field.jl:12-33, time_zeeman_kernel! src/micro/kernels.jl:39-49 */
+static void field_zeeman_time(orc_sim *s, term_t *t, const double *m)
+{
+    double f[3] = {1, 1, 1};
+    if (t->ft3) t->ft3(s->t_eval, f, t->uuser);
+    const double V = s->volume;
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        h[j] = t->a[j] * f[0];
+        h[j + 1] = t->a[j + 1] * f[1];
+        h[j + 2] = t->a[j + 2] * f[2];
+        double mh = m[j] * h[j] + m[j + 1] * h[j + 1] + m[j + 2] * h[j + 2];
+        t->energy[I] = -V * s->mu0_Ms[I] * mh;
+    }
+}
+
 int orc_set_term_time_fn(orc_sim *s, int term, double (*fn)(double, void *), void *user)
 {
     if (term < 0 || term >= s->nterms) return -1;
@@ -1363,6 +1394,7 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_SOT: field_sot(s, t, m); break;
     case T_STT_SL: field_stt_sl(s, t, m); break;
     case T_STT_ZL: field_stt_zl(s, t, m); break;
+    case T_ZEEMAN_T: field_zeeman_time(s, t, m); break;
     }
 }
 

@@ -74,6 +74,7 @@ def lib():
             "orc_add_stt_slonczewski": [C.c_void_p, _dp] + [C.c_double] * 7,
             "orc_add_stt_zhangli": [C.c_void_p, _dp, C.c_double, _dp, C.c_int],
             "orc_set_term_time_fn": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
+            "orc_add_zeeman_time": [C.c_void_p, _dp, C.c_void_p, C.c_void_p],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
             "orc_advance_step": [C.c_void_p],
@@ -87,6 +88,20 @@ def lib():
 
 
 _TIMEFN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+_TIMEFN3 = C.CFUNCTYPE(None, C.c_double, _dp, C.c_void_p)
+
+
+def _factor3(ft):
+    """A time_fun returning a number or a 3-tuple, as (fx, fy, fz): src/micro/field.jl:16-26."""
+
+    def cb(t, f, user):
+        v = ft(t)
+        if np.ndim(v) == 0:
+            f[0] = f[1] = f[2] = float(v)
+        else:
+            f[0], f[1], f[2] = map(float, v)
+
+    return _TIMEFN3(cb)
 
 
 def _p(a):
@@ -205,6 +220,15 @@ class OracleSim:
         if H.size == 3:
             H = np.tile(H, self.N)
         return lib().orc_add_zeeman(self.h, _p(_f64(H.reshape(-1), 3 * self.N)))
+
+    def add_zeeman_time(self, H, ft):
+        """add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200."""
+        H = np.asarray(H, dtype=np.float64)
+        if H.size == 3:
+            H = np.tile(H, self.N)
+        cb = _factor3(ft)
+        self._keep = getattr(self, "_keep", []) + [cb]
+        return lib().orc_add_zeeman_time(self.h, _p(_f64(H.reshape(-1), 3 * self.N)), C.cast(cb, C.c_void_p), None)
 
     def add_demag(self, Nx=0, Ny=0, Nz=0, fft=True):
         f = lib().orc_add_demag if fft else lib().orc_add_demag_direct
