@@ -216,6 +216,49 @@ def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
             "identical_to_plain": same, "tuned": {"cells_per_thread": cpt, "grid": grid, "work_items": items}}
 
 
+def bench_neb(mmx, precision, steps=20, warmup=3, chains=2, frames=22, seed=6006):
+    """Batching form, NEB half: chains of 24 images (22 free) on a 128x128x4
+    film between a seeded skyrmion and the uniform state, stepped by the NEB
+    driver (BS23, tol 1e-5); the field of every free image is one launch per
+    step, the band's tangent, force and stage updates one launch each per stage."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz, d = 128, 128, 4, 2e-9
+    N = nx * ny * nz
+    negs = []
+    for _ in range(chains):
+        mesh = mmx.FDMesh(dx=d, dy=d, dz=d, nx=nx, ny=ny, nz=nz, pbc="xy")
+        s = mmx.Sim(mesh, precision=precision)
+        mmx.set_Ms(s, rng.uniform(4e5, 1.2e6))
+        mmx.add_exch(s, rng.uniform(5e-12, 2e-11))
+        mmx.add_dmi(s, rng.uniform(1e-3, 4e-3), type="interfacial")
+        mmx.add_anis(s, rng.uniform(2e5, 1e6), axis=(0, 0, 1))
+        mmx.add_zeeman(s, (0, 0, 0.1 / mmx.mu_0))
+        c = rng.uniform(-20e-9, 20e-9, 2)
+        R = rng.uniform(6e-9, 12e-9)
+        x = (np.arange(nx) - (nx - 1) / 2) * d - c[0]
+        y = (np.arange(ny) - (ny - 1) / 2) * d - c[1]
+        X, Y = np.meshgrid(x, y)
+        th = np.pi * np.exp(-(X * X + Y * Y) / (R * R))
+        ph = np.arctan2(Y, X) + np.pi / 2
+        left = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=-1)
+        left = np.tile(left.reshape(-1, 3), (nz, 1)).reshape(-1)
+        negs.append((s, mmx.NEB(s, [left, (0.0, 0.0, 1.0)], [frames])))
+    for _, nb in negs:
+        nb.advance(warmup)
+    t0 = time.perf_counter()
+    for _, nb in negs:
+        nb.advance(steps)  # synchronous: one host sync per BS23 try
+    t = time.perf_counter() - t0
+    n_img = negs[0][1].n_images
+    st = [nb.state for _, nb in negs]
+    work = chains * n_img * N * steps
+    return {"workload": f"{chains} NEB chains of {n_img + 2} images on a 128x128x4 film (seed {seed}), "
+                        f"exch+iDMI+Ku+Zeeman, BS23 tol 1e-5",
+            "value": round(work / t / 1e6, 1), "unit": "Mcell-image-steps/s",
+            "ms_per_chain_step": round(t / (chains * steps) * 1e3, 4),
+            "fevals_per_step": round(sum(x["nfevals"] for x in st) / sum(x["nsteps"] for x in st), 3)}
+
+
 def bench_adjoint(mmx, precision, nsteps=2000, every=50, n_fd=8, seed=5005):
     """Differentiation form on the 128x128x8 film: Ku(x,y) and D(x,y) Gaussian
     random fields (means 6e5 J/m^3 and 2.5e-3 J/m^2, 20 % std, 10 nm
@@ -431,6 +474,10 @@ def main():
         log(f"[bench] batch FP64: {b64['value']} Mcell-evals/s vs plain {b64['plain_value']}")
         b32 = bench_batch(mmx, mmx.F32, steps=100, warmup=5)
         log(f"[bench] batch FP32: {b32['value']} Mcell-evals/s vs plain {b32['plain_value']}")
+        n64 = bench_neb(mmx, mmx.F64)
+        log(f"[bench] NEB FP64: {n64['value']} {n64['unit']}, {n64['ms_per_chain_step']} ms per chain step")
+        n32 = bench_neb(mmx, mmx.F32)
+        log(f"[bench] NEB FP32: {n32['value']} {n32['unit']}, {n32['ms_per_chain_step']} ms per chain step")
         log("[bench] differentiation form (FP64 with finite-difference probes, FP32)")
         a64 = bench_adjoint(mmx, mmx.F64, n_fd=args.fd_cells)
         log(f"[bench] adjoint FP64: {a64['value']} Mcell-evals/s (plain {a64['plain_value']}), "
@@ -441,7 +488,8 @@ def main():
         log(f"[bench] adjoint FP32: {a32['value']} Mcell-evals/s, error vs FP64 {a32['grad_rel_err_vs_fp64']}")
         for a in (a64, a32):
             del a["grads"]
-        forms = {"batched": {"fp64": b64, "fp32": b32}, "adjoint": {"fp64": a64, "fp32": a32}}
+        forms = {"batched": {"fp64": b64, "fp32": b32, "neb_fp64": n64, "neb_fp32": n32},
+                 "adjoint": {"fp64": a64, "fp32": a32}}
 
     cpu = None
     if not args.no_cpu:

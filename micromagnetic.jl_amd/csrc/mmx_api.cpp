@@ -501,6 +501,23 @@ struct Engine : CtxBase {
         return nb;
     }
 
+    // fields and per-cell energies of nimg stacked images (three planes of N
+    // each) through the five passes at once: the images' spectra lie side by
+    // side in A and B (3*nimg components), so each pass is one launch.
+    void eval_images(const T *Y, int nimg, T *F, T *ecell, cplx<T> *A, cplx<T> *B)
+    {
+        const int nx = G.nx, ny = G.ny, nz = G.nz;
+        launch_p1<T>(G, Y, N, mu0Ms.p, nx, ny, nz, 3 * nimg, A, tw.p, C->stream);
+        launch_coly<T>(G, A, B, ny, ny, G.Py, G.Py, nz, 3 * nimg, 0, tw.p, C->stream);
+        launch_convz<T>(G, B, Kt.p, Lline.p, nz, nz, 3, 0, tw.p, C->stream, nimg);
+        launch_coly<T>(G, B, A, G.Py, G.Py, ny, ny, nz, 3 * nimg, 1, tw.p, C->stream);
+        Epi<T> x = epi(EPI_FIELD);
+        x.out = F;
+        x.e_out = ecell;
+        launch_p5_images<T>(G, A, params_dev(), dyn(), x, Y, nimg, tw.p, C->stream);
+        HIPCHECK(hipGetLastError());
+    }
+
     double reduce_scalar(int nblocks, bool is_max)
     {
         launch_reduce(partial.p, nblocks, 1, 1, is_max ? 1 : 0, red.p, C->stream);
@@ -1202,6 +1219,8 @@ struct NebEngine : CtxBase {
     T *k[4];
     DevBuf<double> energy, dist, coef, part;
     DevBuf<FieldParams<T>> Pneb;  // the sim's block with the band's LLG parameters
+    DevBuf<cplx<T>> An, Bn;       // demag spectra of every free image (eval_images)
+    DevBuf<T> ecell_n;            // per-cell energies of every free image
     double spring = 1e5;
     int climb = -1;
     double alpha = 0.2, gamma = 2.21e5;
@@ -1260,7 +1279,22 @@ struct NebEngine : CtxBase {
     void refresh()
     {
         if (fresh) return;
-        for (int q = 0; q < n; q++) field_energy(Y.p + (size_t)q * V, F.p + (size_t)q * V, energy.p + q + 1);
+        if (!e.demag) {  // every image in one launch
+            launch_neb_field<T>(e.params_dev(), e.dyn(), e.all_mask(), N, n, nb, Y.p, F.p, part.p, energy.p + 1,
+                                st());
+            HIPCHECK(hipGetLastError());
+        } else {
+            if (!An.p) {
+                An.alloc((size_t)n * e.Abuf.n);
+                Bn.alloc((size_t)n * e.Bbuf.n);
+                ecell_n.alloc((size_t)n * N);
+                HIPCHECK(hipMemsetAsync(An.p, 0, An.n * sizeof(cplx<T>), st()));
+                HIPCHECK(hipMemsetAsync(Bn.p, 0, Bn.n * sizeof(cplx<T>), st()));
+            }
+            e.eval_images(Y.p, n, F.p, ecell_n.p, An.p, Bn.p);
+            launch_sum_images<T>(ecell_n.p, N, n, nb, part.p, energy.p + 1, st());
+            HIPCHECK(hipGetLastError());
+        }
         launch_neb_dist<T>(N, n, Y.p, L.p, R.p, nb, part.p, dist.p, st());
         HIPCHECK(hipGetLastError());
         fresh = true;
@@ -1373,12 +1407,11 @@ struct NebEngine : CtxBase {
 
     double max_dm_vs_prespin()
     {
+        launch_neb_maxdiff<T>(prespin.p, Y.p, N, n, nb, part.p, e.red.p, st());
+        HIPCHECK(hipGetLastError());
         double mx = 0;
-        for (int q = 0; q < n; q++) {
-            int g = launch_maxdiff<T>(prespin.p + (size_t)q * V, Y.p + (size_t)q * V, N, e.partial.p, st());
-            HIPCHECK(hipGetLastError());
-            mx = std::max(mx, e.reduce_scalar(g, true));
-        }
+        HIPCHECK(hipMemcpyAsync(&mx, e.red.p, sizeof(double), hipMemcpyDeviceToHost, st()));
+        HIPCHECK(hipStreamSynchronize(st()));
         return mx;
     }
 

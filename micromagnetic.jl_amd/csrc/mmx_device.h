@@ -390,92 +390,6 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
     }
 }
 
-// Field and energy of a term that needs only this cell (Zeeman, anisotropies,
-// demag, thermal); returns false for the neighbour terms.
-template <class T>
-__device__ __forceinline__ bool local_term(const MMX_CONST TermDev<T> &D, const MMX_CONST FieldParams<T> &P, const Dyn &dyn, int64_t I,
-                                           T mx, T my, T mz, T Ms, T hdx, T hdy, T hdz, T &hx, T &hy, T &hz,
-                                           double &e)
-{
-#pragma clang fp contract(off)
-    const T V = P.volume;
-    const int64_t N = P.N;
-    hx = hy = hz = T(0);
-    e = 0;
-    switch (D.kind) {
-    case K_ZEEMAN:
-    case K_ZEEMAN_U: {  // static field; energy src/micro/kernels.jl:14-20
-        if (D.kind == K_ZEEMAN) {
-            hx = D.a[I];
-            hy = D.b[I];
-            hz = D.c[I];
-        } else {
-            hx = D.s[0];
-            hy = D.s[1];
-            hz = D.s[2];
-        }
-        T mh = mx * hx + my * hy + mz * hz;
-        e = (double)(-V * Ms * mh);
-    } break;
-    case K_ANIS:
-    case K_ANIS_S: {  // anisotropy_kernel! / spatial: kernels.jl:54-105
-        if (Ms == T(0)) break;
-        T ax, ay, az;
-        if (D.kind == K_ANIS) {
-            ax = D.s[0];
-            ay = D.s[1];
-            az = D.s[2];
-        } else {
-            ax = D.b[I];
-            ay = D.b[N + I];
-            az = D.b[2 * N + I];
-        }
-        T Ku = D.uniform ? D.s[3] : D.a[I];
-        T Ms_inv = T(2.0) / Ms;
-        T sa = mx * ax + my * ay + mz * az;
-        hx = Ku * sa * ax * Ms_inv;
-        hy = Ku * sa * ay * Ms_inv;
-        hz = Ku * sa * az * Ms_inv;
-        e = (double)(Ku * (T(1.0) - sa * sa) * V);
-    } break;
-    case K_CUBIC: {  // cubic_anisotropy_kernel!: kernels.jl:110-137
-        if (Ms == T(0)) break;
-        T Kc = D.uniform ? D.s[9] : D.a[I];
-        T Ms_inv = T(4.0) * Kc / Ms;
-        T mxp = D.s[0] * mx + D.s[1] * my + D.s[2] * mz;
-        T myp = D.s[3] * mx + D.s[4] * my + D.s[5] * mz;
-        T mzp = D.s[6] * mx + D.s[7] * my + D.s[8] * mz;
-        T mxp3 = mxp * mxp * mxp;
-        T myp3 = myp * myp * myp;
-        T mzp3 = mzp * mzp * mzp;
-        hx = Ms_inv * (mxp3 * D.s[0] + myp3 * D.s[3] + mzp3 * D.s[6]);
-        hy = Ms_inv * (mxp3 * D.s[1] + myp3 * D.s[4] + mzp3 * D.s[7]);
-        hz = Ms_inv * (mxp3 * D.s[2] + myp3 * D.s[5] + mzp3 * D.s[8]);
-        e = (double)(-Kc * (mxp * mxp3 + myp * myp3 + mzp * mzp3) * V);
-    } break;
-    case K_DEMAG: {  // collect_h_kernel!: src/micro/demag.jl:435-449
-        hx = hdx;
-        hy = hdy;
-        hz = hdz;
-        T mh = mx * hx + my * hy + mz * hz;
-        e = (double)(T(-0.5) * Ms * V * mh);
-    } break;
-    case K_THERMAL: {  // stochastic_field_kernel!: kernels.jl:469-494
-        if (!(Ms > T(0))) break;
-        T Tl = (D.uniform ? D.s[1] : D.a[I]) + D.s[0];
-        T scale = (T)sqrt((T)dyn.thermal_factor * Tl / Ms);
-        uint64_t base = 3 * (uint64_t)I;
-        hx = (T)philox_normal(P.seed, dyn.noise_step, base) * scale;
-        hy = (T)philox_normal(P.seed, dyn.noise_step, base + 1) * scale;
-        hz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
-        e = (double)(-Ms * V * (mx * hx + my * hy + mz * hz));
-    } break;
-    default:
-        return false;
-    }
-    return true;
-}
-
 // ---- slot form ---------------------------------------------------------------
 // When every kind of term appears at most once, the kernels take this form.
 // SlotCfg holds what every cell needs from the parameter block (wave-uniform

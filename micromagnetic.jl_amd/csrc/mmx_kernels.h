@@ -31,10 +31,17 @@ void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, 
 int convz_grouped_tensor(const DemagGeom &G, int elsize);
 template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in,
-                  int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s);
+                  int ncomp, int fwd_only, const cplx<T> *tw, hipStream_t s, int nimg = 1);
 template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
               const T *m, int fast, uint32_t sig, const cplx<T> *tw, hipStream_t s);
+// fields and per-cell energies (EPI_FIELD) of nimg stacked images whose spectra
+// lie side by side in A (grid y = image)
+template <class T>
+void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn,
+                      const Epi<T> &E, const T *m, int nimg, const cplx<T> *tw, hipStream_t s);
+template <class T>
+void launch_sum_images(const T *v, int64_t N, int n, int nb, double *partial, double *out, hipStream_t s);
 // kernels compiled for a term signature (mmx_kernels_sig.hip); -1 when none matches
 template <class T>
 int launch_p5_sig(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
@@ -86,6 +93,11 @@ int launch_sum(const T *v, int64_t n, double *partial, hipStream_t s);
 template <class T>
 void launch_neb_dist(int64_t N, int n, const T *Y, const T *L, const T *R, int nb, double *partial, double *dist,
                      hipStream_t s);
+template <class T>
+void launch_neb_field(const FieldParams<T> *P, const Dyn &dyn, uint32_t mask, int64_t N, int n, int nb, const T *Y,
+                      T *F, double *partial, double *energy, hipStream_t s);
+template <class T>
+void launch_neb_maxdiff(const T *a, const T *b, int64_t N, int n, int nb, double *partial, double *out, hipStream_t s);
 template <class T>
 void launch_neb_tangent(int64_t N, int n, const T *Ys, const T *L, const T *R, const double *energy, const T *F,
                         T *tan, int nb, double *partial, const double *dist, double k, int climb, double *coef,

@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(Shape<L>::BZ) k_convz(DemagGeom G, cplx<T> *__
     const bool ok = kx < G.Kx && (half == 0 || (ky0 >= 1 && ky0 < Py / 2));
     const int64_t cs_b = (int64_t)nzb * Py * KS;  // component stride in B
     const int64_t zs_b = (int64_t)Py * KS;        // z stride in B
-    cplx<T> *col = B + (int64_t)(ok ? ky : 0) * KS + (ok ? kx : 0);
+    cplx<T> *col = B + (int64_t)blockIdx.z * 3 * cs_b + (int64_t)(ok ? ky : 0) * KS + (ok ? kx : 0);
     cplx<T> v[NC][EPT];
 #pragma unroll
     for (int c = 0; c < NC; c++)
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     const int kxs = ok ? kx : 0;
     const int64_t cs_b = (int64_t)nzb * Py * KS;
     const int64_t zs_b = (int64_t)Py * KS;
-    cplx<T> *col = B + (int64_t)ky * KS + kxs;
+    cplx<T> *col = B + (int64_t)blockIdx.z * 3 * cs_b + (int64_t)ky * KS + kxs;  // grid z: image
     cplx<T> v[3][EPT];
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -461,6 +461,71 @@ __global__ void k_batch_tick(Dyn *__restrict__ dyns, int M)
 // image-major, each as three planes of N values.  Grids are (blocks per
 // image, images); per-image block partials go to partial[(q*gridDim.x+b)*S+v].
 // ---------------------------------------------------------------------------
+// effective_field_energy(neb, spin) for meshes without demag (neb.jl:220-230):
+// the field and energy of every free image in one launch.  Image q reads
+// m + q*3N and writes its field to out + q*3N; each block leaves the sum of its
+// cells' energies, each rounded to T as the per-cell energy array holds it, in
+// partial[q*gridDim.x + b].
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_neb_field(const FieldParams<T> *__restrict__ Pp, Dyn dyn, uint32_t mask,
+                                                     const T *__restrict__ Y, T *__restrict__ F,
+                                                     double *__restrict__ partial)
+{
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
+    const unsigned nx = P.nx, ny = P.ny;
+    const int N = (int)P.N;
+    const int64_t off = (int64_t)blockIdx.y * 3 * N;
+    const T *m = Y + off;
+    T *out = F + off;
+    double r[1] = {0};
+    for (int I = blockIdx.x * BLOCK + threadIdx.x; I < N; I += gridDim.x * BLOCK) {
+        const unsigned q = (unsigned)I / nx;
+        const int i = (int)((unsigned)I - q * nx);
+        const unsigned k = q / ny;
+        const int j = (int)(q - k * ny);
+        T Hx, Hy, Hz;
+        double es = 0;
+        cell_field(P, dyn, mask, I, i, j, (int)k, m, T(0), T(0), T(0), Hx, Hy, Hz, &es);
+        r[0] += (double)(T)es;
+        out[I] = Hx;
+        out[N + I] = Hy;
+        out[2 * N + I] = Hz;
+    }
+    block_reduce_store<1>(r, 1, false, partial + (size_t)blockIdx.y * gridDim.x, 1);
+}
+
+// per-segment sums: out[q] = sum_b partial[q*nb + b]
+__global__ void k_segment_sum(const double *__restrict__ partial, int nb, double *__restrict__ out)
+{
+    __shared__ double s[BLOCK];
+    const int q = blockIdx.x;
+    double a = 0;
+    for (int i = threadIdx.x; i < nb; i += BLOCK) a += partial[(size_t)q * nb + i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = s[0];
+}
+
+// max over every image and cell of |a - b| (relax(::NEB), compute_dm!), one launch
+template <class T>
+__global__ void k_neb_maxdiff(const T *__restrict__ a, const T *__restrict__ b, int64_t N,
+                              double *__restrict__ partial)
+{
+    const int64_t off = (int64_t)blockIdx.y * 3 * N;
+    a += off;
+    b += off;
+    double red[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLOCK) {
+        T dx = a[i] - b[i], dy = a[N + i] - b[N + i], dz = a[2 * N + i] - b[2 * N + i];
+        red[0] = pmax(red[0], (double)sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    block_reduce_store<1>(red, 1, true, partial + (size_t)blockIdx.y * gridDim.x, 1);
+}
+
 // sum of n values (per-cell energies -> image energy)
 template <class T>
 __global__ void k_sum(const T *__restrict__ v, int64_t n, double *__restrict__ partial)
@@ -1107,16 +1172,16 @@ int convz_grouped_tensor(const DemagGeom &G, int elsize)
 
 template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in, int ncomp,
-                  int fwd_only, const cplx<T> *tw, hipStream_t s)
+                  int fwd_only, const cplx<T> *tw, hipStream_t s, int nimg)
 {
     (void)ncomp;
 #define CZ_CALL(L)                                                                                   \
     {                                                                                                \
-        dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py / 2 + 1);                           \
+        dim3 grid((G.Kx + Shape<L>::TZ - 1) / Shape<L>::TZ, G.Py / 2 + 1, nimg);                     \
         if constexpr (FftShape<L, P3W<T>::E>::TPS <= 64) {                                           \
             if (!fwd_only && G.ktw) {                                                                \
                 constexpr int TW = 64 / FftShape<L, P3W<T>::E>::TPS, WX = P3W<T>::template wx<L>();  \
-                dim3 gw((G.Kx + WX * TW - 1) / (WX * TW), WX == 2 ? G.Py / 2 + 1 : G.Py);           \
+                dim3 gw((G.Kx + WX * TW - 1) / (WX * TW), WX == 2 ? G.Py / 2 + 1 : G.Py, nimg);     \
                 hipLaunchKernelGGL((k_convw<T, L>), gw, dim3(256), 0, s, G, B, Kt, Lline, n_in, nzb, tw); \
                 break;                                                                               \
             }                                                                                        \
@@ -1157,6 +1222,38 @@ int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, con
     MMX_LG_SWITCH(G.lgH, P5_CALL)
 #undef P5_CALL
     return g;
+}
+
+template <class T>
+void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn,
+                      const Epi<T> &E, const T *m, int nimg, const cplx<T> *tw, hipStream_t s)
+{
+#define P5I_CALL(H)                                                                                  \
+    {                                                                                                \
+        const int g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;               \
+        hipLaunchKernelGGL((k_p5<T, H, 0, 0u, true>), dim3(g, nimg), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, \
+                           tw);                                                                      \
+    }
+    MMX_LG_SWITCH(G.lgH, P5I_CALL)
+#undef P5I_CALL
+}
+
+// sums of n stacked vectors of N values: partial[q*gridDim.x + b], then out[q]
+template <class T>
+__global__ void k_sum_images(const T *__restrict__ v, int64_t N, double *__restrict__ partial)
+{
+    const T *x = v + (int64_t)blockIdx.y * N;
+    double r[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLOCK)
+        r[0] += (double)x[i];
+    block_reduce_store<1>(r, 1, false, partial + (size_t)blockIdx.y * gridDim.x, 1);
+}
+
+template <class T>
+void launch_sum_images(const T *v, int64_t N, int n, int nb, double *partial, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_images<T>, dim3(nb, n), dim3(BLOCK), 0, s, v, N, partial);
+    hipLaunchKernelGGL(k_segment_sum, dim3(n), dim3(BLOCK), 0, s, partial, nb, out);
 }
 
 int p5_blocks(const DemagGeom &G)
@@ -1265,6 +1362,21 @@ void launch_neb_dist(int64_t N, int n, const T *Y, const T *L, const T *R, int n
 }
 
 template <class T>
+void launch_neb_field(const FieldParams<T> *P, const Dyn &dyn, uint32_t mask, int64_t N, int n, int nb, const T *Y,
+                      T *F, double *partial, double *energy, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neb_field<T>, dim3(nb, n), dim3(BLOCK), 0, s, P, dyn, mask, Y, F, partial);
+    hipLaunchKernelGGL(k_segment_sum, dim3(n), dim3(BLOCK), 0, s, partial, nb, energy);
+}
+
+template <class T>
+void launch_neb_maxdiff(const T *a, const T *b, int64_t N, int n, int nb, double *partial, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neb_maxdiff<T>, dim3(nb, n), dim3(BLOCK), 0, s, a, b, N, partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, partial, nb * n, 1, 1, 1, out);
+}
+
+template <class T>
 void launch_neb_tangent(int64_t N, int n, const T *Ys, const T *L, const T *R, const double *energy, const T *F,
                         T *tan, int nb, double *partial, const double *dist, double k, int climb, double *coef,
                         hipStream_t s)
@@ -1330,7 +1442,10 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
     template void launch_coly<T>(const DemagGeom &, const cplx<T> *, cplx<T> *, int, int, int, int, int, int,  \
                                  int, const cplx<T> *, hipStream_t);                                         \
     template void launch_convz<T>(const DemagGeom &, cplx<T> *, const T *, const T *, int, int, int, int,      \
-                                  const cplx<T> *, hipStream_t);                                                              \
+                                  const cplx<T> *, hipStream_t, int);                                                         \
+    template void launch_p5_images<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,  \
+                                      const Epi<T> &, const T *, int, const cplx<T> *, hipStream_t);          \
+    template void launch_sum_images<T>(const T *, int64_t, int, int, double *, double *, hipStream_t);        \
     template int launch_p5<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,        \
                               const Epi<T> &, const T *, int, uint32_t, const cplx<T> *, hipStream_t);                 \
     template void launch_tensor_extract<T>(const DemagGeom &, const cplx<double> *, int, double, T *,         \
@@ -1348,6 +1463,9 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
     template int launch_sum<T>(const T *, int64_t, double *, hipStream_t);                          \
     template void launch_neb_dist<T>(int64_t, int, const T *, const T *, const T *, int, double *, double *,  \
                                      hipStream_t);                                                          \
+    template void launch_neb_field<T>(const FieldParams<T> *, const Dyn &, uint32_t, int64_t, int, int, const T *, \
+                                      T *, double *, double *, hipStream_t);                                \
+    template void launch_neb_maxdiff<T>(const T *, const T *, int64_t, int, int, double *, double *, hipStream_t); \
     template void launch_neb_tangent<T>(int64_t, int, const T *, const T *, const T *, const double *,        \
                                         const T *, T *, int, double *, const double *, double, int, double *, \
                                         hipStream_t);                                                       \

@@ -110,11 +110,20 @@ struct RowWave {
 #define MMX_P5_WPE 3
 #endif
 #define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu(MMX_P5_WPE)))
-template <class T, int H, int V, uint32_t SIG = 0>
+// IMG: the launch covers a stack of images (grid y = image), each with its own
+// spectra, spin and field planes; fields and per-cell energies only.
+template <class T, int H, int V, uint32_t SIG = 0, bool IMG = false>
 __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
                                                       const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
                                                       const T *__restrict__ m, const cplx<T> *__restrict__ tw)
 {
+    if constexpr (IMG) {
+        const int64_t q = blockIdx.y, Nc = (int64_t)G.nx * G.ny * G.nz;
+        A += q * 3 * (int64_t)G.ny * G.nz * G.KS;
+        m += q * 3 * Nc;
+        E.out += q * 3 * Nc;
+        if (E.e_out) E.e_out += q * Nc;
+    }
     using S = Shape<H>;
     constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
     constexpr int HS = FftShape<H>::XS + 1;

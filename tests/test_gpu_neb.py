@@ -26,6 +26,8 @@ def _skyrmion(nx, ny, nz, d, R=5e-9):
 def _setup(case):
     if case == "film":
         nx, ny, nz, d, pbc = 12, 12, 1, 2e-9, "xy"
+    elif case == "demag-ragged":  # rows and planes that fill no tile exactly
+        nx, ny, nz, d, pbc = 12, 10, 3, 2e-9, "x"
     else:
         nx, ny, nz, d, pbc = 8, 8, 2, 2e-9, ""
     N = nx * ny * nz
@@ -44,7 +46,7 @@ def _setup(case):
     mmx.add_exch(sim, 3.25e-12)
     mmx.add_dmi(sim, 5.83e-4, type="interfacial")
     mmx.add_zeeman(sim, (0, 0, 0.12 / MU0))
-    if case == "demag":
+    if case.startswith("demag"):
         o.add_demag()
         mmx.add_demag(sim)
         o.set_pins(pins)
@@ -53,7 +55,7 @@ def _setup(case):
     return o, sim, [left, (0.0, 0.0, 1.0)], pins
 
 
-@pytest.mark.parametrize("case,clib", [("film", -1), ("demag", -1), ("film", 3)])
+@pytest.mark.parametrize("case,clib", [("film", -1), ("demag", -1), ("film", 3), ("demag-ragged", 2)])
 def test_neb_matches_oracle(case, clib):
     o, sim, given, pins = _setup(case)
     right = np.tile([0.0, 0.0, 1.0], o.N)
