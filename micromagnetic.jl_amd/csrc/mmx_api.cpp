@@ -1254,6 +1254,7 @@ struct NebEngine : CtxBase {
     // field and energy of one image (effective_field_energy(sim, m), field.jl:538-547)
     void field_energy(const T *m, T *out, double *E)
     {
+        e.t_eval = 0.0;
         Epi<T> x = e.epi(EPI_FIELD);
         x.out = out ? out : Ytmp.p;  // the field of an end image is not kept
         x.e_out = ecell.p;
@@ -1279,6 +1280,7 @@ struct NebEngine : CtxBase {
     void refresh()
     {
         if (fresh) return;
+        e.t_eval = 0.0;  // effective_field_energy(sim, m, 0.0), neb.jl:227
         if (!e.demag) {  // every image in one launch
             launch_neb_field<T>(e.params_dev(), e.dyn(), e.all_mask(), N, n, nb, Y.p, F.p, part.p, energy.p + 1,
                                 st());
