@@ -118,6 +118,10 @@ class Term:
 
     def __init__(self, sim, index, name):
         self.sim, self.index, self.name = sim, index, name
+        # table columns the reference's add_* call registers for this term:
+        # (names, units, fn(sim, saver)) before and after its E_<name> column
+        self.items_before, self.items_after = [], []
+        self.energy_unit = "<J>"  # None: the term registers no energy column
 
     @property
     def field(self):
@@ -340,9 +344,16 @@ def add_zeeman(sim, H0, ft=None, name=None):
         return _add_time_zeeman(sim, H0, ft, name or "timezeeman")
     name = name or "zeeman"
     if isinstance(H0, tuple) and len(H0) == 3:
-        return sim._add(L.mmx_add_zeeman(sim._h, None, *map(float, H0)), name)
-    H = np.ascontiguousarray(init_vector(sim.mesh, H0))
-    return sim._add(L.mmx_add_zeeman(sim._h, dptr(H), 0.0, 0.0, 0.0), name)
+        t = sim._add(L.mmx_add_zeeman(sim._h, None, *map(float, H0)), name)
+        t.H0 = tuple(map(float, H0))
+    else:
+        H = np.ascontiguousarray(init_vector(sim.mesh, H0))
+        t = sim._add(L.mmx_add_zeeman(sim._h, dptr(H), 0.0, 0.0, 0.0), name)
+        t.H0 = (0.0, 0.0, 0.0)
+    # add_field.jl:28-41: Hx, Hy, Hz (prefixed unless the name is "zeeman") = H0
+    cols = ("Hx", "Hy", "Hz") if name == "zeeman" else tuple(f"{name}_H{c}" for c in "xyz")
+    t.items_before.append((cols, ("<A/m>",) * 3, lambda o, sv, t=t: t.H0))
+    return t
 
 
 def update_zeeman(sim, H0, name="zeeman"):
@@ -351,6 +362,7 @@ def update_zeeman(sim, H0, name="zeeman"):
         if t.name == name:
             if isinstance(H0, tuple) and len(H0) == 3:
                 check(L.mmx_update_zeeman(sim._h, t.index, None, *map(float, H0)))
+                t.H0 = tuple(map(float, H0))  # add_field.jl:72-76
             else:
                 H = np.ascontiguousarray(init_vector(sim.mesh, H0))
                 check(L.mmx_update_zeeman(sim._h, t.index, dptr(H), 0.0, 0.0, 0.0))
@@ -367,9 +379,15 @@ def add_demag(sim, name="demag", Nx=0, Ny=0, Nz=0, fft=True):
 def add_thermal_noise(sim, Temp, name="thermal", T0=0.0, seed=0):
     L = _lib.load()
     if np.ndim(Temp) == 0 and not callable(Temp):
-        return sim._add(L.mmx_add_thermal_noise(sim._h, None, float(Temp), float(T0), int(seed)), name)
-    a = init_scalar(sim.mesh, Temp)
-    return sim._add(L.mmx_add_thermal_noise(sim._h, dptr(a), 0.0, float(T0), int(seed)), name)
+        t = sim._add(L.mmx_add_thermal_noise(sim._h, None, float(Temp), float(T0), int(seed)), name)
+        Tm = float(Temp)
+    else:
+        a = init_scalar(sim.mesh, Temp)
+        t = sim._add(L.mmx_add_thermal_noise(sim._h, dptr(a), 0.0, float(T0), int(seed)), name)
+        Tm = float(np.mean(a))
+    # add_field.jl:763-766: T_<name> = mean temperature (scaling factor 1)
+    t.items_after.append((f"T_{name}", "<K>", lambda o, sv: Tm))
+    return t
 
 
 _TIMEFN = None
@@ -403,6 +421,16 @@ def _add_time_zeeman(sim, H0, ft, name):
         idx = L.mmx_add_zeeman_time(sim._h, dptr(H), 0.0, 0.0, 0.0, C.cast(cb, C.c_void_p), None)
     t = sim._add(idx, name)
     t._ft = cb  # the library keeps only the pointer
+    t.energy_unit = "J"  # add_field.jl:192-194
+    if isinstance(H0, tuple) and len(H0) == 3:  # add_field.jl:182-190: H0 .* f(t)
+        H0f = tuple(map(float, H0))
+
+        def cols(o, sv):
+            v = ft(sv.t)
+            f = (v, v, v) if n == 1 else v
+            return tuple(h * float(x) for h, x in zip(H0f, f))
+
+        t.items_before.append((tuple(f"{name}_H{c}" for c in "xyz"), ("<A/m>",) * 3, cols))
     return t
 
 
@@ -437,8 +465,9 @@ def add_sot(sim, aj, bj, p, name="sot"):
     L = _lib.load()
     uniform = np.ndim(aj) == 0 and not callable(aj)
     a = None if uniform else init_scalar(sim.mesh, aj)
-    return sim._add(L.mmx_add_sot(sim._h, dptr(a), float(aj) if uniform else 0.0, float(bj),
-                                  *map(float, p)), name)
+    t = sim._add(L.mmx_add_sot(sim._h, dptr(a), float(aj) if uniform else 0.0, float(bj), *map(float, p)), name)
+    t.energy_unit = None  # the torques register no table column
+    return t
 
 
 def add_stt(sim, model="slonczewski", **params):
@@ -474,6 +503,7 @@ def _add_zhang_li(sim, name, params):
         Ja, J3 = np.ascontiguousarray(init_vector(sim.mesh, J)), (0.0, 0.0, 0.0)
     t = sim._add(L.mmx_add_stt_zhangli(sim._h, dptr(Ja), *J3, float(b), dptr(xa),
                                        float(xi) if xi_uniform else 0.0, int(has_P and not has_b)), name)
+    t.energy_unit = None
     if "ft" in params:
         _set_time_fn(sim, t, params["ft"])
     return t
@@ -492,6 +522,7 @@ def _add_slonczewski(sim, name, params):
     Ja = None if uniform else init_scalar(sim.mesh, J)
     t = sim._add(L.mmx_add_stt_slonczewski(sim._h, dptr(Ja), float(J) if uniform else 0.0, beta, float(Lambda),
                                            float(xi), float(params["P"]), *map(float, params["p"])), name)
+    t.energy_unit = None
     if "ft" in params:
         _set_time_fn(sim, t, params["ft"])
     return t

@@ -193,8 +193,9 @@ class SaverItem:
 
 class DataSaver:
     """DataSaver + write_data: src/fileio.jl:39-104.  Default items as
-    create_saver + the per-term energies the add_* calls register:
-    step, time, E_total, m_x, m_y, m_z, E_<term>..."""
+    create_saver + the columns the add_* calls register, in insertion order:
+    step, time, E_total, m_x, m_y, m_z, then per term its field columns
+    (Zeeman), E_<term> (not for the torques) and T_<term> (thermal)."""
 
     def __init__(self, name, items=None):
         self.name = name
@@ -210,9 +211,9 @@ class DataSaver:
         s = cls(name or f"{getattr(sim, 'name', 'dyn')}_llg.txt")
         state = {}
 
-        def energies(o):
+        def energies(o):  # compute_system_energy(sim, sim.spin, t_end), sim.jl:555,569
             if state.get("at") != s.nsteps:
-                state["E"] = compute_system_energy(o)
+                state["E"] = compute_system_energy(o, t=s.t)
                 state["at"] = s.nsteps
             return state["E"]
 
@@ -222,7 +223,13 @@ class DataSaver:
         s.items.append(SaverItem(("m_x", "m_y", "m_z"), ("<unitless>", "<unitless>", "<unitless>"),
                                  lambda o: tuple(float(v) for v in average_m(o))))
         for q, term in enumerate(sim.interactions):
-            s.items.append(SaverItem(f"E_{term.name}", "<J>", lambda o, q=q: float(energies(o)[q])))
+            for names, units, fn in getattr(term, "items_before", ()):
+                s.items.append(SaverItem(names, units, lambda o, fn=fn: fn(o, s)))
+            unit = getattr(term, "energy_unit", "<J>")
+            if unit is not None:
+                s.items.append(SaverItem(f"E_{term.name}", unit, lambda o, q=q: float(energies(o)[q])))
+            for names, units, fn in getattr(term, "items_after", ()):
+                s.items.append(SaverItem(names, units, lambda o, fn=fn: fn(o, s)))
         return s
 
     def write(self, sim):

@@ -177,3 +177,40 @@ def test_batch_refuses_time_zeeman():
     mmx.add_zeeman(s, (0, 0, 1e5), lambda t: 1.0)
     with pytest.raises(mmx.MmxError):
         mmx.Batch([s])
+
+
+def test_table_columns_of_zeeman_terms(tmp_path):
+    """The columns add_zeeman registers (add_field.jl:28-41, 182-194): Hx, Hy, Hz
+    of the term named "zeeman", <name>_Hx.. = H0 .* f(t) of a time Zeeman term,
+    its E_<name> in "J"; no column for a torque; energies at the row's time."""
+    note("table columns of Zeeman terms")
+    import os
+
+    from mmx.io import read_table
+
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=6, ny=4, nz=1)
+    sim = mmx.Sim(mesh, name=str(tmp_path / "zt"))
+    mmx.set_Ms(sim, 8e5)
+    mmx.init_m0(sim, (1, 1, 1))
+    mmx.add_exch(sim, 1e-11)
+    mmx.add_zeeman(sim, (1e3, 0, 2e3))
+    mmx.add_zeeman(sim, (0, 0, 5e4), time_fun, name="tz")
+    mmx.add_sot(sim, 1e9, 0.0, (0, 1, 0))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        for t in (1e-13, 2e-13):
+            mmx.run_until(sim, t, save_data=True)
+        E = mmx.compute_system_energy(sim, t=2e-13)
+        data, units = read_table(sim.saver.name)
+    finally:
+        os.chdir(cwd)
+    cols = list(data)
+    assert cols == ["step", "time", "E_total", "m_x", "m_y", "m_z", "E_exch", "Hx", "Hy", "Hz", "E_zeeman",
+                    "tz_Hx", "tz_Hy", "tz_Hz", "E_tz"]
+    assert units["E_tz"] == "J" and units["Hx"] == "A/m"
+    assert data["Hx"][-1] == 1e3 and data["Hz"][-1] == 2e3
+    f = time_fun(2e-13)
+    assert np.isclose(data["tz_Hz"][-1], 5e4 * f[2], rtol=1e-12, atol=0)
+    assert np.isclose(data["E_tz"][-1], E[2], rtol=1e-11, atol=0)
+    assert np.isclose(data["E_total"][-1], E.sum(), rtol=1e-11, atol=0)
