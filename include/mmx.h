@@ -115,7 +115,9 @@ int mmx_add_stt_slonczewski(mmx_ctx *ctx, const double *J, double J0, double bet
  * with xi0); xi_factor = 1 applies 1/(1 + xi^2) as the (P, Ms) form does. */
 int mmx_add_stt_zhangli(mmx_ctx *ctx, const double *J, double Jx, double Jy, double Jz, double b,
                         const double *xi, double xi0, int xi_factor);
-/* The ft / ufun time profile of a Slonczewski or Zhang-Li term (field.jl:335,351):
+/* The ft / ufun time profile of a Slonczewski or Zhang-Li term (field.jl:335,351),
+ * or time_D of a bulk DMI term, which makes it add_dmi(sim, D, ft) (add_field.jl:360-393,
+ * TimeBulkDMI / TimeSpatialBulkDMI field.jl:145-193: D scaled by time_D(t)):
  * called on the host at every evaluation time (stage times inside the steppers);
  * fn = NULL restores the constant 1. */
 int mmx_set_term_time_fn(mmx_ctx *ctx, int term, double (*fn)(double t, void *user), void *user);

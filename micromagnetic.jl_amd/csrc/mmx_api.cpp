@@ -287,6 +287,7 @@ struct Engine : CtxBase {
             TermDev<T> &d = P.t[i];
             d.kind = h.kind;
             d.uniform = h.uniform;
+            d.tdep = h.ufun != nullptr && (h.kind == K_DMI_BULK || h.kind == K_DMI_BULK_S);
             for (int q = 0; q < 12; q++) d.s[q] = (T)h.s[q];
             d.a = tA[i];
             d.b = tB[i];
@@ -304,7 +305,7 @@ struct Engine : CtxBase {
         P.fast = 1;
         for (int i = 0; i < P.nterms; i++) {
             int sk;
-            switch (C->terms[i].kind) {
+            switch (C->terms[i].ufun ? K_NONE : C->terms[i].kind) {  // time-dependent: generic
             case K_EXCH: case K_EXCH_S: sk = SL_EXCH; break;
             case K_DMI_BULK: case K_DMI_BULK_S: sk = SL_DMIB; break;
             case K_DMI_INT: sk = SL_DMII; break;
@@ -391,6 +392,12 @@ struct Engine : CtxBase {
             const double u = h.ufun ? h.ufun(t_eval, h.uuser) : 1.0;
             if (h.kind == K_STT_SL) d.tfac[i] = u * h.s[7];    // ft = ufun(t)*beta, field.jl:351
             if (h.kind == K_STT_ZL) d.tfac[i] = u / C->gamma;  // ut = ufun(t)/gamma, field.jl:335
+            if (h.kind == K_DMI_BULK && h.ufun) {              // Ds = (time_D(t) D)/d, field.jl:150-152
+                d.tfac[i] = (double)((T)(u * h.s[6]) / (T)h.s[9]);
+                d.tfy[i] = (double)((T)(u * h.s[7]) / (T)h.s[10]);
+                d.tfz[i] = (double)((T)(u * h.s[8]) / (T)h.s[11]);
+            }
+            if (h.kind == K_DMI_BULK_S && h.ufun) d.tfac[i] = u;  // time_D(t) .* D, field.jl:185
             if (h.kind == K_ZEEMAN_T) {                         // time_fun(t), field.jl:16-26
                 double f[3] = {1, 1, 1};
                 if (h.ft3) h.ft3(t_eval, f, h.uuser);
@@ -1048,7 +1055,7 @@ struct Engine : CtxBase {
         if (alphas.p) return fail(MMX_ESTATE, "the adjoint takes a uniform alpha");
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
-            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T)
+            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T || h.ufun)
                 return fail(MMX_EINVAL, "the adjoint covers time-independent field terms without spin torques");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
         const int64_t nck = (nsteps + every - 1) / every;
@@ -1912,6 +1919,13 @@ int mmx_add_dmi(mmx_ctx *c, int type, const double *Dx, const double *Dy, const 
             h.s[0] = (double)((TT)Dx0 / dx);
             h.s[1] = (double)((TT)Dy0 / dy);
             h.s[2] = (double)((TT)Dz0 / dz);
+            // D and d as stored, for a time profile (TimeBulkDMI)
+            h.s[6] = (double)(TT)Dx0;
+            h.s[7] = (double)(TT)Dy0;
+            h.s[8] = (double)(TT)Dz0;
+            h.s[9] = (double)dx;
+            h.s[10] = (double)dy;
+            h.s[11] = (double)dz;
             return add_term(c, e, h, nullptr, nullptr, nullptr);
         }
         if (!Dx || !Dy || !Dz) return fail(MMX_EINVAL, "spatial bulk DMI needs all three arrays");
@@ -2194,7 +2208,7 @@ int mmx_set_term_time_fn(mmx_ctx *c, int term, double (*fn)(double t, void *user
     if (!c) return fail(MMX_EINVAL, "null context");
     if (term < 0 || term >= (int)c->terms.size()) return fail(MMX_EINVAL, "no term %d", term);
     HostTerm &h = c->terms[term];
-    if (h.kind != K_STT_SL && h.kind != K_STT_ZL)
+    if (h.kind != K_STT_SL && h.kind != K_STT_ZL && h.kind != K_DMI_BULK && h.kind != K_DMI_BULK_S)
         return fail(MMX_EINVAL, "term %d has no time-dependent amplitude", term);
     h.ufun = fn;
     h.uuser = user;

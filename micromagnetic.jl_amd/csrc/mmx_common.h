@@ -54,6 +54,7 @@ template <class T>
 struct TermDev {
     int kind;
     int uniform;     // parameter given as a scalar instead of an array
+    int tdep;        // bulk DMI scaled by time_D(t): values in Dyn::tfac/tfy/tfz
     T s[12];
     const T *a, *b, *c;
 };

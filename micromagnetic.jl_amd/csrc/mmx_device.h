@@ -189,7 +189,10 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
         } break;
         case K_DMI_BULK: {  // bulkdmi_kernel!: kernels.jl:306-344, a_j x m_j expanded
             if (Ms == T(0)) break;
-            const T Dsx = D.s[0], Dsy = D.s[1], Dsz = D.s[2];
+            // TimeBulkDMI (field.jl:145-163): Ds = (time_D(t) Dx)/dx, set per evaluation
+            const T Dsx = D.tdep ? (T)dyn.tfac[tt] : D.s[0];
+            const T Dsy = D.tdep ? (T)dyn.tfy[tt] : D.s[1];
+            const T Dsz = D.tdep ? (T)dyn.tfz[tt] : D.s[2];
             if (ok[0]) { fy += Dsx * (-nmz[0]); fz += Dsx * nmy[0]; }
             if (ok[1]) { fy += Dsx * nmz[1];    fz += Dsx * (-nmy[1]); }
             if (ok[2]) { fx += Dsy * nmz[2];    fz += Dsy * (-nmx[2]); }
@@ -204,11 +207,13 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
         } break;
         case K_DMI_BULK_S: {  // spatial_vector_bulkdmi_kernel!: kernels.jl:347-422
             if (Ms == T(0)) break;
-            const T DxI = D.a[I], DyI = D.b[I], DzI = D.c[I];
+            // TimeSpatialBulkDMI (field.jl:180-193): every D read is time_D(t) * D
+            const T ft = D.tdep ? (T)dyn.tfac[tt] : T(1);
+            const T DxI = D.a[I] * ft, DyI = D.b[I] * ft, DzI = D.c[I] * ft;
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 if (ok[q]) {
-                    T Dn = D.a[nb[q]];
+                    T Dn = D.a[nb[q]] * ft;
                     if (DxI != T(0) && Dn != T(0)) {
                         T De = T(2) * DxI * Dn / (DxI + Dn);
                         T ax = q == 0 ? D.s[0] : -D.s[0];
@@ -220,7 +225,7 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
 #pragma unroll
             for (int q = 2; q < 4; q++) {
                 if (ok[q]) {
-                    T Dn = D.b[nb[q]];
+                    T Dn = D.b[nb[q]] * ft;
                     if (DyI != T(0) && Dn != T(0)) {
                         T De = T(2) * DyI * Dn / (DyI + Dn);
                         T ay = q == 2 ? D.s[1] : -D.s[1];
@@ -232,7 +237,7 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
 #pragma unroll
             for (int q = 4; q < 6; q++) {
                 if (ok[q]) {
-                    T Dn = D.c[nb[q]];
+                    T Dn = D.c[nb[q]] * ft;
                     if (DzI != T(0) && Dn != T(0)) {
                         T De = T(2) * DzI * Dn / (DzI + Dn);
                         T az = q == 4 ? D.s[2] : -D.s[2];

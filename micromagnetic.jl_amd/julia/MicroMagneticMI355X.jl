@@ -100,6 +100,25 @@ function add_term!(h, t::MicroMagnetic.SpatialVectorBulkDMI)
           (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 0,
           host(t.Dx), host(t.Dy), host(t.Dz), 0, 0, 0) |> check
 end
+function add_term!(h, t::MicroMagnetic.TimeBulkDMI)
+    # D scaled by time_D(t) at every evaluation time: src/micro/field.jl:145-163
+    idx = ccall((:mmx_add_dmi, libmmx), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 0,
+                C_NULL, C_NULL, C_NULL, t.Dx, t.Dy, t.Dz)
+    check(idx)
+    set_time_fn!(h, idx, t.time_D)
+    return idx
+end
+function add_term!(h, t::MicroMagnetic.TimeSpatialBulkDMI)
+    # one per-cell D for all three components: src/micro/field.jl:180-193
+    D = host(t.D)
+    idx = ccall((:mmx_add_dmi, libmmx), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 0,
+                D, D, D, 0, 0, 0)
+    check(idx)
+    set_time_fn!(h, idx, t.time_D)
+    return idx
+end
 function add_term!(h, t::MicroMagnetic.InterfacialDMI)
     ccall((:mmx_add_dmi, libmmx), Cint,
           (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h, 1,

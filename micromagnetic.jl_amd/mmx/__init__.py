@@ -288,8 +288,27 @@ def add_exch(sim, A, name="exch"):
     return sim._add(L.mmx_add_exch(sim._h, dptr(a), 0.0, 0.0, 0.0), name)
 
 
-def add_dmi(sim, D, name="dmi", type="bulk"):
+def add_dmi(sim, D, ft=None, name=None, type="bulk"):
+    """add_dmi(sim, D; name="dmi", type): src/micro/add_field.jl:293-335; with a
+    time function add_dmi(sim, D, ft; name="time_dmi") (add_field.jl:360-393):
+    always bulk DMI, D (number, 3-tuple, or per cell for all three
+    components) scaled by ft(t) at every evaluation time."""
     L = _lib.load()
+    if ft is not None:
+        name = name or "time_dmi"
+        if np.ndim(D) == 0 and not callable(D):
+            idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, float(D), float(D), float(D))
+        elif isinstance(D, tuple) and len(D) == 3:
+            idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, *map(float, D))
+        else:
+            a = init_scalar(sim.mesh, D)
+            idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, dptr(a), dptr(a), dptr(a), 0.0, 0.0, 0.0)
+        t = sim._add(idx, name)
+        _set_time_fn(sim, t, ft)
+        # add_field.jl:384-388: time_D<name> = ft(integrator.t)
+        t.items_after.append((f"time_D{name}", "<J/m^2>", lambda o, sv: float(ft(sv.t))))
+        return t
+    name = name or "dmi"
     if type == "interfacial":
         if np.ndim(D) == 0 and not callable(D):
             return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_INTERFACIAL, None, None, None, float(D), 0.0, 0.0), name)

@@ -474,7 +474,9 @@ static void field_exch_spatial(orc_sim *s, term_t *t, const double *m)
 /* bulkdmi_kernel!: src/micro/kernels.jl:306-344 */
 static void field_dmi_bulk(orc_sim *s, term_t *t, const double *m)
 {
-    const double Dx = t->p[0], Dy = t->p[1], Dz = t->p[2];
+    /* TimeBulkDMI (field.jl:145-163): D = time_D(t) * D before the kernel */
+    const double f = t->ufun ? t->ufun(s->t_eval, t->uuser) : 1.0;
+    const double Dx = f * t->p[0], Dy = f * t->p[1], Dz = f * t->p[2];
     const double Ds[6] = {Dx / s->dx, Dx / s->dx, Dy / s->dy, Dy / s->dy, Dz / s->dz, Dz / s->dz};
     const double ax[6] = {1, -1, 0, 0, 0, 0};
     const double ay[6] = {0, 0, 1, -1, 0, 0};
@@ -512,6 +514,9 @@ static void field_dmi_bulk_spatial(orc_sim *s, term_t *t, const double *m)
 {
     const double axes[6] = {1 / s->dx, -1 / s->dx, 1 / s->dy, -1 / s->dy, 1 / s->dz, -1 / s->dz};
     const double *Ms = s->mu0_Ms, *Dxs = t->a, *Dys = t->b, *Dzs = t->c;
+    /* TimeSpatialBulkDMI (field.jl:180-193): the kernel gets time_D(t) .* D */
+    const double f = t->ufun ? t->ufun(s->t_eval, t->uuser) : 1.0;
+#define DSC(v) (t->ufun ? f * (v) : (v))
 #pragma omp parallel for schedule(static)
     for (int I = 0; I < s->N; I++) {
         size_t i = 3 * (size_t)I;
@@ -521,13 +526,13 @@ static void field_dmi_bulk_spatial(orc_sim *s, term_t *t, const double *m)
             h[i] = h[i + 1] = h[i + 2] = 0;
             continue;
         }
-        double Dx_I = Dxs[I], Dy_I = Dys[I], Dz_I = Dzs[I];
+        double Dx_I = DSC(Dxs[I]), Dy_I = DSC(Dys[I]), Dz_I = DSC(Dzs[I]);
         double fx = 0, fy = 0, fz = 0;
         const int32_t *g = s->ngbs + 6 * (size_t)I;
         for (int j = 0; j < 2; j++) {
             int32_t id = g[j];
             if (id > 0 && Ms[id - 1] > 0) {
-                double Dx_n = Dxs[id - 1];
+                double Dx_n = DSC(Dxs[id - 1]);
                 if (Dx_I != 0 && Dx_n != 0) {
                     size_t k = 3 * (size_t)(id - 1);
                     double D_eff = 2 * Dx_I * Dx_n / (Dx_I + Dx_n);
@@ -540,7 +545,7 @@ static void field_dmi_bulk_spatial(orc_sim *s, term_t *t, const double *m)
         for (int j = 2; j < 4; j++) {
             int32_t id = g[j];
             if (id > 0 && Ms[id - 1] > 0) {
-                double Dy_n = Dys[id - 1];
+                double Dy_n = DSC(Dys[id - 1]);
                 if (Dy_I != 0 && Dy_n != 0) {
                     size_t k = 3 * (size_t)(id - 1);
                     double D_eff = 2 * Dy_I * Dy_n / (Dy_I + Dy_n);
@@ -553,7 +558,7 @@ static void field_dmi_bulk_spatial(orc_sim *s, term_t *t, const double *m)
         for (int j = 4; j < 6; j++) {
             int32_t id = g[j];
             if (id > 0 && Ms[id - 1] > 0) {
-                double Dz_n = Dzs[id - 1];
+                double Dz_n = DSC(Dzs[id - 1]);
                 if (Dz_I != 0 && Dz_n != 0) {
                     size_t k = 3 * (size_t)(id - 1);
                     double D_eff = 2 * Dz_I * Dz_n / (Dz_I + Dz_n);
@@ -570,6 +575,7 @@ static void field_dmi_bulk_spatial(orc_sim *s, term_t *t, const double *m)
         h[i + 2] = fz * Ms_inv;
     }
 }
+#undef DSC
 
 /* interfacial_dmi_kernel!: src/micro/kernels.jl:426-466 */
 static void field_dmi_int(orc_sim *s, term_t *t, const double *m)

@@ -200,6 +200,18 @@ class OracleSim:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
 
+    def add_dmi_bulk_time(self, D, ft):
+        """add_dmi(sim, D, ft): src/micro/add_field.jl:360-393 (a number or 3-tuple:
+        TimeBulkDMI; per cell: TimeSpatialBulkDMI with D for all three components)."""
+        if np.ndim(D) == 0 or (isinstance(D, tuple) and len(D) == 3):
+            Dv = (float(D),) * 3 if np.ndim(D) == 0 else tuple(map(float, D))
+            k = lib().orc_add_dmi_bulk(self.h, *Dv)
+        else:
+            a = _f64(D, self.N)
+            k = lib().orc_add_dmi_bulk_spatial(self.h, _p(a), _p(a), _p(a))
+        self.set_term_time_fn(k, ft)
+        return k
+
     def add_anis(self, Ku, axis=(0, 0, 1)):
         Ku = _f64(Ku, self.N)
         if isinstance(axis, tuple):
@@ -255,7 +267,7 @@ class OracleSim:
                                          _p(_f64(xi, self.N)), int(bool(xi_factor)))
 
     def set_term_time_fn(self, term, fn):
-        """ufun(t) of a torque term; fn(t) -> float, or None for 1."""
+        """ufun(t) of a torque term or time_D(t) of a bulk DMI term; fn(t) -> float, or None for 1."""
         if fn is None:
             lib().orc_set_term_time_fn(self.h, term, None, None)
             return
