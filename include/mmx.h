@@ -87,6 +87,9 @@ int mmx_add_cubic_anis(mmx_ctx *ctx, const double *Kc, double Kc0, const double 
 int mmx_add_zeeman(mmx_ctx *ctx, const double *H, double Hx, double Hy, double Hz);
 /* update_zeeman(sim, H0): src/micro/add_field.jl:60-100 */
 int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double Hy, double Hz);
+/* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606, hexagonal_anisotropy_kernel!
+ * src/micro/kernels.jl:142-167 -- E = K1 (1-mz^2) + K2 (1-mz^2)^2 + K3 Re((mx + i my)^6). */
+int mmx_add_hex_anis(mmx_ctx *ctx, double K1, double K2, double K3);
 /* add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200, TimeZeeman field.jl:12-33,
  * time_zeeman_kernel! src/micro/kernels.jl:39-49 -- H = H0 .* (fx, fy, fz)(t); fn writes the
  * three factors (a scalar time_fun gives three equal ones) and runs on the host at every

@@ -1055,8 +1055,10 @@ struct Engine : CtxBase {
         if (alphas.p) return fail(MMX_ESTATE, "the adjoint takes a uniform alpha");
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
-            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T || h.ufun)
-                return fail(MMX_EINVAL, "the adjoint covers time-independent field terms without spin torques");
+            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T ||
+                h.kind == K_HEX || h.ufun)
+                return fail(MMX_EINVAL, "the adjoint covers the time-independent exchange, DMI, uniaxial and cubic "
+                                        "anisotropy, Zeeman and demag terms");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
         const int64_t nck = (nsteps + every - 1) / every;
         adj.recorded = false;
@@ -2177,6 +2179,21 @@ int mmx_add_stt_zhangli(mmx_ctx *c, const double *J, double Jx, double Jy, doubl
         }
         const TT *bb = e.new_array(bJ.data(), 3 * N, true);
         return add_term(c, e, h, xa, bb, nullptr);
+    });
+}
+
+int mmx_add_hex_anis(mmx_ctx *c, double K1, double K2, double K3)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // HexagonalAnisotropy(T(K1), T(K2), T(K3)): add_field.jl:590-606
+        h.kind = K_HEX;
+        h.uniform = 1;
+        h.s[0] = (double)(TT)K1;
+        h.s[1] = (double)(TT)K2;
+        h.s[2] = (double)(TT)K3;
+        return add_term(c, e, h, nullptr, nullptr, nullptr);
     });
 }
 

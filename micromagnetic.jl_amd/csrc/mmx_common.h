@@ -48,6 +48,7 @@ enum TermKind : int {
     K_STT_SL,       // Slonczewski torque, J (a) or uniform s[3]; p s[0..2], Lambda^2 s[4], P s[5], xi s[6]
     K_STT_ZL,       // Zhang-Li torque, b*J planes (b), xi (a) or uniform s[3]; dx, dy, dz s[0..2]
     K_ZEEMAN_T,     // time-dependent Zeeman: static planes (a,b,c) or uniform s[0..2], times ft(t)
+    K_HEX,          // hexagonal anisotropy, s = {K1, K2, K3}
 };
 
 template <class T>

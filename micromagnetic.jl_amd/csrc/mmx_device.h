@@ -330,6 +330,20 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             hz = (T)philox_normal(P.seed, dyn.noise_step, base + 2) * scale;
             e = (double)(-Ms * V * (mx * hx + my * hy + mz * hz));
         } break;
+        case K_HEX: {  // hexagonal_anisotropy_kernel!: kernels.jl:142-167
+            if (Ms == T(0)) break;
+            const T K1 = D.s[0], K2 = D.s[1], K3 = D.s[2];
+            const T Ms_inv = T(1) / Ms;
+            const T x2 = mx * mx, y2 = my * my, z2 = mz * mz;
+            const T x3 = x2 * mx, y3 = y2 * my, x4 = x2 * x2, y4 = y2 * y2;
+            const T c6 = T(-6) * K3 * Ms_inv;
+            hx = c6 * ((x4 * mx - T(10) * x3 * y2) + T(5) * mx * y4);
+            hy = c6 * ((T(-5) * x4 * my + T(10) * x2 * y3) - y4 * my);
+            const T u = T(1) - z2;
+            hz = T(2) * mz * Ms_inv * (K1 + T(2) * K2 * u);
+            const T p6 = ((x4 * x2 - T(15) * x4 * y2) + T(15) * x2 * y4) - y4 * y2;
+            e = (double)(((K1 * u + K2 * u * u) + K3 * p6) * V);
+        } break;
         case K_ZEEMAN_T: {  // time_zeeman_kernel!: kernels.jl:39-49, factors from field.jl:12-33
             const T fx = (T)dyn.tfac[tt], fy = (T)dyn.tfy[tt], fz = (T)dyn.tfz[tt];
             if (D.uniform) {

@@ -141,6 +141,10 @@ function add_term!(h, t::MicroMagnetic.CubicAnisotropy)
     ccall((:mmx_add_cubic_anis, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}, Ptr{Float64}),
           h, host(t.Kc), 0, a1, a2) |> check
 end
+function add_term!(h, t::MicroMagnetic.HexagonalAnisotropy)
+    ccall((:mmx_add_hex_anis, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble), h, t.K1, t.K2,
+          t.K3) |> check
+end
 function add_term!(h, t::MicroMagnetic.Zeeman)
     ccall((:mmx_add_zeeman, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Cdouble, Cdouble), h,
           host(t.field), 0, 0, 0) |> check

@@ -354,6 +354,11 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
+def add_hex_anis(sim, K1=0.0, K2=0.0, K3=0.0, name="hex"):
+    """add_hex_anis(sim; K1, K2, K3, name): src/micro/add_field.jl:590-606."""
+    return sim._add(_lib.load().mmx_add_hex_anis(sim._h, float(K1), float(K2), float(K3)), name)
+
+
 def add_zeeman(sim, H0, ft=None, name=None):
     """add_zeeman(sim, H0; name="zeeman"): src/micro/add_field.jl:11-45, and with a
     time function add_zeeman(sim, H0, ft; name="timezeeman"): add_field.jl:161-200;

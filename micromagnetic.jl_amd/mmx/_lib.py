@@ -58,6 +58,7 @@ SIGNATURES = {
     "mmx_add_stt_zhangli": [_vp, _dp] + [C.c_double] * 4 + [_dp, C.c_double, C.c_int],
     "mmx_set_term_time_fn": [_vp, C.c_int, C.c_void_p, C.c_void_p],
     "mmx_add_zeeman_time": [_vp, _dp, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p],
+    "mmx_add_hex_anis": [_vp, C.c_double, C.c_double, C.c_double],
     "mmx_set_alpha": [_vp, _dp],
     "mmx_num_terms": [_vp],
     "mmx_effective_field": [_vp, C.c_double, _dp],

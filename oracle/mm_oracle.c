@@ -40,7 +40,8 @@ enum {
     T_SOT,
     T_STT_SL,
     T_STT_ZL,
-    T_ZEEMAN_T
+    T_ZEEMAN_T,
+    T_HEX
 };
 
 typedef struct {
@@ -1343,6 +1344,43 @@ static void field_stt_zl(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606 */
+int orc_add_hex_anis(orc_sim *s, double K1, double K2, double K3)
+{
+    term_t *t = new_term(s, T_HEX);
+    t->p[0] = K1;
+    t->p[1] = K2;
+    t->p[2] = K3;
+    return s->nterms - 1;
+}
+
+/* hexagonal_anisotropy_kernel!: src/micro/kernels.jl:142-167.  The integer
+ * powers are products (mx^5 = mx^4 * mx); the reference's Float64 power
+ * differs from them in the last bits only. */
+static void field_hex(orc_sim *s, term_t *t, const double *m)
+{
+    const double K1 = t->p[0], K2 = t->p[1], K3 = t->p[2];
+    const double *Ms = s->mu0_Ms;
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        if (Ms[I] == 0) {
+            t->energy[I] = 0;
+            h[j] = h[j + 1] = h[j + 2] = 0;
+            continue;
+        }
+        const double Ms_inv = 1.0 / Ms[I];
+        const double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        const double x2 = mx * mx, y2 = my * my, x4 = x2 * x2, y4 = y2 * y2;
+        h[j] = -6 * K3 * Ms_inv * (x4 * mx - 10 * (x2 * mx) * y2 + 5 * mx * y4);
+        h[j + 1] = -6 * K3 * Ms_inv * (-5 * x4 * my + 10 * x2 * (y2 * my) - y4 * my);
+        h[j + 2] = 2 * mz * Ms_inv * (K1 + 2 * K2 * (1 - mz * mz));
+        const double u = 1 - mz * mz;
+        t->energy[I] = (K1 * u + K2 * u * u + K3 * (x4 * x2 - 15 * x4 * y2 + 15 * x2 * y4 - y4 * y2)) * s->volume;
+    }
+}
+
 /* add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200 (H0 3N) */
 int orc_add_zeeman_time(orc_sim *s, const double *H, void (*fn)(double, double *, void *), void *user)
 {
@@ -1401,6 +1439,7 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_STT_SL: field_stt_sl(s, t, m); break;
     case T_STT_ZL: field_stt_zl(s, t, m); break;
     case T_ZEEMAN_T: field_zeeman_time(s, t, m); break;
+    case T_HEX: field_hex(s, t, m); break;
     }
 }
 

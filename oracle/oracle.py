@@ -75,6 +75,7 @@ def lib():
             "orc_add_stt_zhangli": [C.c_void_p, _dp, C.c_double, _dp, C.c_int],
             "orc_set_term_time_fn": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
             "orc_add_zeeman_time": [C.c_void_p, _dp, C.c_void_p, C.c_void_p],
+            "orc_add_hex_anis": [C.c_void_p, C.c_double, C.c_double, C.c_double],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
             "orc_advance_step": [C.c_void_p],
@@ -199,6 +200,10 @@ class OracleSim:
         else:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
+
+    def add_hex_anis(self, K1=0.0, K2=0.0, K3=0.0):
+        """add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606."""
+        return lib().orc_add_hex_anis(self.h, float(K1), float(K2), float(K3))
 
     def add_dmi_bulk_time(self, D, ft):
         """add_dmi(sim, D, ft): src/micro/add_field.jl:360-393 (a number or 3-tuple:
