@@ -90,6 +90,13 @@ int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double
 /* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606, hexagonal_anisotropy_kernel!
  * src/micro/kernels.jl:142-167 -- E = K1 (1-mz^2) + K2 (1-mz^2)^2 + K3 Re((mx + i my)^6). */
 int mmx_add_hex_anis(mmx_ctx *ctx, double K1, double K2, double K3);
+/* add_exch_int(sim, J; k1, k2): src/micro/add_field.jl:789-816, interlayer_exch_kernel!
+ * src/micro/kernels.jl:496-536 -- J holds nx*ny values (one per column), layers 1-based,
+ * k2 = -1 for the top layer; H(k1) = J/(mu0Ms dz) m(k2) and the other way round. */
+int mmx_add_exch_int(mmx_ctx *ctx, const double *J, int k1, int k2);
+/* add_dmi_int(sim, D; k1, k2): src/micro/add_field.jl:835-860, interlayer_dmi_kernel!
+ * src/micro/kernels.jl:539-584 -- H(k1) = D x m(k2)/(mu0Ms dz), H(k2) = -D x m(k1)/(mu0Ms dz). */
+int mmx_add_dmi_int(mmx_ctx *ctx, double Dx, double Dy, double Dz, int k1, int k2);
 /* add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200, TimeZeeman field.jl:12-33,
  * time_zeeman_kernel! src/micro/kernels.jl:39-49 -- H = H0 .* (fx, fy, fz)(t); fn writes the
  * three factors (a scalar time_fun gives three equal ones) and runs on the host at every

@@ -1056,7 +1056,7 @@ struct Engine : CtxBase {
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
             if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T ||
-                h.kind == K_HEX || h.ufun)
+                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.ufun)
                 return fail(MMX_EINVAL, "the adjoint covers the time-independent exchange, DMI, uniaxial and cubic "
                                         "anisotropy, Zeeman and demag terms");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
@@ -2193,6 +2193,45 @@ int mmx_add_hex_anis(mmx_ctx *c, double K1, double K2, double K3)
         h.s[0] = (double)(TT)K1;
         h.s[1] = (double)(TT)K2;
         h.s[2] = (double)(TT)K3;
+        return add_term(c, e, h, nullptr, nullptr, nullptr);
+    });
+}
+
+int mmx_add_exch_int(mmx_ctx *c, const double *J, int k1, int k2)
+{
+    if (!c || !J) return fail(MMX_EINVAL, "null argument");
+    const int nz = c->mesh.nz;
+    if (k2 == -1) k2 = nz;  // add_field.jl:802-804
+    if (k1 < 1 || k1 > nz || k2 < 1 || k2 > nz) return fail(MMX_EINVAL, "layers k1=%d, k2=%d outside 1..%d", k1, k2, nz);
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // InterlayerExchange(Js, k1, k2): add_field.jl:789-816
+        h.kind = K_EXCH_IL;
+        h.s[3] = (double)(TT)c->mesh.dz;
+        h.s[4] = k1 - 1;
+        h.s[5] = k2 - 1;
+        const TT *a = e.new_array(J, (int64_t)c->mesh.nx * c->mesh.ny, false);
+        return add_term(c, e, h, a, nullptr, nullptr);
+    });
+}
+
+int mmx_add_dmi_int(mmx_ctx *c, double Dx, double Dy, double Dz, int k1, int k2)
+{
+    if (!c) return fail(MMX_EINVAL, "null context");
+    const int nz = c->mesh.nz;
+    if (k2 == -1) k2 = nz;  // add_field.jl:840-842
+    if (k1 < 1 || k1 > nz || k2 < 1 || k2 > nz) return fail(MMX_EINVAL, "layers k1=%d, k2=%d outside 1..%d", k1, k2, nz);
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // InterlayerDMI(T(Dx), T(Dy), T(Dz), k1, k2): add_field.jl:835-860
+        h.kind = K_DMI_IL;
+        h.uniform = 1;
+        h.s[0] = (double)(TT)Dx;
+        h.s[1] = (double)(TT)Dy;
+        h.s[2] = (double)(TT)Dz;
+        h.s[3] = (double)(TT)c->mesh.dz;
+        h.s[4] = k1 - 1;
+        h.s[5] = k2 - 1;
         return add_term(c, e, h, nullptr, nullptr, nullptr);
     });
 }

@@ -49,6 +49,8 @@ enum TermKind : int {
     K_STT_ZL,       // Zhang-Li torque, b*J planes (b), xi (a) or uniform s[3]; dx, dy, dz s[0..2]
     K_ZEEMAN_T,     // time-dependent Zeeman: static planes (a,b,c) or uniform s[0..2], times ft(t)
     K_HEX,          // hexagonal anisotropy, s = {K1, K2, K3}
+    K_EXCH_IL,      // interlayer exchange: J per (i,j) in a (nx*ny), s = {-, -, -, dz, k1, k2}
+    K_DMI_IL,       // interlayer DMI: s = {Dx, Dy, Dz, dz, k1, k2} (layers 0-based)
 };
 
 template <class T>

@@ -344,6 +344,31 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             const T p6 = ((x4 * x2 - T(15) * x4 * y2) + T(15) * x2 * y4) - y4 * y2;
             e = (double)(((K1 * u + K2 * u * u) + K3 * p6) * V);
         } break;
+        case K_EXCH_IL:
+        case K_DMI_IL: {  // interlayer_exch_kernel! / interlayer_dmi_kernel!: kernels.jl:496-584
+            const int k1 = (int)D.s[4], k2 = (int)D.s[5];
+            // the kernel writes layer k1's field, then layer k2's (the last wins if k1 == k2)
+            const bool top = k == k2, bot = !top && k == k1;
+            if (!top && !bot) break;
+            const int64_t Ip = I + (int64_t)((top ? k1 : k2) - k) * ((int64_t)P.nx * P.ny);
+            const T Msp = P.mu0Ms[Ip];
+            if (!(Ms > T(0) && Msp > T(0))) break;
+            const T px = m[Ip], py = m[N + Ip], pz = m[2 * N + Ip];
+            const T dz = D.s[3];
+            if (D.kind == K_EXCH_IL) {
+                const T c = D.a[i + (int64_t)j * P.nx] / (Ms * dz);
+                hx = c * px;
+                hy = c * py;
+                hz = c * pz;
+            } else {
+                const T c = (top ? T(-1) : T(1)) / (Ms * dz);
+                const T Dx = D.s[0], Dy = D.s[1], Dz = D.s[2];
+                hx = c * (-Dz * py + Dy * pz);
+                hy = c * (Dz * px - Dx * pz);
+                hz = c * (-Dy * px + Dx * py);
+            }
+            e = (double)(T(-0.5) * Ms * ((hx * mx + hy * my) + hz * mz) * V);
+        } break;
         case K_ZEEMAN_T: {  // time_zeeman_kernel!: kernels.jl:39-49, factors from field.jl:12-33
             const T fx = (T)dyn.tfac[tt], fy = (T)dyn.tfy[tt], fz = (T)dyn.tfz[tt];
             if (D.uniform) {

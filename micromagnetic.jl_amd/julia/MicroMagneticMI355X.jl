@@ -141,6 +141,14 @@ function add_term!(h, t::MicroMagnetic.CubicAnisotropy)
     ccall((:mmx_add_cubic_anis, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cdouble, Ptr{Float64}, Ptr{Float64}),
           h, host(t.Kc), 0, a1, a2) |> check
 end
+function add_term!(h, t::MicroMagnetic.InterlayerExchange)
+    ccall((:mmx_add_exch_int, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint), h, host(t.Js), t.k1,
+          t.k2) |> check
+end
+function add_term!(h, t::MicroMagnetic.InterlayerDMI)
+    ccall((:mmx_add_dmi_int, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cint, Cint), h, t.Dx, t.Dy,
+          t.Dz, t.k1, t.k2) |> check
+end
 function add_term!(h, t::MicroMagnetic.HexagonalAnisotropy)
     ccall((:mmx_add_hex_anis, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble), h, t.K1, t.K2,
           t.K3) |> check

@@ -354,6 +354,22 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
+def add_exch_int(sim, J, k1=1, k2=-1, name="exch_int"):
+    """add_exch_int(sim, J; k1, k2, name): src/micro/add_field.jl:789-816.  J is
+    expanded on the one-layer mesh (nx, ny, 1) as the reference does."""
+    m = sim.mesh
+    flat = FDMesh(dx=m.dx, dy=m.dy, dz=m.dz, nx=m.nx, ny=m.ny, nz=1)
+    Js = np.ascontiguousarray(init_scalar(flat, J))
+    return sim._add(_lib.load().mmx_add_exch_int(sim._h, dptr(Js), int(k1), int(k2)), name)
+
+
+def add_dmi_int(sim, D, k1=1, k2=-1, name="dmi_int"):
+    """add_dmi_int(sim, D; k1, k2, name): src/micro/add_field.jl:835-860 (D a 3-tuple)."""
+    if not (isinstance(D, tuple) and len(D) == 3):
+        raise ValueError("add_dmi_int takes D as a 3-tuple")
+    return sim._add(_lib.load().mmx_add_dmi_int(sim._h, *map(float, D), int(k1), int(k2)), name)
+
+
 def add_hex_anis(sim, K1=0.0, K2=0.0, K3=0.0, name="hex"):
     """add_hex_anis(sim; K1, K2, K3, name): src/micro/add_field.jl:590-606."""
     return sim._add(_lib.load().mmx_add_hex_anis(sim._h, float(K1), float(K2), float(K3)), name)

@@ -59,6 +59,8 @@ SIGNATURES = {
     "mmx_set_term_time_fn": [_vp, C.c_int, C.c_void_p, C.c_void_p],
     "mmx_add_zeeman_time": [_vp, _dp, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p],
     "mmx_add_hex_anis": [_vp, C.c_double, C.c_double, C.c_double],
+    "mmx_add_exch_int": [_vp, _dp, C.c_int, C.c_int],
+    "mmx_add_dmi_int": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
     "mmx_set_alpha": [_vp, _dp],
     "mmx_num_terms": [_vp],
     "mmx_effective_field": [_vp, C.c_double, _dp],

@@ -41,7 +41,9 @@ enum {
     T_STT_SL,
     T_STT_ZL,
     T_ZEEMAN_T,
-    T_HEX
+    T_HEX,
+    T_EXCH_IL,
+    T_DMI_IL
 };
 
 typedef struct {
@@ -1381,6 +1383,75 @@ static void field_hex(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_exch_int(sim, J; k1, k2): src/micro/add_field.jl:789-816 (J: nx*ny values, 1-based layers) */
+int orc_add_exch_int(orc_sim *s, const double *J, int k1, int k2)
+{
+    term_t *t = new_term(s, T_EXCH_IL);
+    t->a = dup_array(J, (size_t)s->nx * s->ny);
+    t->p[4] = k1;
+    t->p[5] = k2 == -1 ? s->nz : k2;
+    return s->nterms - 1;
+}
+
+/* add_dmi_int(sim, D; k1, k2): src/micro/add_field.jl:835-860 */
+int orc_add_dmi_int(orc_sim *s, double Dx, double Dy, double Dz, int k1, int k2)
+{
+    term_t *t = new_term(s, T_DMI_IL);
+    t->p[0] = Dx;
+    t->p[1] = Dy;
+    t->p[2] = Dz;
+    t->p[4] = k1;
+    t->p[5] = k2 == -1 ? s->nz : k2;
+    return s->nterms - 1;
+}
+
+/* interlayer_exch_kernel! / interlayer_dmi_kernel!: src/micro/kernels.jl:496-584, one
+ * (i, j) column at a time: layer k1 is written first, then layer k2; every other cell
+ * keeps a zero field. */
+static void field_interlayer(orc_sim *s, term_t *t, const double *m)
+{
+    const int nx = s->nx, ny = s->ny, K1 = (int)t->p[4], K2 = (int)t->p[5];
+    const double dz = s->dz, V = s->volume, *Ms = s->mu0_Ms;
+    const double Dx = t->p[0], Dy = t->p[1], Dz = t->p[2];
+    double *h = t->field;
+    memset(h, 0, 3 * (size_t)s->N * sizeof(double));
+    memset(t->energy, 0, (size_t)s->N * sizeof(double));
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const size_t id1 = (size_t)(K1 - 1) * nx * ny + (size_t)j * nx + i;
+            const size_t id2 = (size_t)(K2 - 1) * nx * ny + (size_t)j * nx + i;
+            const double mbx = m[3 * id1], mby = m[3 * id1 + 1], mbz = m[3 * id1 + 2];
+            const double mtx = m[3 * id2], mty = m[3 * id2 + 1], mtz = m[3 * id2 + 2];
+            const double Ms1 = Ms[id1], Ms2 = Ms[id2];
+            if (!(Ms1 > 0 && Ms2 > 0)) continue;
+            double *h1 = h + 3 * id1, *h2 = h + 3 * id2;
+            if (t->kind == T_EXCH_IL) {
+                const double J = t->a[(size_t)j * nx + i];
+                double c = J / (Ms1 * dz);
+                h1[0] = c * mtx;
+                h1[1] = c * mty;
+                h1[2] = c * mtz;
+                t->energy[id1] = -0.5 * Ms1 * (h1[0] * mbx + h1[1] * mby + h1[2] * mbz) * V;
+                c = J / (Ms2 * dz);
+                h2[0] = c * mbx;
+                h2[1] = c * mby;
+                h2[2] = c * mbz;
+                t->energy[id2] = -0.5 * Ms2 * (h2[0] * mtx + h2[1] * mty + h2[2] * mtz) * V;
+            } else {
+                double c = 1.0 / (Ms1 * dz);
+                h1[0] = c * cross_x(Dx, Dy, Dz, mtx, mty, mtz);
+                h1[1] = c * cross_y(Dx, Dy, Dz, mtx, mty, mtz);
+                h1[2] = c * cross_z(Dx, Dy, Dz, mtx, mty, mtz);
+                t->energy[id1] = -0.5 * Ms1 * (h1[0] * mbx + h1[1] * mby + h1[2] * mbz) * V;
+                c = -1.0 / (Ms2 * dz);
+                h2[0] = c * cross_x(Dx, Dy, Dz, mbx, mby, mbz);
+                h2[1] = c * cross_y(Dx, Dy, Dz, mbx, mby, mbz);
+                h2[2] = c * cross_z(Dx, Dy, Dz, mbx, mby, mbz);
+                t->energy[id2] = -0.5 * Ms2 * (h2[0] * mtx + h2[1] * mty + h2[2] * mtz) * V;
+            }
+        }
+}
+
 /* add_zeeman(sim, H0, ft): src/micro/add_field.jl:161-200 (H0 3N) */
 int orc_add_zeeman_time(orc_sim *s, const double *H, void (*fn)(double, double *, void *), void *user)
 {
@@ -1440,6 +1511,8 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_STT_ZL: field_stt_zl(s, t, m); break;
     case T_ZEEMAN_T: field_zeeman_time(s, t, m); break;
     case T_HEX: field_hex(s, t, m); break;
+    case T_EXCH_IL:
+    case T_DMI_IL: field_interlayer(s, t, m); break;
     }
 }
 

@@ -76,6 +76,8 @@ def lib():
             "orc_set_term_time_fn": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
             "orc_add_zeeman_time": [C.c_void_p, _dp, C.c_void_p, C.c_void_p],
             "orc_add_hex_anis": [C.c_void_p, C.c_double, C.c_double, C.c_double],
+            "orc_add_exch_int": [C.c_void_p, _dp, C.c_int, C.c_int],
+            "orc_add_dmi_int": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
             "orc_advance_step": [C.c_void_p],
@@ -200,6 +202,14 @@ class OracleSim:
         else:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
+
+    def add_exch_int(self, J, k1=1, k2=-1):
+        """add_exch_int(sim, J; k1, k2): src/micro/add_field.jl:789-816 (J: one value per column)."""
+        return lib().orc_add_exch_int(self.h, _p(_f64(J, self.nx * self.ny)), int(k1), int(k2))
+
+    def add_dmi_int(self, D, k1=1, k2=-1):
+        """add_dmi_int(sim, D; k1, k2): src/micro/add_field.jl:835-860."""
+        return lib().orc_add_dmi_int(self.h, *map(float, D), int(k1), int(k2))
 
     def add_hex_anis(self, K1=0.0, K2=0.0, K3=0.0):
         """add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606."""
