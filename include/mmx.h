@@ -90,6 +90,12 @@ int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double
 /* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606, hexagonal_anisotropy_kernel!
  * src/micro/kernels.jl:142-167 -- E = K1 (1-mz^2) + K2 (1-mz^2)^2 + K3 Re((mx + i my)^6). */
 int mmx_add_hex_anis(mmx_ctx *ctx, double K1, double K2, double K3);
+/* add_twin_mono_anis(sim; Ka, Kb, Kaa, Kbb, Kab, Ku, axis_a, axis_b, axis_u111):
+ * src/micro/add_field.jl:629-660, twin_monoclinic_anisotropy_kernel! src/micro/kernels.jl:169-225
+ * -- K = {Ka, Kb, Kaa, Kbb, Kab, Ku}; three per-cell axes, 3N values each (AoS),
+ * normalised here as the reference normalises them. */
+int mmx_add_twin_mono_anis(mmx_ctx *ctx, const double K[6], const double *axis_a, const double *axis_b,
+                           const double *axis_u111);
 /* add_exch_int(sim, J; k1, k2): src/micro/add_field.jl:789-816, interlayer_exch_kernel!
  * src/micro/kernels.jl:496-536 -- J holds nx*ny values (one per column), layers 1-based,
  * k2 = -1 for the top layer; H(k1) = J/(mu0Ms dz) m(k2) and the other way round. */

@@ -1056,7 +1056,7 @@ struct Engine : CtxBase {
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
             if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T ||
-                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.ufun)
+                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.ufun)
                 return fail(MMX_EINVAL, "the adjoint covers the time-independent exchange, DMI, uniaxial and cubic "
                                         "anisotropy, Zeeman and demag terms");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
@@ -2194,6 +2194,40 @@ int mmx_add_hex_anis(mmx_ctx *c, double K1, double K2, double K3)
         h.s[1] = (double)(TT)K2;
         h.s[2] = (double)(TT)K3;
         return add_term(c, e, h, nullptr, nullptr, nullptr);
+    });
+}
+
+int mmx_add_twin_mono_anis(mmx_ctx *c, const double K[6], const double *axis_a, const double *axis_b,
+                           const double *axis_u111)
+{
+    if (!c || !K || !axis_a || !axis_b || !axis_u111) return fail(MMX_EINVAL, "null argument");
+    const int64_t N = c->N;
+    // normalise(axes, n_total): src/helper.jl:255-271 (zero vectors stay zero)
+    std::vector<double> ax[3];
+    const double *in[3] = {axis_a, axis_b, axis_u111};
+    for (int q = 0; q < 3; q++) {
+        if (!finite_all(in[q], 3 * N)) return fail(MMX_EINVAL, "NaN in an axis");
+        ax[q].assign(in[q], in[q] + 3 * N);
+        for (int64_t i = 0; i < N; i++) {
+            double *v = ax[q].data() + 3 * i;
+            const double m2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (m2 > 0) {
+                const double f = 1.0 / std::sqrt(m2);
+                v[0] *= f;
+                v[1] *= f;
+                v[2] *= f;
+            }
+        }
+    }
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // TwinMonoclinicAnisotropy: add_field.jl:629-660
+        h.kind = K_TWIN;
+        for (int q = 0; q < 6; q++) h.s[q] = (double)(TT)K[q];
+        const TT *a = e.new_array(ax[0].data(), 3 * N, true);
+        const TT *b = e.new_array(ax[1].data(), 3 * N, true);
+        const TT *u = e.new_array(ax[2].data(), 3 * N, true);
+        return add_term(c, e, h, a, b, u);
     });
 }
 

@@ -344,6 +344,26 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             const T p6 = ((x4 * x2 - T(15) * x4 * y2) + T(15) * x2 * y4) - y4 * y2;
             e = (double)(((K1 * u + K2 * u * u) + K3 * p6) * V);
         } break;
+        case K_TWIN: {  // twin_monoclinic_anisotropy_kernel!: kernels.jl:169-225
+            if (Ms == T(0)) break;
+            const T ax = D.a[I], ay = D.a[N + I], az = D.a[2 * N + I];
+            const T bx = D.b[I], by = D.b[N + I], bz = D.b[2 * N + I];
+            const T ux = D.c[I], uy = D.c[N + I], uz = D.c[2 * N + I];
+            const T Ka = D.s[0], Kb = D.s[1], Kaa = D.s[2], Kbb = D.s[3], Kab = D.s[4], Ku = D.s[5];
+            const T pa = (mx * ax + my * ay) + mz * az;
+            const T pb = (mx * bx + my * by) + mz * bz;
+            const T pu = (mx * ux + my * uy) + mz * uz;
+            const T pa2 = pa * pa, pb2 = pb * pb;
+            const T dEa = (T(2) * Ka * pa + T(4) * Kaa * pa2 * pa) + T(2) * Kab * pa * pb2;
+            const T dEb = (T(2) * Kb * pb + T(4) * Kbb * pb2 * pb) + T(2) * Kab * pb * pa2;
+            const T dEu = T(-2) * Ku * pu;
+            const T nMs_inv = -(T(1) / Ms);
+            hx = nMs_inv * ((dEa * ax + dEb * bx) + dEu * ux);
+            hy = nMs_inv * ((dEa * ay + dEb * by) + dEu * uy);
+            hz = nMs_inv * ((dEa * az + dEb * bz) + dEu * uz);
+            e = (double)((((((Ka * pa2 + Kb * pb2) + Kaa * pa2 * pa2) + Kbb * pb2 * pb2) + Kab * pa2 * pb2) -
+                          Ku * pu * pu) * V);
+        } break;
         case K_EXCH_IL:
         case K_DMI_IL: {  // interlayer_exch_kernel! / interlayer_dmi_kernel!: kernels.jl:496-584
             const int k1 = (int)D.s[4], k2 = (int)D.s[5];

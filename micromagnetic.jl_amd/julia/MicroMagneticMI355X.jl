@@ -149,6 +149,12 @@ function add_term!(h, t::MicroMagnetic.InterlayerDMI)
     ccall((:mmx_add_dmi_int, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cint, Cint), h, t.Dx, t.Dy,
           t.Dz, t.k1, t.k2) |> check
 end
+function add_term!(h, t::MicroMagnetic.TwinMonoclinicAnisotropy)
+    K = Float64[t.Ka, t.Kb, t.Kaa, t.Kbb, t.Kab, t.Ku]
+    ccall((:mmx_add_twin_mono_anis, libmmx), Cint,
+          (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), h, K, host(t.axis_a),
+          host(t.axis_b), host(t.axis_u111)) |> check
+end
 function add_term!(h, t::MicroMagnetic.HexagonalAnisotropy)
     ccall((:mmx_add_hex_anis, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble), h, t.K1, t.K2,
           t.K3) |> check

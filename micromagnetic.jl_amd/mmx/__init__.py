@@ -354,6 +354,15 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
+def add_twin_mono_anis(sim, Ka=0, Kb=0, Kaa=0, Kbb=0, Kab=0, Ku=0, axis_a=(1, 0, 0), axis_b=(0, 1, 0),
+                       axis_u111=(0, 0, 1), name="twin_mono_anis"):
+    """add_twin_mono_anis(sim; Ka, Kb, Kaa, Kbb, Kab, Ku, axis_a, axis_b, axis_u111, name):
+    src/micro/add_field.jl:629-660; axes are tuples, arrays or functions (init_vector)."""
+    K = np.array([Ka, Kb, Kaa, Kbb, Kab, Ku], dtype=np.float64)
+    ax = [np.ascontiguousarray(init_vector(sim.mesh, v)) for v in (axis_a, axis_b, axis_u111)]
+    return sim._add(_lib.load().mmx_add_twin_mono_anis(sim._h, dptr(K), *(dptr(a) for a in ax)), name)
+
+
 def add_exch_int(sim, J, k1=1, k2=-1, name="exch_int"):
     """add_exch_int(sim, J; k1, k2, name): src/micro/add_field.jl:789-816.  J is
     expanded on the one-layer mesh (nx, ny, 1) as the reference does."""

@@ -43,7 +43,8 @@ enum {
     T_ZEEMAN_T,
     T_HEX,
     T_EXCH_IL,
-    T_DMI_IL
+    T_DMI_IL,
+    T_TWIN
 };
 
 typedef struct {
@@ -1383,6 +1384,62 @@ static void field_hex(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_twin_mono_anis: src/micro/add_field.jl:629-660; axes 3N AoS, normalised here
+ * (normalise, src/helper.jl:255-271); p = {Ka, Kb, Kaa, Kbb, Kab, Ku} */
+int orc_add_twin_mono_anis(orc_sim *s, const double K[6], const double *axis_a, const double *axis_b,
+                           const double *axis_u)
+{
+    term_t *t = new_term(s, T_TWIN);
+    for (int q = 0; q < 6; q++) t->p[q] = K[q];
+    double **dst[3] = {&t->a, &t->b, &t->c};
+    const double *src[3] = {axis_a, axis_b, axis_u};
+    for (int q = 0; q < 3; q++) {
+        double *v = dup_array(src[q], 3 * (size_t)s->N);
+        for (int i = 0; i < s->N; i++) {
+            double *w = v + 3 * (size_t)i;
+            double m2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+            if (m2 > 0) {
+                double f = 1.0 / sqrt(m2);
+                w[0] *= f;
+                w[1] *= f;
+                w[2] *= f;
+            }
+        }
+        *dst[q] = v;
+    }
+    return s->nterms - 1;
+}
+
+/* twin_monoclinic_anisotropy_kernel!: src/micro/kernels.jl:169-225 */
+static void field_twin(orc_sim *s, term_t *t, const double *m)
+{
+    const double Ka = t->p[0], Kb = t->p[1], Kaa = t->p[2], Kbb = t->p[3], Kab = t->p[4], Ku = t->p[5];
+    const double *Ms = s->mu0_Ms;
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        if (Ms[I] == 0) {
+            t->energy[I] = 0;
+            h[j] = h[j + 1] = h[j + 2] = 0;
+            continue;
+        }
+        const double *A = t->a + j, *B = t->b + j, *U = t->c + j;
+        const double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        const double al_a = mx * A[0] + my * A[1] + mz * A[2];
+        const double al_b = mx * B[0] + my * B[1] + mz * B[2];
+        const double al_u = mx * U[0] + my * U[1] + mz * U[2];
+        const double a2 = al_a * al_a, b2 = al_b * al_b;
+        const double dEa = 2 * Ka * al_a + 4 * Kaa * a2 * al_a + 2 * Kab * al_a * b2;
+        const double dEb = 2 * Kb * al_b + 4 * Kbb * b2 * al_b + 2 * Kab * al_b * a2;
+        const double dEu = -2 * Ku * al_u;
+        const double Ms_inv = 1 / Ms[I];
+        for (int c = 0; c < 3; c++) h[j + c] = -Ms_inv * (dEa * A[c] + dEb * B[c] + dEu * U[c]);
+        t->energy[I] = (Ka * a2 + Kb * b2 + Kaa * a2 * a2 + Kbb * b2 * b2 + Kab * a2 * b2 - Ku * al_u * al_u) *
+                       s->volume;
+    }
+}
+
 /* add_exch_int(sim, J; k1, k2): src/micro/add_field.jl:789-816 (J: nx*ny values, 1-based layers) */
 int orc_add_exch_int(orc_sim *s, const double *J, int k1, int k2)
 {
@@ -1513,6 +1570,7 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_HEX: field_hex(s, t, m); break;
     case T_EXCH_IL:
     case T_DMI_IL: field_interlayer(s, t, m); break;
+    case T_TWIN: field_twin(s, t, m); break;
     }
 }
 

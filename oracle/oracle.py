@@ -77,6 +77,7 @@ def lib():
             "orc_add_zeeman_time": [C.c_void_p, _dp, C.c_void_p, C.c_void_p],
             "orc_add_hex_anis": [C.c_void_p, C.c_double, C.c_double, C.c_double],
             "orc_add_exch_int": [C.c_void_p, _dp, C.c_int, C.c_int],
+            "orc_add_twin_mono_anis": [C.c_void_p, _dp, _dp, _dp, _dp],
             "orc_add_dmi_int": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
@@ -202,6 +203,16 @@ class OracleSim:
         else:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
+
+    def add_twin_mono_anis(self, Ka=0, Kb=0, Kaa=0, Kbb=0, Kab=0, Ku=0, axis_a=(1, 0, 0), axis_b=(0, 1, 0),
+                           axis_u111=(0, 0, 1)):
+        """add_twin_mono_anis: src/micro/add_field.jl:629-660 (axes: 3-tuples or 3N AoS)."""
+        K = np.array([Ka, Kb, Kaa, Kbb, Kab, Ku], dtype=np.float64)
+        ax = []
+        for v in (axis_a, axis_b, axis_u111):
+            v = np.asarray(v, dtype=np.float64)
+            ax.append(np.ascontiguousarray(np.tile(v, self.N) if v.size == 3 else v.reshape(-1)))
+        return lib().orc_add_twin_mono_anis(self.h, _p(K), *(_p(a) for a in ax))
 
     def add_exch_int(self, J, k1=1, k2=-1):
         """add_exch_int(sim, J; k1, k2): src/micro/add_field.jl:789-816 (J: one value per column)."""

@@ -1,7 +1,8 @@
-"""Hexagonal anisotropy (add_hex_anis, kernels.jl:142-167) on the HIP path,
-through the C ABI, against the oracle: fields, energies and dm/dt with and
-without demag, an RK4 trajectory, a batch member against the same member
-advanced alone, and the adjoint's refusal.
+"""Hexagonal anisotropy (add_hex_anis, kernels.jl:142-167) and twin monoclinic
+anisotropy (add_twin_mono_anis, kernels.jl:169-225) on the HIP path, through
+the C ABI, against the oracle: fields, energies and dm/dt with and without
+demag, an RK4 trajectory, a batch member against the same member advanced
+alone, and the adjoint's refusal.
 
 FP64 gate: fields within 1e-12 (max-norm relative), trajectories within 1e-12
 absolute; FP32 fields within 1e-5.
@@ -47,6 +48,11 @@ def build(oracle, precision=64, demag=True, pbc="xy", seed=13):
     o.add_exch(1.2e-11)
     mmx.add_hex_anis(g, K1=4e5, K2=-1.2e5, K3=3e4)
     o.add_hex_anis(4e5, -1.2e5, 3e4)
+    axes = [rng.standard_normal(3 * N) for _ in range(3)]
+    tw = dict(Ka=2e4, Kb=-1e4, Kaa=5e3, Kbb=-3e3, Kab=7e3, Ku=1.5e4)
+    mmx.add_twin_mono_anis(g, **tw, axis_a=axes[0].reshape(N, 3), axis_b=axes[1].reshape(N, 3),
+                           axis_u111=axes[2].reshape(N, 3))
+    o.add_twin_mono_anis(**tw, axis_a=axes[0], axis_b=axes[1], axis_u111=axes[2])
     if demag:
         mmx.add_demag(g)
         o.add_demag()
