@@ -90,6 +90,12 @@ int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double
 /* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606, hexagonal_anisotropy_kernel!
  * src/micro/kernels.jl:142-167 -- E = K1 (1-mz^2) + K2 (1-mz^2)^2 + K3 Re((mx + i my)^6). */
 int mmx_add_hex_anis(mmx_ctx *ctx, double K1, double K2, double K3);
+/* add_sahe_torque(sim, sigma_s; sigma_sa1, sigma_sa2, a1, a2, a3, beta): src/micro/add_field.jl:892-936,
+ * sahe_torque_kernel! src/micro/kernels.jl:589-613 -- six 3N AoS vector fields; the torque enters as
+ * H = (beta sigma + m x sigma)/gamma with sigma = sigma_s x a1 - (m.a2)^2 sigma_sa1 x a1
+ * + (m.a3) sigma_sa2 .* m (no energy). */
+int mmx_add_sahe_torque(mmx_ctx *ctx, const double *sigma_s, const double *sigma_sa1, const double *sigma_sa2,
+                        const double *a1, const double *a2, const double *a3, double beta);
 /* add_twin_mono_anis(sim; Ka, Kb, Kaa, Kbb, Kab, Ku, axis_a, axis_b, axis_u111):
  * src/micro/add_field.jl:629-660, twin_monoclinic_anisotropy_kernel! src/micro/kernels.jl:169-225
  * -- K = {Ka, Kb, Kaa, Kbb, Kab, Ku}; three per-cell axes, 3N values each (AoS),

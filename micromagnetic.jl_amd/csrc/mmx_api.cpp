@@ -258,8 +258,8 @@ struct Engine : CtxBase {
         if (vec) {
             upload_soa(h, b->p);
             HIPCHECK(hipStreamSynchronize(C->stream));
-        } else {
-            upload_scalar(h, b->p, n);
+        } else {  // through the 3N staging buffer, in pieces for longer arrays
+            for (int64_t off = 0; off < n; off += 3 * N) upload_scalar(h + off, b->p + off, std::min<int64_t>(3 * N, n - off));
         }
         const T *p = b->p;
         arrays.push_back(std::move(b));
@@ -1056,7 +1056,7 @@ struct Engine : CtxBase {
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
             if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T ||
-                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.ufun)
+                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.kind == K_SAHE || h.ufun)
                 return fail(MMX_EINVAL, "the adjoint covers the time-independent exchange, DMI, uniaxial and cubic "
                                         "anisotropy, Zeeman and demag terms");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
@@ -2194,6 +2194,43 @@ int mmx_add_hex_anis(mmx_ctx *c, double K1, double K2, double K3)
         h.s[1] = (double)(TT)K2;
         h.s[2] = (double)(TT)K3;
         return add_term(c, e, h, nullptr, nullptr, nullptr);
+    });
+}
+
+int mmx_add_sahe_torque(mmx_ctx *c, const double *sigma_s, const double *sigma_sa1, const double *sigma_sa2,
+                        const double *a1, const double *a2, const double *a3, double beta)
+{
+    if (!c || !sigma_s || !sigma_sa1 || !sigma_sa2 || !a1 || !a2 || !a3) return fail(MMX_EINVAL, "null argument");
+    const int64_t N = c->N;
+    for (const double *v : {sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3})
+        if (!finite_all(v, 3 * N)) return fail(MMX_EINVAL, "NaN in a vector field");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        // SAHETorqueField(sigma_s x a1, sigma_sa1 x a1, sigma_sa2, a2, a3, beta): add_field.jl:892-936,
+        // the cross products in T (cross_product, src/util.jl:35-46), as 15 planes of N
+        std::vector<double> q((size_t)15 * N);
+        auto P = [&](int f, int comp, int64_t i) -> double & { return q[(size_t)(3 * f + comp) * N + i]; };
+        for (int64_t i = 0; i < N; i++) {
+            const int64_t k = 3 * i;
+            const TT x1 = (TT)a1[k], x2 = (TT)a1[k + 1], x3 = (TT)a1[k + 2];
+            for (int f = 0; f < 2; f++) {
+                const double *sv = f == 0 ? sigma_s : sigma_sa1;
+                const TT s1 = (TT)sv[k], s2 = (TT)sv[k + 1], s3 = (TT)sv[k + 2];
+                P(f, 0, i) = (double)(TT)(-s3 * x2 + s2 * x3);
+                P(f, 1, i) = (double)(TT)(s3 * x1 - s1 * x3);
+                P(f, 2, i) = (double)(TT)(-s2 * x1 + s1 * x2);
+            }
+            for (int comp = 0; comp < 3; comp++) {
+                P(2, comp, i) = sigma_sa2[k + comp];
+                P(3, comp, i) = a2[k + comp];
+                P(4, comp, i) = a3[k + comp];
+            }
+        }
+        HostTerm h;
+        h.kind = K_SAHE;
+        h.s[0] = (double)(TT)beta;
+        const TT *a = e.new_array(q.data(), 15 * N, false);
+        return add_term(c, e, h, a, nullptr, nullptr);
     });
 }
 

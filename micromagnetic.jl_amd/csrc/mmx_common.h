@@ -51,6 +51,7 @@ enum TermKind : int {
     K_HEX,          // hexagonal anisotropy, s = {K1, K2, K3}
     K_EXCH_IL,      // interlayer exchange: J per (i,j) in a (nx*ny), s = {-, -, -, dz, k1, k2}
     K_DMI_IL,       // interlayer DMI: s = {Dx, Dy, Dz, dz, k1, k2} (layers 0-based)
+    K_SAHE,         // spin anomalous Hall torque: a = 15 planes (sigma_s x a1, sigma_sa1 x a1, sigma_sa2, a2, a3), s[0] = beta
     K_TWIN,         // twin monoclinic anisotropy: s = {Ka, Kb, Kaa, Kbb, Kab, Ku}; axes a, b, u111 (3 planes each)
 };
 

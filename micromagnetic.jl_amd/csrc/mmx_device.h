@@ -344,6 +344,22 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             const T p6 = ((x4 * x2 - T(15) * x4 * y2) + T(15) * x2 * y4) - y4 * y2;
             e = (double)(((K1 * u + K2 * u * u) + K3 * p6) * V);
         } break;
+        case K_SAHE: {  // sahe_torque_kernel!: kernels.jl:589-613 (no energy)
+            if (Ms == T(0)) break;
+            const T *Q = D.a;  // plane f*3 + c of the packed vector fields
+#define MMX_SAHE(f, c) Q[(int64_t)(3 * (f) + (c)) * N + I]
+            const T pa2 = (mx * MMX_SAHE(3, 0) + my * MMX_SAHE(3, 1)) + mz * MMX_SAHE(3, 2);
+            const T ma2 = pa2 * pa2;
+            const T ma3 = (mx * MMX_SAHE(4, 0) + my * MMX_SAHE(4, 1)) + mz * MMX_SAHE(4, 2);
+            const T sx = (MMX_SAHE(0, 0) - ma2 * MMX_SAHE(1, 0)) + ma3 * MMX_SAHE(2, 0) * mx;
+            const T sy = (MMX_SAHE(0, 1) - ma2 * MMX_SAHE(1, 1)) + ma3 * MMX_SAHE(2, 1) * my;
+            const T sz = (MMX_SAHE(0, 2) - ma2 * MMX_SAHE(1, 2)) + ma3 * MMX_SAHE(2, 2) * mz;
+#undef MMX_SAHE
+            const T beta = D.s[0], g = P.gamma;
+            hx = (beta * sx + (-mz * sy + my * sz)) / g;
+            hy = (beta * sy + (mz * sx - mx * sz)) / g;
+            hz = (beta * sz + (-my * sx + mx * sy)) / g;
+        } break;
         case K_TWIN: {  // twin_monoclinic_anisotropy_kernel!: kernels.jl:169-225
             if (Ms == T(0)) break;
             const T ax = D.a[I], ay = D.a[N + I], az = D.a[2 * N + I];

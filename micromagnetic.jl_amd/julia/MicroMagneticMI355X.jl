@@ -149,6 +149,25 @@ function add_term!(h, t::MicroMagnetic.InterlayerDMI)
     ccall((:mmx_add_dmi_int, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cdouble, Cint, Cint), h, t.Dx, t.Dy,
           t.Dz, t.k1, t.k2) |> check
 end
+function add_term!(h, t::MicroMagnetic.SAHETorqueField)
+    # the struct keeps only the products sigma x a1, while libmmx takes the inputs of
+    # add_sahe_torque (add_field.jl:892-936)
+    error("attach the spin anomalous Hall torque with attach_sahe!(sim, sigma_s, sigma_sa1, " *
+          "sigma_sa2, a1, a2, a3, beta)")
+end
+"""
+    attach_sahe!(sim, sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3, beta)
+
+add_sahe_torque(sim, sigma_s; ...) on the MI355X path: 3-tuples or 3N AoS vectors.
+"""
+function attach_sahe!(sim, sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3, beta)
+    h = contexts[sim]
+    n = sim.n_total
+    vec(v) = v isa Tuple ? repeat(collect(Float64, v), n) : collect(Float64, v)
+    ccall((:mmx_add_sahe_torque, libmmx), Cint,
+          (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+           Cdouble), h, vec(sigma_s), vec(sigma_sa1), vec(sigma_sa2), vec(a1), vec(a2), vec(a3), beta) |> check
+end
 function add_term!(h, t::MicroMagnetic.TwinMonoclinicAnisotropy)
     K = Float64[t.Ka, t.Kb, t.Kaa, t.Kbb, t.Kab, t.Ku]
     ccall((:mmx_add_twin_mono_anis, libmmx), Cint,

@@ -354,6 +354,16 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
+def add_sahe_torque(sim, sigma_s, sigma_sa1=(0, 0, 0), sigma_sa2=(0, 0, 0), a1=(1, 0, 0), a2=(0, 0, 1),
+                    a3=(0, 0, 1), beta=0, name="she"):
+    """add_sahe_torque(sim, sigma_s; sigma_sa1, sigma_sa2, a1, a2, a3, beta, name):
+    src/micro/add_field.jl:892-936 (tuples, arrays or functions; no table column)."""
+    vs = [np.ascontiguousarray(init_vector(sim.mesh, v)) for v in (sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3)]
+    t = sim._add(_lib.load().mmx_add_sahe_torque(sim._h, *(dptr(v) for v in vs), float(beta)), name)
+    t.energy_unit = None
+    return t
+
+
 def add_twin_mono_anis(sim, Ka=0, Kb=0, Kaa=0, Kbb=0, Kab=0, Ku=0, axis_a=(1, 0, 0), axis_b=(0, 1, 0),
                        axis_u111=(0, 0, 1), name="twin_mono_anis"):
     """add_twin_mono_anis(sim; Ka, Kb, Kaa, Kbb, Kab, Ku, axis_a, axis_b, axis_u111, name):

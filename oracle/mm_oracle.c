@@ -44,7 +44,8 @@ enum {
     T_HEX,
     T_EXCH_IL,
     T_DMI_IL,
-    T_TWIN
+    T_TWIN,
+    T_SAHE
 };
 
 typedef struct {
@@ -53,6 +54,7 @@ typedef struct {
     double *energy; /* N */
     double p[12];
     double *a, *b, *c; /* per-cell parameter arrays */
+    double *ext[2];    /* two more (spin anomalous Hall torque: a2, a3) */
     /* demag */
     int Px, Py, Pz;
     double *ktensor[6]; /* real k-space tensors, Px*Py*Pz each */
@@ -165,6 +167,8 @@ static void free_term(term_t *t)
     free(t->a);
     free(t->b);
     free(t->c);
+    free(t->ext[0]);
+    free(t->ext[1]);
     for (int i = 0; i < 6; i++) free(t->ktensor[i]);
     for (int i = 0; i < 3; i++) free(t->buf[i]);
     free(t->rtensor);
@@ -1384,6 +1388,62 @@ static void field_hex(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_sahe_torque: src/micro/add_field.jl:892-936 (six 3N AoS fields).  Kept as
+ * a = sigma_s x a1, b = sigma_sa1 x a1 (cross_product, src/util.jl:35-46),
+ * c = sigma_sa2, ext[0] = a2, ext[1] = a3; p[0] = beta */
+int orc_add_sahe_torque(orc_sim *s, const double *sigma_s, const double *sigma_sa1, const double *sigma_sa2,
+                        const double *a1, const double *a2, const double *a3, double beta)
+{
+    term_t *t = new_term(s, T_SAHE);
+    const size_t n3 = 3 * (size_t)s->N;
+    t->a = dup_array(sigma_s, n3);
+    t->b = dup_array(sigma_sa1, n3);
+    for (int i = 0; i < s->N; i++) {
+        const double *y = a1 + 3 * (size_t)i;
+        double *x = t->a + 3 * (size_t)i, *z = t->b + 3 * (size_t)i;
+        const double r0 = -x[2] * y[1] + x[1] * y[2], r1 = x[2] * y[0] - x[0] * y[2], r2 = -x[1] * y[0] + x[0] * y[1];
+        x[0] = r0;
+        x[1] = r1;
+        x[2] = r2;
+        const double q0 = -z[2] * y[1] + z[1] * y[2], q1 = z[2] * y[0] - z[0] * y[2], q2 = -z[1] * y[0] + z[0] * y[1];
+        z[0] = q0;
+        z[1] = q1;
+        z[2] = q2;
+    }
+    t->c = dup_array(sigma_sa2, n3);
+    t->ext[0] = dup_array(a2, n3);
+    t->ext[1] = dup_array(a3, n3);
+    t->p[0] = beta;
+    return s->nterms - 1;
+}
+
+/* sahe_torque_kernel!: src/micro/kernels.jl:589-613 (no energy) */
+static void field_sahe(orc_sim *s, term_t *t, const double *m)
+{
+    const double beta = t->p[0], gamma = s->gamma, *Ms = s->mu0_Ms;
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        double *h = t->field;
+        t->energy[I] = 0;
+        if (Ms[I] == 0) {
+            h[j] = h[j + 1] = h[j + 2] = 0;
+            continue;
+        }
+        const double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        const double *A2 = t->ext[0] + j, *A3 = t->ext[1] + j, *S1 = t->a + j, *S2 = t->b + j, *S3 = t->c + j;
+        double ma2 = mx * A2[0] + my * A2[1] + mz * A2[2];
+        ma2 = ma2 * ma2;
+        const double ma3 = mx * A3[0] + my * A3[1] + mz * A3[2];
+        const double sx = S1[0] - ma2 * S2[0] + ma3 * S3[0] * mx;
+        const double sy = S1[1] - ma2 * S2[1] + ma3 * S3[1] * my;
+        const double sz = S1[2] - ma2 * S2[2] + ma3 * S3[2] * mz;
+        h[j] = (beta * sx + cross_x(mx, my, mz, sx, sy, sz)) / gamma;
+        h[j + 1] = (beta * sy + cross_y(mx, my, mz, sx, sy, sz)) / gamma;
+        h[j + 2] = (beta * sz + cross_z(mx, my, mz, sx, sy, sz)) / gamma;
+    }
+}
+
 /* add_twin_mono_anis: src/micro/add_field.jl:629-660; axes 3N AoS, normalised here
  * (normalise, src/helper.jl:255-271); p = {Ka, Kb, Kaa, Kbb, Kab, Ku} */
 int orc_add_twin_mono_anis(orc_sim *s, const double K[6], const double *axis_a, const double *axis_b,
@@ -1571,6 +1631,7 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_EXCH_IL:
     case T_DMI_IL: field_interlayer(s, t, m); break;
     case T_TWIN: field_twin(s, t, m); break;
+    case T_SAHE: field_sahe(s, t, m); break;
     }
 }
 

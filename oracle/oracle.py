@@ -78,6 +78,7 @@ def lib():
             "orc_add_hex_anis": [C.c_void_p, C.c_double, C.c_double, C.c_double],
             "orc_add_exch_int": [C.c_void_p, _dp, C.c_int, C.c_int],
             "orc_add_twin_mono_anis": [C.c_void_p, _dp, _dp, _dp, _dp],
+            "orc_add_sahe_torque": [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double],
             "orc_add_dmi_int": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
@@ -203,6 +204,15 @@ class OracleSim:
         else:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
+
+    def add_sahe_torque(self, sigma_s, sigma_sa1=(0, 0, 0), sigma_sa2=(0, 0, 0), a1=(1, 0, 0), a2=(0, 0, 1),
+                        a3=(0, 0, 1), beta=0.0):
+        """add_sahe_torque: src/micro/add_field.jl:892-936 (3-tuples or 3N AoS)."""
+        vs = []
+        for v in (sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3):
+            v = np.asarray(v, dtype=np.float64)
+            vs.append(np.ascontiguousarray(np.tile(v, self.N) if v.size == 3 else v.reshape(-1)))
+        return lib().orc_add_sahe_torque(self.h, *(_p(v) for v in vs), float(beta))
 
     def add_twin_mono_anis(self, Ka=0, Kb=0, Kaa=0, Kbb=0, Kab=0, Ku=0, axis_a=(1, 0, 0), axis_b=(0, 1, 0),
                            axis_u111=(0, 0, 1)):

@@ -180,3 +180,52 @@ def test_zhang_li_domain_wall_velocity():
         v_expect = -(1 + alpha * beta) / (1 + alpha ** 2) * u
         print(f"beta={beta} u={u}: v={v:.4f} expected {v_expect:.4f}")
         assert abs((v - v_expect) / v_expect) < 0.006
+
+
+def test_sahe_torque_reference_and_oracle(oracle):
+    """test/test_sahe_torque.jl:8-30 on the HIP path, then per-cell vector fields
+    against the oracle (FP64 1e-12, FP32 1e-5) and an RK4 trajectory."""
+    note("spin anomalous Hall torque")
+    mesh = mmx.FDMesh(nx=1, ny=1, nz=1, dx=1e-9, dy=1e-9, dz=1e-9)
+    sim = mmx.Sim(mesh)
+    mmx.set_Ms(sim, 8e5)
+    sim.driver.alpha, sim.driver.gamma = 0.05, 2.21e5
+    mmx.add_zeeman(sim, (0, 0, 1e5))
+    t = mmx.add_sahe_torque(sim, (0.1, 0.2, 0.3), sigma_sa1=(0.3, 0.4, 0.5), sigma_sa2=(0.6, 0.7, 0.8), beta=0.135)
+    mmx.init_m0(sim, (-0.3, 0.4, 0.8), norm=False)
+    mmx.effective_field(sim)
+    np.testing.assert_allclose(t.field * 1e7, [2.01628959, 3.74389140, 3.30678733], rtol=1e-6)
+
+    nx, ny, nz = 9, 8, 2
+    N = nx * ny * nz
+    rng = np.random.default_rng(31)
+    vs = [rng.standard_normal(3 * N) for _ in range(6)]
+    Ms = np.where(rng.uniform(size=N) < 0.1, 0.0, 7e5)
+    m0 = rng.standard_normal(3 * N)
+    for precision in (64, 32):
+        mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz, pbc="xy")
+        g = mmx.Sim(mesh, precision=precision)
+        o = oracle.OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9, pbc="xy")
+        mmx.set_Ms(g, Ms)
+        o.set_Ms(Ms)
+        mmx.init_m0(g, m0)
+        o.init_m0(m0)
+        g.driver.alpha, g.driver.gamma = 0.1, 2.21e5
+        o.set_llg(0.1, 2.21e5, True)
+        mmx.add_exch(g, 1e-11)
+        o.add_exch(1e-11)
+        mmx.add_sahe_torque(g, *(v.reshape(N, 3) for v in vs[:1]), sigma_sa1=vs[1].reshape(N, 3),
+                            sigma_sa2=vs[2].reshape(N, 3), a1=vs[3].reshape(N, 3), a2=vs[4].reshape(N, 3),
+                            a3=vs[5].reshape(N, 3), beta=0.2)
+        o.add_sahe_torque(vs[0], sigma_sa1=vs[1], sigma_sa2=vs[2], a1=vs[3], a2=vs[4], a3=vs[5], beta=0.2)
+        tol = 1e-12 if precision == 64 else 1e-5
+        for q in range(2):
+            assert relerr(g._term_field(q)[0], o.term_field(q)[0]) <= tol, (precision, q)
+        assert relerr(mmx.llg_rhs(g), o.llg_rhs()) <= tol
+        if precision == 64:
+            mmx.set_driver(g, integrator="RungeKutta", step=1e-14)
+            o.set_integrator(0, 1e-14, 0)
+            for _ in range(20):
+                o.advance_step()
+            mmx.run_step(g, n=20)
+            assert np.max(np.abs(g.spin - o.spin)) < 1e-12
