@@ -90,6 +90,13 @@ int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double
 /* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606, hexagonal_anisotropy_kernel!
  * src/micro/kernels.jl:142-167 -- E = K1 (1-mz^2) + K2 (1-mz^2)^2 + K3 Re((mx + i my)^6). */
 int mmx_add_hex_anis(mmx_ctx *ctx, double K1, double K2, double K3);
+/* add_magnetoelastic(sim; model, lambda_s, B1, B2, sigma / strain) (add_mel):
+ * src/micro/add_field.jl:1367-1405, kernels src/micro/kernels.jl:742-845 -- data holds 6N
+ * values, cell by cell in Voigt order (xx, yy, zz, xy, xz, yz) as the reference's (6, nx, ny, nz)
+ * array; MMX_MEL_TENSOR: stress and lambda_s; MMX_MEL_CUBIC: strain and B1, B2. */
+#define MMX_MEL_TENSOR 0
+#define MMX_MEL_CUBIC 1
+int mmx_add_magnetoelastic(mmx_ctx *ctx, int model, const double *data, double lambda_s, double B1, double B2);
 /* add_sahe_torque(sim, sigma_s; sigma_sa1, sigma_sa2, a1, a2, a3, beta): src/micro/add_field.jl:892-936,
  * sahe_torque_kernel! src/micro/kernels.jl:589-613 -- six 3N AoS vector fields; the torque enters as
  * H = (beta sigma + m x sigma)/gamma with sigma = sigma_s x a1 - (m.a2)^2 sigma_sa1 x a1

@@ -1056,7 +1056,8 @@ struct Engine : CtxBase {
         if (!(C->step > 0)) return fail(MMX_ESTATE, "the RK4 step is not set");
         for (const auto &h : C->terms)
             if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T ||
-                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.kind == K_SAHE || h.ufun)
+                h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.kind == K_SAHE || h.kind == K_MEL_T || h.kind == K_MEL_C ||
+                h.ufun)
                 return fail(MMX_EINVAL, "the adjoint covers the time-independent exchange, DMI, uniaxial and cubic "
                                         "anisotropy, Zeeman and demag terms");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
@@ -2194,6 +2195,28 @@ int mmx_add_hex_anis(mmx_ctx *c, double K1, double K2, double K3)
         h.s[1] = (double)(TT)K2;
         h.s[2] = (double)(TT)K3;
         return add_term(c, e, h, nullptr, nullptr, nullptr);
+    });
+}
+
+int mmx_add_magnetoelastic(mmx_ctx *c, int model, const double *data, double lambda_s, double B1, double B2)
+{
+    if (!c || !data) return fail(MMX_EINVAL, "null argument");
+    if (model != MMX_MEL_TENSOR && model != MMX_MEL_CUBIC)
+        return fail(MMX_EINVAL, "Unknown model: use :tensor or :cubic");
+    const int64_t N = c->N;
+    if (!finite_all(data, 6 * N)) return fail(MMX_EINVAL, "NaN in the stress / strain field");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        // Magnetoelastic(model, ...): add_field.jl:1367-1405; the (6, N) Voigt field as 6 planes
+        std::vector<double> q((size_t)6 * N);
+        for (int64_t i = 0; i < N; i++)
+            for (int comp = 0; comp < 6; comp++) q[(size_t)comp * N + i] = data[6 * i + comp];
+        HostTerm h;
+        h.kind = model == MMX_MEL_TENSOR ? K_MEL_T : K_MEL_C;
+        h.s[0] = (double)(TT)(model == MMX_MEL_TENSOR ? lambda_s : B1);
+        h.s[1] = (double)(TT)(model == MMX_MEL_TENSOR ? 0.0 : B2);
+        const TT *a = e.new_array(q.data(), 6 * N, false);
+        return add_term(c, e, h, a, nullptr, nullptr);
     });
 }
 

@@ -51,6 +51,8 @@ enum TermKind : int {
     K_HEX,          // hexagonal anisotropy, s = {K1, K2, K3}
     K_EXCH_IL,      // interlayer exchange: J per (i,j) in a (nx*ny), s = {-, -, -, dz, k1, k2}
     K_DMI_IL,       // interlayer DMI: s = {Dx, Dy, Dz, dz, k1, k2} (layers 0-based)
+    K_MEL_T,        // magnetoelastic, tensor model: a = 6 planes of stress (xx,yy,zz,xy,xz,yz), s[0] = lambda_s
+    K_MEL_C,        // magnetoelastic, cubic model: a = 6 planes of strain, s = {B1, B2}
     K_SAHE,         // spin anomalous Hall torque: a = 15 planes (sigma_s x a1, sigma_sa1 x a1, sigma_sa2, a2, a3), s[0] = beta
     K_TWIN,         // twin monoclinic anisotropy: s = {Ka, Kb, Kaa, Kbb, Kab, Ku}; axes a, b, u111 (3 planes each)
 };

@@ -344,6 +344,31 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
             const T p6 = ((x4 * x2 - T(15) * x4 * y2) + T(15) * x2 * y4) - y4 * y2;
             e = (double)(((K1 * u + K2 * u * u) + K3 * p6) * V);
         } break;
+        case K_MEL_T:
+        case K_MEL_C: {  // magnetoelastic_tensor_kernel! / _cubic_kernel!: kernels.jl:742-845
+            if (Ms == T(0)) break;
+            const T *Q = D.a;  // Voigt planes xx, yy, zz, xy, xz, yz
+            const T qxx = Q[I], qyy = Q[N + I], qzz = Q[2 * N + I];
+            const T qxy = Q[3 * N + I], qxz = Q[4 * N + I], qyz = Q[5 * N + I];
+            if (D.kind == K_MEL_T) {
+                const T ls = D.s[0];
+                const T sum = ((((qxx * mx * mx + qyy * my * my) + qzz * mz * mz) + T(2) * qxy * mx * my) +
+                               T(2) * qxz * mx * mz) + T(2) * qyz * my * mz;
+                e = (double)(-T(1.5) * ls * sum * V);
+                const T f = T(3) * ls / Ms;
+                hx = f * ((qxx * mx + qxy * my) + qxz * mz);
+                hy = f * ((qxy * mx + qyy * my) + qyz * mz);
+                hz = f * ((qxz * mx + qyz * my) + qzz * mz);
+            } else {
+                const T B1 = D.s[0], B2 = D.s[1];
+                e = (double)((B1 * ((qxx * mx * mx + qyy * my * my) + qzz * mz * mz) +
+                              T(2) * B2 * ((qxy * mx * my + qxz * mx * mz) + qyz * my * mz)) * V);
+                const T nf = -(T(2) / Ms);
+                hx = nf * (B1 * qxx * mx + B2 * (qxy * my + qxz * mz));
+                hy = nf * (B1 * qyy * my + B2 * (qxy * mx + qyz * mz));
+                hz = nf * (B1 * qzz * mz + B2 * (qxz * mx + qyz * my));
+            }
+        } break;
         case K_SAHE: {  // sahe_torque_kernel!: kernels.jl:589-613 (no energy)
             if (Ms == T(0)) break;
             const T *Q = D.a;  // plane f*3 + c of the packed vector fields

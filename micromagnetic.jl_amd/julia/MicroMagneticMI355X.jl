@@ -168,6 +168,12 @@ function attach_sahe!(sim, sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3, beta)
           (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
            Cdouble), h, vec(sigma_s), vec(sigma_sa1), vec(sigma_sa2), vec(a1), vec(a2), vec(a3), beta) |> check
 end
+function add_term!(h, t::MicroMagnetic.Magnetoelastic)
+    # field_data: the 6N Voigt values, cell by cell (add_field.jl:1413-1435)
+    model = t.model == :tensor ? Cint(0) : Cint(1)
+    ccall((:mmx_add_magnetoelastic, libmmx), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Cdouble, Cdouble, Cdouble),
+          h, model, host(t.field_data), t.lambda_s, t.B1, t.B2) |> check
+end
 function add_term!(h, t::MicroMagnetic.TwinMonoclinicAnisotropy)
     K = Float64[t.Ka, t.Kb, t.Kaa, t.Kbb, t.Kab, t.Ku]
     ccall((:mmx_add_twin_mono_anis, libmmx), Cint,

@@ -354,6 +354,50 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
+def _six_component(mesh, data):
+    """_init_six_component_field: src/micro/add_field.jl:1413-1435 -- a (6, nx, ny, nz)
+    array (Fortran order, as Julia holds it) or f(i,j,k,dx,dy,dz) / f(x,y,z) with
+    centred coordinates (init_vector6!, src/helper.jl:104-138); 6N values out."""
+    N = mesh.n_total
+    if callable(data):
+        import inspect
+
+        nargs = len(inspect.signature(data).parameters)
+        i, j, k = _ijk(mesh)
+        out = np.zeros((N, 6))
+        for q, (a, b, c) in enumerate(zip(i, j, k)):
+            if nargs == 6:
+                v = data(a, b, c, mesh.dx, mesh.dy, mesh.dz)
+            elif nargs == 3:
+                v = data((a - 0.5 - mesh.nx / 2) * mesh.dx, (b - 0.5 - mesh.ny / 2) * mesh.dy,
+                         (c - 0.5 - mesh.nz / 2) * mesh.dz)
+            else:
+                raise ValueError("The input function should have either 6 or 3 arguments.")
+            if v is not None:
+                out[q] = v
+        return np.ascontiguousarray(out.reshape(-1))
+    a = np.asarray(data, dtype=np.float64)
+    if a.shape != (6, mesh.nx, mesh.ny, mesh.nz):
+        raise ValueError(f"Array size must be (6, nx, ny, nz) = (6, {mesh.nx}, {mesh.ny}, {mesh.nz})")
+    return np.ascontiguousarray(a.reshape(-1, order="F"))
+
+
+def add_magnetoelastic(sim, model="tensor", lambda_s=0.0, B1=0.0, B2=0.0, sigma=None, strain=None,
+                       name="magnetoelastic"):
+    """add_magnetoelastic(sim; model, lambda_s, B1, B2, sigma, strain, name) (add_mel):
+    src/micro/add_field.jl:1367-1405."""
+    model = str(model).lstrip(":")
+    if model not in ("tensor", "cubic"):
+        raise ValueError(f"Unknown model: {model}. Use :tensor or :cubic")
+    data = _six_component(sim.mesh, sigma if model == "tensor" else strain)
+    kind = _lib.MEL_TENSOR if model == "tensor" else _lib.MEL_CUBIC
+    return sim._add(_lib.load().mmx_add_magnetoelastic(sim._h, kind, dptr(data), float(lambda_s), float(B1),
+                                                       float(B2)), name)
+
+
+add_mel = add_magnetoelastic
+
+
 def add_sahe_torque(sim, sigma_s, sigma_sa1=(0, 0, 0), sigma_sa2=(0, 0, 0), a1=(1, 0, 0), a2=(0, 0, 1),
                     a3=(0, 0, 1), beta=0, name="she"):
     """add_sahe_torque(sim, sigma_s; sigma_sa1, sigma_sa2, a1, a2, a3, beta, name):

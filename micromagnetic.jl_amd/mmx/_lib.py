@@ -16,6 +16,7 @@ HEADER = os.path.join(REPO_ROOT, "include", "mmx.h")
 OK, EINVAL, EHIP, ENAN, ETRIES, ESTATE, ENOMEM = 0, -1, -2, -3, -4, -5, -6
 F64, F32 = 64, 32
 DMI_BULK, DMI_INTERFACIAL = 0, 1
+MEL_TENSOR, MEL_CUBIC = 0, 1
 RK4, HEUN, DOPRI54, BS23, SD = 0, 1, 2, 3, 4
 PASSES = ("P1", "P2", "P3", "P4", "P5", "LOCAL")
 
@@ -62,6 +63,7 @@ SIGNATURES = {
     "mmx_add_exch_int": [_vp, _dp, C.c_int, C.c_int],
     "mmx_add_twin_mono_anis": [_vp, _dp, _dp, _dp, _dp],
     "mmx_add_sahe_torque": [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double],
+    "mmx_add_magnetoelastic": [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double],
     "mmx_add_dmi_int": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
     "mmx_set_alpha": [_vp, _dp],
     "mmx_num_terms": [_vp],

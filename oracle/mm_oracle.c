@@ -45,7 +45,9 @@ enum {
     T_EXCH_IL,
     T_DMI_IL,
     T_TWIN,
-    T_SAHE
+    T_SAHE,
+    T_MEL_T,
+    T_MEL_C
 };
 
 typedef struct {
@@ -1388,6 +1390,55 @@ static void field_hex(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_magnetoelastic(sim; model, lambda_s, B1, B2, sigma / strain): src/micro/add_field.jl:1367-1405;
+ * data: 6N values in Voigt order per cell (the reference's (6, nx, ny, nz) array) */
+int orc_add_magnetoelastic(orc_sim *s, int cubic, const double *data, double lambda_s, double B1, double B2)
+{
+    term_t *t = new_term(s, cubic ? T_MEL_C : T_MEL_T);
+    t->a = dup_array(data, 6 * (size_t)s->N);
+    t->p[0] = lambda_s;
+    t->p[1] = B1;
+    t->p[2] = B2;
+    return s->nterms - 1;
+}
+
+/* magnetoelastic_tensor_kernel! / magnetoelastic_cubic_kernel!: src/micro/kernels.jl:742-845 */
+static void field_mel(orc_sim *s, term_t *t, const double *m)
+{
+    const double *Ms = s->mu0_Ms, V = s->volume;
+#pragma omp parallel for schedule(static)
+    for (int id = 0; id < s->N; id++) {
+        size_t j = 3 * (size_t)id;
+        const double *q = t->a + 6 * (size_t)id;
+        double *h = t->field;
+        if (Ms[id] == 0) {
+            t->energy[id] = 0;
+            h[j] = h[j + 1] = h[j + 2] = 0;
+            continue;
+        }
+        const double mx = m[j], my = m[j + 1], mz = m[j + 2];
+        const double xx = q[0], yy = q[1], zz = q[2], xy = q[3], xz = q[4], yz = q[5];
+        if (t->kind == T_MEL_T) {
+            const double ls = t->p[0];
+            t->energy[id] = -1.5 * ls *
+                            (xx * mx * mx + yy * my * my + zz * mz * mz + 2 * xy * mx * my + 2 * xz * mx * mz +
+                             2 * yz * my * mz) * V;
+            const double f = 3 * ls / Ms[id];
+            h[j] = f * (xx * mx + xy * my + xz * mz);
+            h[j + 1] = f * (xy * mx + yy * my + yz * mz);
+            h[j + 2] = f * (xz * mx + yz * my + zz * mz);
+        } else {
+            const double B1 = t->p[1], B2 = t->p[2];
+            t->energy[id] = (B1 * (xx * mx * mx + yy * my * my + zz * mz * mz) +
+                             2 * B2 * (xy * mx * my + xz * mx * mz + yz * my * mz)) * V;
+            const double f = 2 / Ms[id];
+            h[j] = -f * (B1 * xx * mx + B2 * (xy * my + xz * mz));
+            h[j + 1] = -f * (B1 * yy * my + B2 * (xy * mx + yz * mz));
+            h[j + 2] = -f * (B1 * zz * mz + B2 * (xz * mx + yz * my));
+        }
+    }
+}
+
 /* add_sahe_torque: src/micro/add_field.jl:892-936 (six 3N AoS fields).  Kept as
  * a = sigma_s x a1, b = sigma_sa1 x a1 (cross_product, src/util.jl:35-46),
  * c = sigma_sa2, ext[0] = a2, ext[1] = a3; p[0] = beta */
@@ -1632,6 +1683,8 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_DMI_IL: field_interlayer(s, t, m); break;
     case T_TWIN: field_twin(s, t, m); break;
     case T_SAHE: field_sahe(s, t, m); break;
+    case T_MEL_T:
+    case T_MEL_C: field_mel(s, t, m); break;
     }
 }
 

@@ -79,6 +79,7 @@ def lib():
             "orc_add_exch_int": [C.c_void_p, _dp, C.c_int, C.c_int],
             "orc_add_twin_mono_anis": [C.c_void_p, _dp, _dp, _dp, _dp],
             "orc_add_sahe_torque": [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double],
+            "orc_add_magnetoelastic": [C.c_void_p, C.c_int, _dp, C.c_double, C.c_double, C.c_double],
             "orc_add_dmi_int": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
@@ -204,6 +205,12 @@ class OracleSim:
         else:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
+
+    def add_magnetoelastic(self, model="tensor", lambda_s=0.0, B1=0.0, B2=0.0, data=None):
+        """add_magnetoelastic: src/micro/add_field.jl:1367-1405; data (6N, Voigt per cell)."""
+        d = _f64(np.asarray(data, dtype=np.float64).reshape(-1), 6 * self.N)
+        return lib().orc_add_magnetoelastic(self.h, int(model == "cubic"), _p(d), float(lambda_s), float(B1),
+                                            float(B2))
 
     def add_sahe_torque(self, sigma_s, sigma_sa1=(0, 0, 0), sigma_sa2=(0, 0, 0), a1=(1, 0, 0), a2=(0, 0, 1),
                         a3=(0, 0, 1), beta=0.0):
