@@ -344,6 +344,16 @@ def add_anis(sim, Ku, axis=(0, 0, 1), name="anis"):
     return sim._add(L.mmx_add_anis(sim._h, dptr(Kua), Ku0, dptr(ax), 0.0, 0.0, 0.0), name)
 
 
+def update_anis(sim, Ku, name="anis"):
+    """update_anis(sim, Ku; name): src/micro/add_field.jl:514-525 -- new per-cell Ku for the
+    anisotropy term of that name (silently nothing when there is none, as the reference)."""
+    for t in sim.interactions:
+        if t.name == name:
+            set_term_param(sim, t, init_scalar(sim.mesh, Ku))
+            return None
+    return None
+
+
 def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     """add_cubic_anis: src/micro/add_field.jl:541-574 (a number Kc stays uniform)."""
     L = _lib.load()

@@ -108,3 +108,22 @@ def test_hex_batch_member_and_adjoint_refusal():
     del bt
     with pytest.raises(mmx.MmxError):
         mmx.rk4_trajectory_rrule(b, 4, every=2)
+
+
+def test_update_anis_equals_fresh_term():
+    """update_anis(sim, Ku; name) (add_field.jl:514-525): the updated term's field is
+    the field of a term built with the new Ku."""
+    note("update_anis")
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=12, ny=6, nz=2)
+    rng = np.random.default_rng(2)
+    m0 = rng.standard_normal(3 * mesh.n_total)
+    Ku2 = rng.uniform(1e5, 5e5, mesh.n_total)
+    a = mmx.Sim(mesh)
+    b = mmx.Sim(mesh)
+    for s in (a, b):
+        mmx.set_Ms(s, 6e5)
+        mmx.init_m0(s, m0)
+    ta = mmx.add_anis(a, 3e5, axis=(0, 1, 1))
+    mmx.update_anis(a, Ku2)
+    tb = mmx.add_anis(b, Ku2, axis=(0, 1, 1))
+    assert np.array_equal(ta.field, tb.field) and np.array_equal(ta.energy, tb.energy)
