@@ -469,6 +469,64 @@ def add_zeeman(sim, H0, ft=None, name=None):
     return t
 
 
+def _surface_x(x, y, z, b, c, sigma):
+    """Field of a charged rectangle (half sides b, c) in the plane x = 0 at (x, y, z):
+    field_x_surface, src/micro/demagc.jl:1-26 (F = atan(yz/(xr)), G = -log|r + z|)."""
+
+    def F(x, y, z):
+        r = np.sqrt(x * x + y * y + z * z)
+        return np.arctan(y * z / (x * r))
+
+    def G(x, y, z):
+        r = np.sqrt(x * x + y * y + z * z)
+        return -np.log(np.abs(r + z))
+
+    def comb(f, x, y, z, b, c):
+        return (f(x, y + b, z + c) - f(x, y - b, z + c) - f(x, y + b, z - c) + f(x, y - b, z - c)) / (4 * math.pi)
+
+    return (-comb(F, x, y, z, b, c) * sigma, -comb(G, x, y, z, b, c) * sigma, -comb(G, x, z, y, c, b) * sigma)
+
+
+def demag_charge_field(m, Ms, x=(0, 0), y=(0, 0), z=(0, 0)):
+    """compute_demag_charge_field, src/micro/demagc.jl:38-83: the field that removes the
+    surface charges (x-, x+), (y-, y+), (z-, z+) (in units of Ms) of mesh m, at the cell
+    centres (lengths in nm, as the reference); 3N values, AoS."""
+    dx, dy, dz = m.dx * 1e9, m.dy * 1e9, m.dz * 1e9
+    Lx, Ly, Lz = m.nx * dx, m.ny * dy, m.nz * dz
+    a, b, c = Lx / 2, Ly / 2, Lz / 2
+    K, J, I = np.meshgrid(np.arange(1, m.nz + 1), np.arange(1, m.ny + 1), np.arange(1, m.nx + 1), indexing="ij")
+    H = [np.zeros(I.shape) for _ in range(3)]
+    (xm, xp), (ym, yp), (zm, zp) = ((v[0] * Ms, v[1] * Ms) for v in (x, y, z))
+    if xm != 0 or xp != 0:
+        X, Y, Z = (I - 0.5) * dx, (J - 0.5) * dy - b, (K - 0.5) * dz - c
+        f1, f2 = _surface_x(X, Y, Z, b, c, xm), _surface_x(X - Lx, Y, Z, b, c, xp)
+        for q in range(3):
+            H[q] += f1[q] + f2[q]
+    if ym != 0 or yp != 0:
+        X, Y, Z = (I - 0.5) * dx - a, (J - 0.5) * dy, (K - 0.5) * dz - c
+        g1, g2 = _surface_x(Y, Z, X, c, a, ym), _surface_x(Y - Ly, Z, X, c, a, yp)  # (Hy, Hz, Hx)
+        for q, src in ((1, 0), (2, 1), (0, 2)):
+            H[q] += g1[src] + g2[src]
+    if zm != 0 or zp != 0:
+        X, Y, Z = (I - 0.5) * dx - a, (J - 0.5) * dy - b, (K - 0.5) * dz
+        h1, h2 = _surface_x(Z, X, Y, a, b, zm), _surface_x(Z - Lz, X, Y, a, b, zp)  # (Hz, Hx, Hy)
+        for q, src in ((2, 0), (0, 1), (1, 2)):
+            H[q] += h1[src] + h2[src]
+    return np.ascontiguousarray(np.stack([h.reshape(-1) for h in H], axis=1).reshape(-1))
+
+
+def rm_demag_charges(sim, Ms, x=(0, 0), y=(0, 0), z=(0, 0), name="demagc"):
+    """rm_demag_charges(sim, Ms; x, y, z, name): src/micro/add_field.jl:1277-1300 -- the
+    field of demag_charge_field as a static Zeeman term (energy column only)."""
+    for v, n in ((x, "x"), (y, "y"), (z, "z")):
+        if len(v) != 2:
+            raise ValueError(f"{n} must be a 2-element tuple")
+    field = demag_charge_field(sim.mesh, Ms, x, y, z)
+    t = sim._add(_lib.load().mmx_add_zeeman(sim._h, dptr(field), 0.0, 0.0, 0.0), name)
+    t.H0 = (0.0, 0.0, 0.0)
+    return t
+
+
 def update_zeeman(sim, H0, name="zeeman"):
     L = _lib.load()
     for t in sim.interactions:

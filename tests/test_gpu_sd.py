@@ -60,3 +60,26 @@ def test_sd_relax_matches_oracle():
     assert i1 == i2 == 60
     assert abs(md1 - md2) <= 1e-6 * md1
     assert np.abs(sim.spin - osd.spin.reshape(-1)).max() < 1e-8
+
+
+@pytest.mark.parametrize("shape", [(30, 8, 3), (9, 30, 3), (3, 8, 30)])
+def test_rm_demag_charges_relax(shape):
+    """test/test_rm_charges.jl:8-44: with the charges on the long axis's faces
+    removed, the SD-relaxed corners of a bar lie along that axis to 5e-3."""
+    nx, ny, nz = shape
+    axis = int(np.argmax(shape))
+    print(f"[case] rm_demag_charges relax {shape}", flush=True)
+    mesh = mmx.FDMesh(nx=nx, ny=ny, nz=nz, dx=2e-9, dy=2e-9, dz=2e-9)
+    sim = mmx.Sim(mesh, driver="SD")
+    mmx.set_Ms(sim, 8e5)
+    mmx.add_exch(sim, 1.3e-12)
+    mmx.add_demag(sim)
+    kw = {0: dict(x=(-1, 1)), 1: dict(y=(-1, 1)), 2: dict(z=(-1, 1))}[axis]
+    mmx.rm_demag_charges(sim, 8e5, **kw)
+    mmx.init_m0(sim, (1, 1, 1))
+    mmx.relax(sim, max_steps=20000, stopping_dmdt=0.01)
+    m = sim.spin.reshape(nz, ny, nx, 3)
+    corners = [m[0, 0, 0], m[0, -1, 0], m[0, 0, -1], m[0, -1, -1]]
+    mp = max(np.sqrt(1 - c[axis] ** 2) for c in corners)
+    print(f"max transverse part at the corners: {mp:.2e}")
+    assert mp < 5e-3
