@@ -90,6 +90,12 @@ int mmx_update_zeeman(mmx_ctx *ctx, int term, const double *H, double Hx, double
 /* add_hex_anis(sim; K1, K2, K3): src/micro/add_field.jl:590-606, hexagonal_anisotropy_kernel!
  * src/micro/kernels.jl:142-167 -- E = K1 (1-mz^2) + K2 (1-mz^2)^2 + K3 Re((mx + i my)^6). */
 int mmx_add_hex_anis(mmx_ctx *ctx, double K1, double K2, double K3);
+/* add_torque(sim, fun): src/micro/add_field.jl:998-1006, TorqueField field.jl:282-293,
+ * torque_kernel! src/micro/kernels.jl:639-651 -- at every evaluation fn(t, m, y, user) gets
+ * the state m (3N AoS, the stage vector inside the steppers) and writes the torque y
+ * (3N AoS, zeroed before the call); the term is H = (1/gamma) m x y.  Runs on the host:
+ * every evaluation then copies the state out and the torque in. */
+int mmx_add_torque(mmx_ctx *ctx, void (*fn)(double t, const double *m, double *y, void *user), void *user);
 /* add_magnetoelastic(sim; model, lambda_s, B1, B2, sigma / strain) (add_mel):
  * src/micro/add_field.jl:1367-1405, kernels src/micro/kernels.jl:742-845 -- data holds 6N
  * values, cell by cell in Voigt order (xx, yy, zz, xy, xz, yz) as the reference's (6, nx, ny, nz)

@@ -96,6 +96,8 @@ struct HostTerm {
     double (*ufun)(double, void *) = nullptr;  // torque amplitude ufun(t) (null: 1)
     void *uuser = nullptr;
     void (*ft3)(double, double *, void *) = nullptr;  // time Zeeman (fx, fy, fz)(t)
+    // add_torque: fn(t, m, y, user) writes the torque y (3N AoS) for the state m (3N AoS)
+    void (*tfn)(double, const double *, double *, void *) = nullptr;
 };
 
 struct CtxBase {
@@ -148,6 +150,7 @@ struct Engine : CtxBase {
     DevBuf<T> alphas;  // SpatialLLG damping per cell (empty: uniform alpha)
     DevBuf<uint8_t> pins;
     bool have_pins = false;
+    bool any_tfn = false;  // an add_torque term is present
     DevBuf<uint8_t> nbm;  // neighbour / pin mask, refreshed when Ms or the pins change
     // state: current spin, three stage buffers, seven k buffers (SoA 3N each)
     DevBuf<T> ybuf[4];
@@ -463,8 +466,31 @@ struct Engine : CtxBase {
     const T *a_ready = nullptr;
 
     // ---- one RHS evaluation with a fused epilogue; returns partial blocks --
+    // add_torque terms: the state of this evaluation goes to the host, each
+    // function writes its torque, which returns to the term's planes before the
+    // passes read it (field.jl:282-293 calls torque_fun on the same stream order)
+    std::vector<double> tfn_m, tfn_y;
+    void run_torque_fns(const T *m)
+    {
+        for (size_t q = 0; q < C->terms.size(); q++) {
+            const HostTerm &h = C->terms[q];
+            if (!h.tfn) continue;
+            if (tfn_m.empty()) {
+                tfn_m.resize(3 * N);
+                tfn_y.resize(3 * N);
+                download_aos(m, tfn_m.data());
+            }
+            std::fill(tfn_y.begin(), tfn_y.end(), 0.0);
+            h.tfn(t_eval, tfn_m.data(), tfn_y.data(), h.uuser);
+            upload_soa(tfn_y.data(), const_cast<T *>(tA[q]));
+            HIPCHECK(hipStreamSynchronize(C->stream));
+        }
+        tfn_m.clear();
+    }
+
     int eval(const T *m, const Epi<T> &E, bool generic = false)
     {
+        if (any_tfn) run_torque_fns(m);
         const FieldParams<T> *P = params_dev();
         const Dyn d = dyn();
         const int fast = generic ? 0 : Plast.fast;
@@ -1057,7 +1083,7 @@ struct Engine : CtxBase {
         for (const auto &h : C->terms)
             if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T ||
                 h.kind == K_HEX || h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.kind == K_SAHE || h.kind == K_MEL_T || h.kind == K_MEL_C ||
-                h.ufun)
+                h.kind == K_TORQUE_FN || h.ufun)
                 return fail(MMX_EINVAL, "the adjoint covers the time-independent exchange, DMI, uniaxial and cubic "
                                         "anisotropy, Zeeman and demag terms");
         if (nsteps < 1 || every < 1) return fail(MMX_EINVAL, "nsteps and every must be >= 1");
@@ -2198,6 +2224,22 @@ int mmx_add_hex_anis(mmx_ctx *c, double K1, double K2, double K3)
     });
 }
 
+int mmx_add_torque(mmx_ctx *c, void (*fn)(double t, const double *m, double *y, void *user), void *user)
+{
+    if (!c || !fn) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) {
+        using TT = std::remove_pointer_t<decltype(e.y)>;
+        HostTerm h;  // TorqueField(fun): add_field.jl:998-1006
+        h.kind = K_TORQUE_FN;
+        h.tfn = fn;
+        h.uuser = user;
+        std::vector<double> zero(3 * c->N, 0.0);
+        const TT *a = e.new_array(zero.data(), 3 * c->N, true);
+        e.any_tfn = true;
+        return add_term(c, e, h, a, nullptr, nullptr);
+    });
+}
+
 int mmx_add_magnetoelastic(mmx_ctx *c, int model, const double *data, double lambda_s, double B1, double B2)
 {
     if (!c || !data) return fail(MMX_EINVAL, "null argument");
@@ -2659,8 +2701,9 @@ int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n)
         if (c->ikind != members[0]->ikind) return fail(MMX_EINVAL, "members must use the same integrator");
         if (!(c->step > 0)) return fail(MMX_EINVAL, "member %d has no step size", i);
         for (const auto &h : c->terms)
-            if (h.ufun || h.ft3)
-                return fail(MMX_EINVAL, "member %d has a time-dependent term; batches take constant ones", i);
+            if (h.ufun || h.ft3 || h.tfn)
+                return fail(MMX_EINVAL, "member %d has a time-dependent or host-function term; batches take "
+                                        "constant ones", i);
         for (int j = 0; j < i; j++)
             if (members[j] == c) return fail(MMX_EINVAL, "member %d appears twice", i);
     }
@@ -2731,6 +2774,8 @@ int mmx_neb_create(mmx_neb **out, mmx_ctx *sim, const double *images, int n_free
     if (!out || !sim || !images) return fail(MMX_EINVAL, "null argument");
     *out = nullptr;
     if (n_free < 1) return fail(MMX_EINVAL, "The number of free images should larger than 1 (typically 10~20).");
+    for (const auto &h : sim->terms)
+        if (h.tfn) return fail(MMX_EINVAL, "the NEB band takes no add_torque term");
     if (!finite_all(images, (int64_t)(n_free + 2) * 3 * sim->N)) return fail(MMX_EINVAL, "NaN in the images");
     auto b = std::make_unique<mmx_neb>();
     b->sim = sim;

@@ -53,6 +53,7 @@ enum TermKind : int {
     K_DMI_IL,       // interlayer DMI: s = {Dx, Dy, Dz, dz, k1, k2} (layers 0-based)
     K_MEL_T,        // magnetoelastic, tensor model: a = 6 planes of stress (xx,yy,zz,xy,xz,yz), s[0] = lambda_s
     K_MEL_C,        // magnetoelastic, cubic model: a = 6 planes of strain, s = {B1, B2}
+    K_TORQUE_FN,    // add_torque: the torque T (3 planes in a) written by a host function per evaluation
     K_SAHE,         // spin anomalous Hall torque: a = 15 planes (sigma_s x a1, sigma_sa1 x a1, sigma_sa2, a2, a3), s[0] = beta
     K_TWIN,         // twin monoclinic anisotropy: s = {Ka, Kb, Kaa, Kbb, Kab, Ku}; axes a, b, u111 (3 planes each)
 };

@@ -369,6 +369,13 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
                 hz = nf * (B1 * qzz * mz + B2 * (qxz * mx + qyz * my));
             }
         } break;
+        case K_TORQUE_FN: {  // torque_kernel!: kernels.jl:639-651, H = (1/gamma) m x T
+            const T tx = D.a[I], ty = D.a[N + I], tz = D.a[2 * N + I];
+            const T a = T(1) / P.gamma;
+            hx = a * (-mz * ty + my * tz);
+            hy = a * (mz * tx - mx * tz);
+            hz = a * (-my * tx + mx * ty);
+        } break;
         case K_SAHE: {  // sahe_torque_kernel!: kernels.jl:589-613 (no energy)
             if (Ms == T(0)) break;
             const T *Q = D.a;  // plane f*3 + c of the packed vector fields

@@ -168,6 +168,20 @@ function attach_sahe!(sim, sigma_s, sigma_sa1, sigma_sa2, a1, a2, a3, beta)
           (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
            Cdouble), h, vec(sigma_s), vec(sigma_sa1), vec(sigma_sa2), vec(a1), vec(a2), vec(a3), beta) |> check
 end
+function add_term!(h, t::MicroMagnetic.TorqueField)
+    # torque_fun(y, m, t) on host arrays wrapping libmmx's buffers (3N AoS each)
+    fun = t.torque_fun
+    n = 3 * length(t.field) ÷ 3
+    f = (s::Cdouble, m::Ptr{Cdouble}, y::Ptr{Cdouble}, user::Ptr{Cvoid}) -> begin
+        fun(unsafe_wrap(Array, y, n), unsafe_wrap(Array, m, n), s)
+        nothing
+    end
+    cf = @cfunction($f, Cvoid, (Cdouble, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cvoid}))
+    idx = ccall((:mmx_add_torque, libmmx), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), h, cf, C_NULL)
+    check(idx)
+    timefns[(h, idx)] = cf
+    return idx
+end
 function add_term!(h, t::MicroMagnetic.Magnetoelastic)
     # field_data: the 6N Voigt values, cell by cell (add_field.jl:1413-1435)
     model = t.model == :tensor ? Cint(0) : Cint(1)

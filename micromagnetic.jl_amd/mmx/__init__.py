@@ -364,6 +364,28 @@ def add_cubic_anis(sim, Kc, axis1=(1, 0, 0), axis2=(0, 1, 0), name="cubic"):
     return sim._add(L.mmx_add_cubic_anis(sim._h, dptr(a), float(Kc) if uniform else 0.0, dptr(a1), dptr(a2)), name)
 
 
+_TORQUEFN = None
+
+
+def add_torque(sim, fun, name="torque"):
+    """add_torque(sim, fun; name): src/micro/add_field.jl:998-1006 -- fun(y, m, t) fills the
+    torque y for the state m (numpy views, 3N AoS) at every evaluation time; the term is
+    H = (1/gamma) m x y (torque_kernel!, kernels.jl:639-651).  No table column."""
+    global _TORQUEFN
+    if _TORQUEFN is None:
+        _TORQUEFN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+    n = 3 * sim.n_total
+
+    def cb(t, m, y, user):
+        fun(np.ctypeslib.as_array(y, shape=(n,)), np.ctypeslib.as_array(m, shape=(n,)), t)
+
+    f = _TORQUEFN(cb)
+    t = sim._add(_lib.load().mmx_add_torque(sim._h, C.cast(f, C.c_void_p), None), name)
+    t._ft = f  # the library keeps only the pointer
+    t.energy_unit = None
+    return t
+
+
 def _six_component(mesh, data):
     """_init_six_component_field: src/micro/add_field.jl:1413-1435 -- a (6, nx, ny, nz)
     array (Fortran order, as Julia holds it) or f(i,j,k,dx,dy,dz) / f(x,y,z) with

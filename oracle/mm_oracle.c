@@ -47,7 +47,8 @@ enum {
     T_TWIN,
     T_SAHE,
     T_MEL_T,
-    T_MEL_C
+    T_MEL_C,
+    T_TORQUE_FN
 };
 
 typedef struct {
@@ -71,6 +72,8 @@ typedef struct {
     void *uuser;
     /* time Zeeman factors (fx, fy, fz)(t) */
     void (*ft3)(double, double *, void *);
+    /* add_torque: fn(t, m, y, user) writes the torque y for the state m (3N AoS) */
+    void (*tfn)(double, const double *, double *, void *);
 } term_t;
 
 enum { I_RK4 = 0, I_HEUN = 1, I_DOPRI54 = 2, I_BS23 = 3 };
@@ -1390,6 +1393,33 @@ static void field_hex(orc_sim *s, term_t *t, const double *m)
     }
 }
 
+/* add_torque(sim, fun): src/micro/add_field.jl:998-1006 */
+int orc_add_torque(orc_sim *s, void (*fn)(double, const double *, double *, void *), void *user)
+{
+    term_t *t = new_term(s, T_TORQUE_FN);
+    t->tfn = fn;
+    t->uuser = user;
+    t->a = (double *)calloc(3 * (size_t)s->N, sizeof(double));
+    return s->nterms - 1;
+}
+
+/* TorqueField: field.jl:282-293 then torque_kernel! src/micro/kernels.jl:639-651 */
+static void field_torque_fn(orc_sim *s, term_t *t, const double *m)
+{
+    double *y = t->a;
+    memset(y, 0, 3 * (size_t)s->N * sizeof(double));
+    t->tfn(s->t_eval, m, y, t->uuser);
+    const double a = 1 / s->gamma;
+    for (int I = 0; I < s->N; I++) {
+        size_t j = 3 * (size_t)I;
+        const double mx = m[j], my = m[j + 1], mz = m[j + 2], hx = y[j], hy = y[j + 1], hz = y[j + 2];
+        t->field[j] = a * cross_x(mx, my, mz, hx, hy, hz);
+        t->field[j + 1] = a * cross_y(mx, my, mz, hx, hy, hz);
+        t->field[j + 2] = a * cross_z(mx, my, mz, hx, hy, hz);
+        t->energy[I] = 0;
+    }
+}
+
 /* add_magnetoelastic(sim; model, lambda_s, B1, B2, sigma / strain): src/micro/add_field.jl:1367-1405;
  * data: 6N values in Voigt order per cell (the reference's (6, nx, ny, nz) array) */
 int orc_add_magnetoelastic(orc_sim *s, int cubic, const double *data, double lambda_s, double B1, double B2)
@@ -1685,6 +1715,7 @@ static void term_eval(orc_sim *s, term_t *t, const double *m)
     case T_SAHE: field_sahe(s, t, m); break;
     case T_MEL_T:
     case T_MEL_C: field_mel(s, t, m); break;
+    case T_TORQUE_FN: field_torque_fn(s, t, m); break;
     }
 }
 

@@ -54,6 +54,7 @@ int orc_add_stt_slonczewski(orc_sim *s, const double *J, double beta, double Lam
 int orc_add_stt_zhangli(orc_sim *s, const double *J /* 3N */, double b, const double *xi, int xi_factor);
 int orc_set_term_time_fn(orc_sim *s, int term, double (*fn)(double, void *), void *user);
 void orc_set_alpha(orc_sim *s, const double *alpha /* N or NULL */);
+int orc_add_torque(orc_sim *s, void (*fn)(double, const double *, double *, void *), void *user);
 int orc_add_magnetoelastic(orc_sim *s, int cubic, const double *data /* 6N */, double lambda_s, double B1,
                            double B2);
 int orc_add_sahe_torque(orc_sim *s, const double *sigma_s, const double *sigma_sa1, const double *sigma_sa2,

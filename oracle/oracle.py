@@ -80,6 +80,7 @@ def lib():
             "orc_add_twin_mono_anis": [C.c_void_p, _dp, _dp, _dp, _dp],
             "orc_add_sahe_torque": [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double],
             "orc_add_magnetoelastic": [C.c_void_p, C.c_int, _dp, C.c_double, C.c_double, C.c_double],
+            "orc_add_torque": [C.c_void_p, C.c_void_p, C.c_void_p],
             "orc_add_dmi_int": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
             "orc_num_terms": [C.c_void_p],
             "orc_demag_tensor": [C.c_void_p, C.c_int, _dp],
@@ -95,6 +96,7 @@ def lib():
 
 _TIMEFN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
 _TIMEFN3 = C.CFUNCTYPE(None, C.c_double, _dp, C.c_void_p)
+_TORQUEFN = C.CFUNCTYPE(None, C.c_double, _dp, _dp, C.c_void_p)
 
 
 def _factor3(ft):
@@ -205,6 +207,17 @@ class OracleSim:
         else:
             Dx = Dy = Dz = _f64(D, self.N)
         return lib().orc_add_dmi_bulk_spatial(self.h, _p(Dx), _p(Dy), _p(Dz))
+
+    def add_torque(self, fun):
+        """add_torque(sim, fun): src/micro/add_field.jl:998-1006; fun(y, m, t) fills y (3N AoS)."""
+        n = 3 * self.N
+
+        def cb(t, m, y, user):
+            fun(np.ctypeslib.as_array(y, shape=(n,)), np.ctypeslib.as_array(m, shape=(n,)), t)
+
+        f = _TORQUEFN(cb)
+        self._keep = getattr(self, "_keep", []) + [f]
+        return lib().orc_add_torque(self.h, C.cast(f, C.c_void_p), None)
 
     def add_magnetoelastic(self, model="tensor", lambda_s=0.0, B1=0.0, B2=0.0, data=None):
         """add_magnetoelastic: src/micro/add_field.jl:1367-1405; data (6N, Voigt per cell)."""
