@@ -171,7 +171,7 @@ end
 function add_term!(h, t::MicroMagnetic.TorqueField)
     # torque_fun(y, m, t) on host arrays wrapping libmmx's buffers (3N AoS each)
     fun = t.torque_fun
-    n = 3 * length(t.field) ÷ 3
+    n = length(t.field)  # 3N
     f = (s::Cdouble, m::Ptr{Cdouble}, y::Ptr{Cdouble}, user::Ptr{Cvoid}) -> begin
         fun(unsafe_wrap(Array, y, n), unsafe_wrap(Array, m, n), s)
         nothing
