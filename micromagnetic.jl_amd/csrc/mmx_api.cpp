@@ -315,6 +315,7 @@ struct Engine : CtxBase {
             case K_ANIS: case K_ANIS_S: sk = SL_ANIS; break;
             case K_CUBIC: sk = SL_CUBIC; break;
             case K_ZEEMAN: case K_ZEEMAN_U: sk = SL_ZEE; break;
+            case K_ZEEMAN_T: sk = C->terms[i].uniform ? SL_ZEE : -1; break;  // factors in Dyn
             case K_DEMAG: sk = SL_DEMAG; break;
             case K_THERMAL: sk = SL_THERM; break;
             default: sk = -1; break;  // torque terms: generic kernels only
@@ -401,6 +402,7 @@ struct Engine : CtxBase {
                 d.tfz[i] = (double)((T)(u * h.s[8]) / (T)h.s[11]);
             }
             if (h.kind == K_DMI_BULK_S && h.ufun) d.tfac[i] = u;  // time_D(t) .* D, field.jl:185
+            if (h.kind == K_ZEEMAN_U) d.tfac[i] = d.tfy[i] = d.tfz[i] = 1.0;  // the slot form scales by these
             if (h.kind == K_ZEEMAN_T) {                         // time_fun(t), field.jl:16-26
                 double f[3] = {1, 1, 1};
                 if (h.ft3) h.ft3(t_eval, f, h.uuser);

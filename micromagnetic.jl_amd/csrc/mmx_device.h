@@ -660,6 +660,11 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
         zhx = C.zx;
         zhy = C.zy;
         zhz = C.zz;
+        if (!SIG) {  // a uniform TimeZeeman takes this form too: H0 .* f(t) (static: factors 1)
+            zhx = zhx * (T)dyn.tfac[C.sz];
+            zhy = zhy * (T)dyn.tfy[C.sz];
+            zhz = zhz * (T)dyn.tfz[C.sz];
+        }
     } else if (ze == 2) {
         zhx = gld(P.t[C.sz].a, I);
         zhy = gld(P.t[C.sz].b, I);

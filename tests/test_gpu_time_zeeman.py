@@ -214,3 +214,47 @@ def test_table_columns_of_zeeman_terms(tmp_path):
     assert np.isclose(data["tz_Hz"][-1], 5e4 * f[2], rtol=1e-12, atol=0)
     assert np.isclose(data["E_tz"][-1], E[2], rtol=1e-11, atol=0)
     assert np.isclose(data["E_total"][-1], E.sum(), rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_uniform_time_zeeman_slot_form(oracle, precision):
+    """One uniform time Zeeman term among the film's terms takes the specialised
+    (slot-form) kernels with stage chaining: fields at several times and an RK4
+    trajectory against the oracle."""
+    note(f"uniform time Zeeman in the slot form FP{precision}")
+    nx, ny, nz = 16, 12, 4
+    N = nx * ny * nz
+    rng = np.random.default_rng(8)
+    m0 = rng.standard_normal(3 * N)
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz, pbc="xy")
+    g = mmx.Sim(mesh, precision=precision)
+    o = oracle.OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9, pbc="xy")
+    mmx.set_Ms(g, 5.8e5)
+    o.set_Ms(5.8e5)
+    mmx.init_m0(g, m0)
+    o.init_m0(m0)
+    g.driver.alpha, g.driver.gamma = 0.1, 2.21e5
+    o.set_llg(0.1, 2.21e5, True)
+    mmx.add_exch(g, 1.5e-11)
+    o.add_exch(1.5e-11)
+    mmx.add_dmi(g, 3e-3, type="interfacial")
+    o.add_dmi(np.full(N, 3e-3), type="interfacial")
+    mmx.add_anis(g, 8e5, axis=(0, 0, 1))
+    o.add_anis(8e5, (0, 0, 1))
+    mmx.add_zeeman(g, (1e4, -2e4, 8e4), rotating)
+    o.add_zeeman_time((1e4, -2e4, 8e4), rotating)
+    mmx.add_demag(g, Nx=1, Ny=1)
+    o.add_demag(Nx=1, Ny=1)
+    tol = 1e-12 if precision == 64 else 1e-5
+    for t in (0.0, 4.4e-13):
+        assert relerr(mmx.effective_field(g, t=t), o.effective_field(t=t)) <= tol
+        assert relerr(mmx.llg_rhs(g, t=t), o.llg_rhs(t=t)) <= tol
+    if precision == 64:
+        mmx.set_driver(g, integrator="RungeKutta", step=2e-14)
+        o.set_integrator(0, 2e-14, 0)
+        for _ in range(30):
+            o.advance_step()
+        mmx.run_step(g, n=30)
+        err = np.max(np.abs(g.spin - o.spin))
+        print(f"max |dm| after 30 steps: {err:.2e}")
+        assert err < 1e-12
