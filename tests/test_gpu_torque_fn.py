@@ -103,3 +103,5 @@ def test_torque_fn_precession_and_refusals():
         mmx.Batch([sim])
     with pytest.raises(mmx.MmxError):
         mmx.rk4_trajectory_rrule(sim, 4, every=2)
+    with pytest.raises(mmx.MmxError):  # the band's fields are not evaluated per stage state
+        mmx.NEB(sim, [(1, 0, 0), (0, 0, 1)], [3])
