@@ -267,6 +267,11 @@ int mmx_synchronize(mmx_ctx *ctx);
 #define MMX_PASS_P5 4    /* x inverse + fused local field, LLG and stage update */
 #define MMX_PASS_LOCAL 5 /* local field + LLG + stage update (no demag) */
 #define MMX_NPASS 6
+/* diagnostic (no counterpart in the reference): a streaming copy of `bytes` between two
+ * fresh device buffers on `device`, timed with HIP events over `reps` launches of `grid`
+ * blocks after one warm-up launch; the practical HBM ceiling the passes are compared with
+ * (read + write = 2 * bytes per copy). */
+int mmx_diag_copy(int device, int64_t bytes, int reps, int grid, double *ms_per_copy);
 int mmx_profile(mmx_ctx *ctx, int enable);
 int mmx_profile_read(mmx_ctx *ctx, double *ms /* MMX_NPASS */, int64_t *launches /* MMX_NPASS */);
 

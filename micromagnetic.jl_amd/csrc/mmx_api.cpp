@@ -2622,6 +2622,43 @@ int mmx_synchronize(mmx_ctx *c)
     return MMX_OK;
 }
 
+int mmx_diag_copy(int device, int64_t bytes, int reps, int grid, double *ms_per_copy)
+{
+    if (bytes < 16 || reps < 1 || grid < 1 || !ms_per_copy) return fail(MMX_EINVAL, "invalid copy probe arguments");
+    const int64_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MMX_OK;
+    try {
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreate(&s));
+        HIPCHECK(hipMalloc(&a, n16 * 16));
+        HIPCHECK(hipMalloc(&b, n16 * 16));
+        HIPCHECK(hipMemsetAsync(a, 0, n16 * 16, s));
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        launch_stream_copy(a, b, n16, grid, s);  // warm-up
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) launch_stream_copy(a, b, n16, grid, s);
+        HIPCHECK(hipEventRecord(e1, s));
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / reps;
+    } catch (const HipError &e) {
+        rc = e.code;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
+
 int mmx_profile(mmx_ctx *c, int enable)
 {
     return with_engine(c, [&](auto &e) {

@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ 
 }
 
 // the thermal noise of every member moves to its next integrator step
-__global__ void k_batch_tick(Dyn *__restrict__ dyns, int M)
+static __global__ void k_batch_tick(Dyn *__restrict__ dyns, int M)
 {
     for (int m = blockIdx.x * BLOCK + threadIdx.x; m < M; m += gridDim.x * BLOCK) dyns[m].noise_step += 1;
 }
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(BLOCK) k_neb_field(const FieldParams<T> *__res
 }
 
 // per-segment sums: out[q] = sum_b partial[q*nb + b]
-__global__ void k_segment_sum(const double *__restrict__ partial, int nb, double *__restrict__ out)
+static __global__ void k_segment_sum(const double *__restrict__ partial, int nb, double *__restrict__ out)
 {
     __shared__ double s[BLOCK];
     const int q = blockIdx.x;
@@ -618,7 +618,7 @@ __global__ void k_neb_tangent(int64_t N, int n, const T *__restrict__ Ys, const 
 // per image: reduce the block partials; coef[q] = {1/|t|, s_q}, where the NEB
 // force is f + s_q t/|t| with s_q = k (d[q+1] - d[q]) - f.t/|t| (or -2 f.t/|t|
 // for the climbing image), src/neb/neb.jl:196-216
-__global__ void k_neb_coef(const double *__restrict__ partial, int nb, const double *__restrict__ dist, double k,
+static __global__ void k_neb_coef(const double *__restrict__ partial, int nb, const double *__restrict__ dist, double k,
                            int climb, double *__restrict__ coef)
 {
     __shared__ double s0[BLOCK], s1[BLOCK];
@@ -647,7 +647,7 @@ __global__ void k_neb_coef(const double *__restrict__ partial, int nb, const dou
 }
 
 // per image pair: distance = sqrt(sum of block partials)
-__global__ void k_neb_dist_final(const double *__restrict__ partial, int nb, double *__restrict__ dist)
+static __global__ void k_neb_dist_final(const double *__restrict__ partial, int nb, double *__restrict__ dist)
 {
     __shared__ double s[BLOCK];
     const int q = blockIdx.x;
@@ -801,7 +801,7 @@ __global__ void k_sd_g(int64_t N, const T *__restrict__ m, const T *__restrict__
 }
 
 // compute_tau (sd.jl:39-66) from the three sums; one thread
-__global__ void k_sd_tau(const double *__restrict__ sums, int odd, double min_tau, double max_tau,
+static __global__ void k_sd_tau(const double *__restrict__ sums, int odd, double min_tau, double max_tau,
                          double *__restrict__ tau)
 {
     const double s1 = sums[0], s2 = sums[1], s3 = sums[2];
@@ -960,7 +960,7 @@ __global__ void k_to_double(const T *__restrict__ src, double *__restrict__ dst,
 
 // padded real-space tensor component (mirror of fill_tensors_kernel!,
 // src/micro/demag.jl:394-411) on the power-of-two padded grid
-__global__ void k_tensor_pad(DemagGeom G, const double *__restrict__ rt, int comp, double *__restrict__ pad)
+static __global__ void k_tensor_pad(DemagGeom G, const double *__restrict__ rt, int comp, double *__restrict__ pad)
 {
     const int64_t P = (int64_t)G.Px * G.Py * G.Pz;
     const int64_t N = (int64_t)G.nx * G.ny * G.nz;
@@ -1013,7 +1013,7 @@ __global__ void k_tensor_extract(DemagGeom G, const cplx<double> *__restrict__ B
     }
 }
 
-__global__ void k_reduce(const double *__restrict__ partial, int nblocks, int stride, int nval, int is_max,
+static __global__ void k_reduce(const double *__restrict__ partial, int nblocks, int stride, int nval, int is_max,
                          double *__restrict__ out)
 {
     __shared__ double s[BLOCK];
@@ -1074,7 +1074,35 @@ static int grid_for(int64_t n, int cap = 8192)
     return (int)(g > cap ? cap : g);
 }
 
+#ifdef MMX_KT_MAIN
 int max_local_blocks() { return 8192; }
+#endif
+
+#ifdef MMX_KT_MAIN
+// Streaming copy of n 16-byte words (the practical HBM ceiling the passes are
+// compared with): grid-stride loop, four independent 16-byte loads in flight
+// per thread before their stores.
+static __global__ void __launch_bounds__(BLOCK) k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                               int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_stream_copy(const void *src, void *dst, int64_t n16, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BLOCK), 0, s, (const uint4 *)src, (uint4 *)dst, n16);
+}
+#endif
+
 
 template <class T>
 void launch_aos_to_soa(const double *aos, T *soa, int64_t N, hipStream_t s)
@@ -1112,22 +1140,6 @@ int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T
     return g;
 }
 
-#define MMX_LG_SWITCH(lg, CALL)                                                 \
-    switch (lg) {                                                              \
-    case 0: CALL(1); break;                                                    \
-    case 1: CALL(2); break;                                                    \
-    case 2: CALL(4); break;                                                    \
-    case 3: CALL(8); break;                                                    \
-    case 4: CALL(16); break;                                                   \
-    case 5: CALL(32); break;                                                   \
-    case 6: CALL(64); break;                                                   \
-    case 7: CALL(128); break;                                                  \
-    case 8: CALL(256); break;                                                  \
-    case 9: CALL(512); break;                                                  \
-    case 10: CALL(1024); break;                                                \
-    case 11: CALL(2048); break;                                                \
-    default: break;                                                            \
-    }
 
 template <class T>
 void launch_p1(const DemagGeom &G, const T *in, int64_t cstride, const T *scale, int n_in, int Ry, int Rz,
@@ -1163,12 +1175,15 @@ void launch_coly(const DemagGeom &G, const cplx<T> *in, cplx<T> *out, int n_in, 
 #undef CY_CALL
 }
 
+#ifdef MMX_KT_MAIN
 int convz_grouped_tensor(const DemagGeom &G, int elsize)
 {
     const int e = elsize == 8 ? P3W<double>::E : P3W<float>::E;
     const int ept = G.Pz < e ? G.Pz : e;
     return (MMX_P3_WAVE && G.Pz / ept <= 64) ? 1 : 0;
 }
+#endif
+
 
 template <class T>
 void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, int nzb, int n_in, int ncomp,
@@ -1195,49 +1210,6 @@ void launch_convz(const DemagGeom &G, cplx<T> *B, const T *Kt, const T *Lline, i
 #undef CZ_CALL
 }
 
-template <class T>
-int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
-              const T *m, int fast, uint32_t sig, const cplx<T> *tw, hipStream_t s)
-{
-    if (fast && sig) {
-        const int gs = launch_p5_sig<T>(G, A, P, dyn, E, m, sig, tw, s);
-        if (gs >= 0) return gs;
-    }
-    int g = 0;
-#define P5_CALL(H)                                                                                   \
-    {                                                                                                \
-        g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;                         \
-        const int v = variant(E.mode, fast, E.np);                                                   \
-        if (v == 4)                                                                                  \
-            hipLaunchKernelGGL((k_p5<T, H, 4>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
-        else if (v == 1)                                                                             \
-            hipLaunchKernelGGL((k_p5<T, H, 1>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
-        else if (v == 2)                                                                             \
-            hipLaunchKernelGGL((k_p5<T, H, 2>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
-        else if (v == 3)                                                                             \
-            hipLaunchKernelGGL((k_p5<T, H, 3>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
-        else                                                                                         \
-            hipLaunchKernelGGL((k_p5<T, H, 0>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
-    }
-    MMX_LG_SWITCH(G.lgH, P5_CALL)
-#undef P5_CALL
-    return g;
-}
-
-template <class T>
-void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn,
-                      const Epi<T> &E, const T *m, int nimg, const cplx<T> *tw, hipStream_t s)
-{
-#define P5I_CALL(H)                                                                                  \
-    {                                                                                                \
-        const int g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;               \
-        hipLaunchKernelGGL((k_p5<T, H, 0, 0u, true>), dim3(g, nimg), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, \
-                           tw);                                                                      \
-    }
-    MMX_LG_SWITCH(G.lgH, P5I_CALL)
-#undef P5I_CALL
-}
-
 // sums of n stacked vectors of N values: partial[q*gridDim.x + b], then out[q]
 template <class T>
 __global__ void k_sum_images(const T *__restrict__ v, int64_t N, double *__restrict__ partial)
@@ -1256,6 +1228,7 @@ void launch_sum_images(const T *v, int64_t N, int n, int nb, double *partial, do
     hipLaunchKernelGGL(k_segment_sum, dim3(n), dim3(BLOCK), 0, s, partial, nb, out);
 }
 
+#ifdef MMX_KT_MAIN
 int p5_blocks(const DemagGeom &G)
 {
     int rx = 1;
@@ -1264,12 +1237,17 @@ int p5_blocks(const DemagGeom &G)
 #undef RX_CALL
     return (G.ny * ((G.nz + rx - 1) / rx) + 7) / 8 * 8;
 }
+#endif
 
+
+#ifdef MMX_KT_MAIN
 void launch_tensor_pad(const DemagGeom &G, const double *rt, int comp, double *pad, hipStream_t s)
 {
     int64_t P = (int64_t)G.Px * G.Py * G.Pz;
     hipLaunchKernelGGL(k_tensor_pad, dim3(grid_for(P)), dim3(BLOCK), 0, s, G, rt, comp, pad);
 }
+#endif
+
 
 template <class T>
 void launch_tensor_extract(const DemagGeom &G, const cplx<double> *Bt, int comp, double scale, T *Kt,
@@ -1279,11 +1257,14 @@ void launch_tensor_extract(const DemagGeom &G, const cplx<double> *Bt, int comp,
     hipLaunchKernelGGL(k_tensor_extract<T>, dim3(grid_for(n)), dim3(BLOCK), 0, s, G, Bt, comp, scale, Kt);
 }
 
+#ifdef MMX_KT_MAIN
 void launch_reduce(const double *partial, int nblocks, int stride, int nval, int is_max, double *out,
                    hipStream_t s)
 {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, partial, nblocks, stride, nval, is_max, out);
 }
+#endif
+
 
 template <class T>
 int launch_maxdiff(const T *a, const T *b, int64_t N, double *partial, hipStream_t s)
@@ -1338,12 +1319,18 @@ void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *con
 #undef BL_CALL
 }
 
+#ifdef MMX_KT_MAIN
 void launch_batch_tick(Dyn *dyns, int M, hipStream_t s)
 {
     hipLaunchKernelGGL(k_batch_tick, dim3((M + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, dyns, M);
 }
+#endif
 
+
+#ifdef MMX_KT_MAIN
 int batch_variant(int mode, int fast, int np) { return variant(mode, fast, np); }
+#endif
+
 
 template <class T>
 int launch_sum(const T *v, int64_t n, double *partial, hipStream_t s)
@@ -1412,10 +1399,13 @@ int launch_sd_g(int64_t N, const T *m, const T *mpre, const T *h, T *g, double *
     return gr;
 }
 
+#ifdef MMX_KT_MAIN
 void launch_sd_tau(const double *sums, int odd, double min_tau, double max_tau, double *tau, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sd_tau, dim3(1), dim3(1), 0, s, sums, odd, min_tau, max_tau, tau);
 }
+#endif
+
 
 template <class T>
 void launch_sd_update(int64_t N, const FieldParams<T> *P, const T *g, T *m, T *mpre, const T *h, const double *tau,
@@ -1443,11 +1433,7 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
                                  int, const cplx<T> *, hipStream_t);                                         \
     template void launch_convz<T>(const DemagGeom &, cplx<T> *, const T *, const T *, int, int, int, int,      \
                                   const cplx<T> *, hipStream_t, int);                                                         \
-    template void launch_p5_images<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,  \
-                                      const Epi<T> &, const T *, int, const cplx<T> *, hipStream_t);          \
     template void launch_sum_images<T>(const T *, int64_t, int, int, double *, double *, hipStream_t);        \
-    template int launch_p5<T>(const DemagGeom &, const cplx<T> *, const FieldParams<T> *, const Dyn &,        \
-                              const Epi<T> &, const T *, int, uint32_t, const cplx<T> *, hipStream_t);                 \
     template void launch_tensor_extract<T>(const DemagGeom &, const cplx<double> *, int, double, T *,         \
                                            hipStream_t);                                                     \
     template int launch_maxdiff<T>(const T *, const T *, int64_t, double *, hipStream_t);          \
@@ -1476,7 +1462,9 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
     template void launch_adj_final<T>(int64_t, const T *, const T *const[4], double, const T *, T *,          \
                                       T *const[4], hipStream_t);
 
-MMX_INST(double)
-MMX_INST(float)
+#ifndef MMX_KT
+#error "compile once per precision with -DMMX_KT=double or -DMMX_KT=float"
+#endif
+MMX_INST(MMX_KT)
 
 }  // namespace mmx

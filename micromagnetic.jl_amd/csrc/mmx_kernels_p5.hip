@@ -1,0 +1,65 @@
+// mmx_kernels_p5.hip -- launchers of the last demag pass (k_p5, mmx_pass.h) for
+// every transform length and epilogue variant.  Compiled once per precision
+// (-DMMX_KT=double / float) so the instantiations build in parallel.
+#include <hip/hip_runtime.h>
+
+#include "mmx_common.h"
+#include "mmx_kernels.h"
+#include "mmx_pass.h"
+
+#ifndef MMX_KT
+#error "compile once per precision with -DMMX_KT=double or -DMMX_KT=float"
+#endif
+
+namespace mmx {
+
+template <class T>
+int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
+              const T *m, int fast, uint32_t sig, const cplx<T> *tw, hipStream_t s)
+{
+    if (fast && sig) {
+        const int gs = launch_p5_sig<T>(G, A, P, dyn, E, m, sig, tw, s);
+        if (gs >= 0) return gs;
+    }
+    int g = 0;
+#define P5_CALL(H)                                                                                   \
+    {                                                                                                \
+        g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;                         \
+        const int v = variant(E.mode, fast, E.np);                                                   \
+        if (v == 4)                                                                                  \
+            hipLaunchKernelGGL((k_p5<T, H, 4>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 1)                                                                             \
+            hipLaunchKernelGGL((k_p5<T, H, 1>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 2)                                                                             \
+            hipLaunchKernelGGL((k_p5<T, H, 2>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 3)                                                                             \
+            hipLaunchKernelGGL((k_p5<T, H, 3>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_p5<T, H, 0>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+    }
+    MMX_LG_SWITCH(G.lgH, P5_CALL)
+#undef P5_CALL
+    return g;
+}
+
+template <class T>
+void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn,
+                      const Epi<T> &E, const T *m, int nimg, const cplx<T> *tw, hipStream_t s)
+{
+#define P5I_CALL(H)                                                                                  \
+    {                                                                                                \
+        const int g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;               \
+        hipLaunchKernelGGL((k_p5<T, H, 0, 0u, true>), dim3(g, nimg), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, \
+                           tw);                                                                      \
+    }
+    MMX_LG_SWITCH(G.lgH, P5I_CALL)
+#undef P5I_CALL
+}
+
+template int launch_p5<MMX_KT>(const DemagGeom &, const cplx<MMX_KT> *, const FieldParams<MMX_KT> *, const Dyn &,
+                               const Epi<MMX_KT> &, const MMX_KT *, int, uint32_t, const cplx<MMX_KT> *, hipStream_t);
+template void launch_p5_images<MMX_KT>(const DemagGeom &, const cplx<MMX_KT> *, const FieldParams<MMX_KT> *,
+                                       const Dyn &, const Epi<MMX_KT> &, const MMX_KT *, int, const cplx<MMX_KT> *,
+                                       hipStream_t);
+
+}  // namespace mmx

@@ -79,7 +79,9 @@ int launch_local_sig(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const E
     template int launch_local_sig<T>(int64_t, const FieldParams<T> *, const Dyn &, const Epi<T> &, const T *, \
                                      uint32_t, int, hipStream_t);
 
-MMX_INST_SIG(double)
-MMX_INST_SIG(float)
+#ifndef MMX_KT
+#error "compile once per precision with -DMMX_KT=double or -DMMX_KT=float"
+#endif
+MMX_INST_SIG(MMX_KT)
 
 }  // namespace mmx

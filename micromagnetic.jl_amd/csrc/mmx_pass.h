@@ -273,6 +273,24 @@ __global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restric
     if (E.mode == EPI_ERR || E.mode == EPI_ENERGY) block_reduce_store<1>(&red, 1, E.mode == EPI_ERR, E.partial, 1);
 }
 
+// dispatch of a run-time log2 length to a compile-time length
+#define MMX_LG_SWITCH(lg, CALL)                                                 \
+    switch (lg) {                                                              \
+    case 0: CALL(1); break;                                                    \
+    case 1: CALL(2); break;                                                    \
+    case 2: CALL(4); break;                                                    \
+    case 3: CALL(8); break;                                                    \
+    case 4: CALL(16); break;                                                   \
+    case 5: CALL(32); break;                                                   \
+    case 6: CALL(64); break;                                                   \
+    case 7: CALL(128); break;                                                  \
+    case 8: CALL(256); break;                                                  \
+    case 9: CALL(512); break;                                                  \
+    case 10: CALL(1024); break;                                                \
+    case 11: CALL(2048); break;                                                \
+    default: break;                                                            \
+    }
+
 // epilogue variant for a mode: the slot-form kernels take the integrator stages
 inline int variant(int mode, int fast, int np)
 {

@@ -83,6 +83,7 @@ SIGNATURES = {
     "mmx_synchronize": [_vp],
     "mmx_profile": [_vp, C.c_int],
     "mmx_profile_read": [_vp, _dp, _i64p],
+    "mmx_diag_copy": [C.c_int, C.c_int64, C.c_int, C.c_int, _dp],
     "mmx_set_term_param": [_vp, C.c_int, _dp],
     "mmx_adjoint_record": [_vp, C.c_int64, C.c_int64],
     "mmx_adjoint_backward": [_vp, _dp, _dp],
