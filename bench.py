@@ -488,6 +488,26 @@ def rocfft_compare(nx, ny, nz, reps=20):
             "what": "fill + 3 rfftn + 6-component tensor product + 3 irfftn + collect, FP64"}
 
 
+def rocfft_child(mesh):
+    """rocfft_compare in a child process: torch brings its own HIP runtime,
+    which does not share the device with the library's in one process."""
+    import subprocess
+    import tempfile
+
+    fd, path = tempfile.mkstemp(suffix=".json", prefix="mmx_rocfft_")
+    os.close(fd)
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--rocfft-child", path, "--mesh", mesh],
+                           timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"rocFFT child exited with status {r.returncode}")
+        with open(path) as f:
+            return json.load(f)
+    finally:
+        if os.path.exists(path):
+            os.unlink(path)
+
+
 def gpu_soft_film(mmx, t_end=1e-9):
     """The 200x50x1 soft film (exchange + demag, alpha 0.5, gamma 2.211e5) run
     0 -> t_end with Dormand-Prince, tol 1e-6, on the GPU."""
@@ -572,11 +592,18 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the library's streaming copy of 1 GiB three times (calibrates HBM counters)")
     ap.add_argument("--forms-child", type=str, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--rocfft-child", type=str, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the path is single-GPU; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.rocfft_child:  # torch's HIP runtime in a process of its own (libmmx not loaded)
+        nx, ny, nz = (int(v) for v in args.mesh.split("x"))
+        with open(args.rocfft_child, "w") as f:
+            json.dump(rocfft_compare(nx, ny, nz), f)
+        return
 
     import mmx
 
@@ -660,7 +687,7 @@ def main():
     extras = {}
     if not args.no_extras:
         for name, fn in (("copy_ceiling", lambda: copy_ceiling(mmx)),
-                         ("rocfft_demag", lambda: rocfft_compare(nx, ny, nz)),
+                         ("rocfft_demag", lambda: rocfft_child(args.mesh)),
                          ("soft_film_200x50x1_1ns", lambda: gpu_soft_film(mmx)),
                          ("soft_film_200x50x1_sample", lambda: gpu_soft_film(mmx, SOFT_SAMPLE_T))):
             try:

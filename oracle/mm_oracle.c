@@ -1184,6 +1184,14 @@ static double philox_normal(uint64_t seed, uint64_t step, uint64_t idx)
     return (idx & 1) ? r * sin(2.0 * M_PI * u2) : r * cos(2.0 * M_PI * u2);
 }
 
+/* the raw block, for the known-answer checks of the round function */
+void orc_philox4x32_10(const uint32_t *ctr_in, const uint32_t *key, uint32_t *out)
+{
+    uint32_t c[4] = {ctr_in[0], ctr_in[1], ctr_in[2], ctr_in[3]};
+    philox4x32_10(c, key);
+    for (int q = 0; q < 4; q++) out[q] = c[q];
+}
+
 void orc_philox_normal(uint64_t seed, uint64_t step, uint64_t n, double *out)
 {
     for (uint64_t i = 0; i < n; i++) out[i] = philox_normal(seed, step, i);

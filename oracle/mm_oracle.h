@@ -93,6 +93,7 @@ int orc_relax(orc_sim *s, int max_steps, double stopping_dmdt, int *steps_done,
 void orc_fft(double *data /* interleaved re,im */, int n, int sign);
 
 /* counter-based normal variates used for the thermal field */
+void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 void orc_philox_normal(uint64_t seed, uint64_t step, uint64_t n, double *out);
 
 #ifdef __cplusplus

@@ -52,6 +52,7 @@ def lib():
             "orc_integrator_state": [C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_void_p],
             "orc_fft": [_dp, C.c_int, C.c_int],
             "orc_philox_normal": [C.c_uint64, C.c_uint64, C.c_uint64, _dp],
+            "orc_philox4x32_10": [C.c_void_p, C.c_void_p, C.c_void_p],
             "orc_set_alpha": [C.c_void_p, _dp],
         }.items():
             getattr(L, name).argtypes = args
@@ -133,6 +134,15 @@ def fft(x, sign=-1):
     buf[1::2] = x.imag
     lib().orc_fft(_p(buf), x.size, sign)
     return buf[0::2] + 1j * buf[1::2]
+
+
+def philox4x32_10(ctr, key):
+    """The raw Philox4x32-10 block of a 4-word counter and a 2-word key."""
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, out)
+    return list(out)
 
 
 def philox_normal(seed, step, n):

@@ -417,3 +417,68 @@ def test_philox_normal_statistics(oracle):
     z = oracle.philox_normal(1234, 5, 200000)
     assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
     assert not np.array_equal(z[:100], oracle.philox_normal(1234, 6, 100))
+
+
+# Random123's published known-answer vectors for Philox4x32-10 (counter, key, output)
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+def test_philox_known_answers(oracle):
+    """The round function, multipliers and key schedule of the thermal noise
+    stream against Random123's Philox4x32-10 known-answer vectors."""
+    for ctr, key, out in PHILOX_KAT:
+        assert tuple(oracle.philox4x32_10(ctr, key)) == out
+
+
+def test_philox_normal_mapping(oracle):
+    """Normal variate idx of (seed, step): counter (pair lo, pair hi, step lo,
+    step hi), key (seed lo, seed hi), two 53-bit uniforms from words (0,1) and
+    (2,3), Box-Muller with the cosine for even idx and the sine for odd."""
+    import math
+
+    seed, step = 0x123456789ABC, 7
+    z = oracle.philox_normal(seed, step, 10)
+    for idx in range(10):
+        pair = idx >> 1
+        c = oracle.philox4x32_10((pair & 0xFFFFFFFF, pair >> 32, step & 0xFFFFFFFF, step >> 32),
+                                 (seed & 0xFFFFFFFF, seed >> 32))
+        u1 = (((c[0] >> 6) << 27 | (c[1] >> 5)) + 0.5) / 9007199254740992.0
+        u2 = (((c[2] >> 6) << 27 | (c[3] >> 5)) + 0.5) / 9007199254740992.0
+        r = math.sqrt(-2.0 * math.log(u1))
+        ref = r * (math.sin if idx & 1 else math.cos)(2.0 * math.pi * u2)
+        assert abs(z[idx] - ref) <= 1e-15 * max(1.0, abs(ref))
+
+
+def test_demag_tensor_dipole_branch(oracle):
+    """Beyond R / max(d) = 60 cells the tensor is the point dipole
+    (src/micro/demag.jl:197-199,250-252 with src/atomistic/demag.jl:106-122):
+    Nxx = -V (2x^2 - y^2 - z^2) / (4 pi R^5), Nxy = -3 V x y / (4 pi R^5); at
+    60 cells the Newell sums agree with that form to O((d/R)^2)."""
+    nx, ny, nz, d = 90, 3, 2, 2e-9
+    N = nx * ny * nz
+    s = oracle.OracleSim(nx, ny, nz, d, d, d)
+    s.set_Ms(np.full(N, 8e5))
+    T = s.demag_tensor(s.add_demag())
+    V = d ** 3
+
+    def at(i, j, k):
+        return (k * ny + j) * nx + i
+
+    for i in (61, 70, 89):
+        for j, k in ((0, 0), (1, 0), (2, 1)):
+            x, y, z = i * d, j * d, k * d
+            R2 = x * x + y * y + z * z
+            R5 = R2 * R2 * np.sqrt(R2)
+            assert abs(T[0, at(i, j, k)] - (-(2 * x * x - y * y - z * z) / R5 * V / (4 * np.pi))) \
+                <= 1e-14 * abs(T[0, at(i, j, k)])
+            if j:
+                assert abs(T[3, at(i, j, k)] - (-3 * x * y / R5 * V / (4 * np.pi))) \
+                    <= 1e-14 * abs(T[3, at(i, j, k)])
+    x = 60 * d
+    dip = 2 * V / (4 * np.pi * x ** 3)  # Nxx on the axis
+    assert abs(T[0, at(60, 0, 0)] + dip) <= 1e-3 * dip
