@@ -106,3 +106,43 @@ def test_neb_fp32_against_fp64_oracle(case):
     print(f"[neb fp32] {case}: max |dm| {np.abs(gn.spin - on.spin).max():.2e}", flush=True)
     assert gn.state["nsteps"] == on.nsteps
     assert np.abs(gn.spin - on.spin).max() < 1e-5
+
+
+def test_neb_macrospin_barrier_reference():
+    """test/test_neb.jl:56-108 on the GPU: NEB(sim, (m0, m1), (1,)) puts the
+    one free image at (1,1,0)/sqrt(2); the macrospin band (Kx = 5e4 along x,
+    Kz = -2e4 along z, 11 free images) relaxes to the barrier Kx V =
+    5e4 (5e-9)^3 within 1e-3, and its band energies follow the oracle's band
+    relaxed the same way to 1e-3 of the barrier (at stopping_dmdt = 0.001 the
+    images still slide along the band, so the two runs stop a few steps apart;
+    step-for-step agreement is checked by test_neb_matches_oracle)."""
+    from test_oracle_neb import macrospin_oracle
+
+    print("[case] NEB macrospin (test_neb.jl)", flush=True)
+
+    def sim(exch):
+        mesh = mmx.FDMesh(dx=5e-9, dy=5e-9, dz=5e-9, nx=1, ny=1, nz=1)
+        s = mmx.Sim(mesh)
+        mmx.set_Ms(s, 8e5)
+        mmx.init_m0(s, (0.6, 0, -0.8))
+        if exch:
+            mmx.add_exch(s, 1.3e-11)
+        mmx.add_anis(s, 5e4, axis=(1, 0, 0), name="Kx")
+        mmx.add_anis(s, -2e4, axis=(0, 0, 1), name="Kz")
+        return s
+
+    two = mmx.NEB(sim(True), [(1, 0, 0), (0, 1, 0)], [1])
+    np.testing.assert_allclose(two.spin.reshape(-1), [0.7071067811865475, 0.7071067811865475, 0.0], rtol=1e-15)
+    neb = mmx.NEB(sim(False), [(1, 0, 0), (0, 1, 1), (-1, 0, 0)], [5, 5])
+    assert neb.spin.size == 3 * (5 + 5 + 1)
+    expected = 5e4 * 5e-9 ** 3
+    init_diff = neb.energy.max() - neb.energy[0]
+    neb.relax(max_steps=500, stopping_dmdt=0.001)
+    final_diff = neb.energy.max() - neb.energy[0]
+    print(f"init {init_diff:.6e} expected {expected:.6e} final {final_diff:.6e}", flush=True)
+    assert init_diff > expected
+    assert abs(final_diff - expected) / expected < 1e-3
+    o = macrospin_oracle()
+    ref = OracleNEB(o, [np.array([1, 0, 0.0]), np.array([0, 1, 1.0]), np.array([-1, 0, 0.0])], [5, 5])
+    ref.relax(max_steps=500, stopping_dmdt=0.001)
+    assert np.max(np.abs(neb.energy - ref.energy)) <= 1e-3 * expected
