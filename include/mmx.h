@@ -165,6 +165,17 @@ int mmx_term_field(mmx_ctx *ctx, int term, double t, double *H, double *E);
 /* compute_system_energy: src/micro/field.jl:520-536 -- sum(term.energy) per term */
 int mmx_compute_energies(mmx_ctx *ctx, double t, double *E);
 
+/* No counterpart in the reference: mmx_compute_energies for callers that cannot
+ * allocate (code under HIP graph capture, serving loops).  Writes the per-term
+ * energies of the current state to E_dev, a device array of mmx_num_terms
+ * doubles, on the context's stream without a host synchronisation (call
+ * mmx_synchronize before reading).  Calls no hipMalloc: the per-block partial
+ * sums live in a static device array of 65536 doubles per GPU; a mesh whose
+ * evaluation needs more blocks than that is refused with MMX_EINVAL.  Calls that
+ * overlap on one GPU share that array and race.  Contexts with an add_torque
+ * term are refused (its host callback allocates). */
+int mmx_compute_energies_noalloc(mmx_ctx *ctx, double t, double *E_dev);
+
 /* LLG driver fields alpha, gamma, precession: src/driver.jl:15-21 */
 int mmx_set_llg(mmx_ctx *ctx, double alpha, double gamma, int precession);
 /* Sim(mesh; driver="SpatialLLG") + set_alpha(sim, alpha): src/driver.jl:32-38,149-152,

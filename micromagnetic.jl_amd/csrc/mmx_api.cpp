@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -52,6 +53,32 @@ struct HipError {
             throw HipError{MMX_EHIP};                                                            \
         }                                                                                        \
     } while (0)
+
+// Device memory of the allocation-free energy entry point
+// (mmx_compute_energies_noalloc): the per-block partial sums of one term's
+// evaluation, reused term by term.  Each GPU the library runs on holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device (the array's name is a host-side
+// stand-in).  Calls that overlap on one GPU share it and race.
+static constexpr int NOALLOC_SLOTS = 65536;
+static __device__ __align__(256) double scratch[NOALLOC_SLOTS];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static double *scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// once per device, with that device current (called from mmx_create)
+static int scratch_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(MMX_EINVAL, "device %d out of range", device);
+    std::lock_guard<std::mutex> l(scratch_mu);
+    if (scratch_ptr[device]) return MMX_OK;
+    double *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(MMX_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+    scratch_ptr[device] = p;
+    return MMX_OK;
+}
 
 static constexpr double MU0 = 4.0 * M_PI * 1e-7;  // src/const.jl:1
 static constexpr double KB = 1.3806505e-23;       // src/const.jl:3
@@ -1208,6 +1235,30 @@ struct Engine : CtxBase {
         return MMX_OK;
     }
 
+    // per-term energies of the current state into out_dev (device, nterms
+    // values) without allocating: block partials in `part` (slots device
+    // doubles), one term after the other, each reduced on the stream
+    int energies_noalloc(double t, double *out_dev, double *part, int slots)
+    {
+        if (any_tfn) return fail(MMX_EINVAL, "the allocation-free energies take no add_torque term");
+        const int nb = demag ? p5_grid(G) : std::min<int64_t>(std::max<int64_t>((N + BLOCK - 1) / BLOCK, 1),
+                                                                max_local_blocks());
+        if (nb > slots)
+            return fail(MMX_EINVAL, "this mesh needs %d partial sums per term; the allocation-free path holds %d",
+                        nb, slots);
+        t_eval = t;
+        for (int q = 0; q < (int)C->terms.size(); q++) {
+            Epi<T> E = epi(EPI_ENERGY);
+            E.term_mask = 1u << q;
+            E.partial = part;
+            const int got = eval(y, E);
+            if (got != nb) return fail(MMX_ESTATE, "internal: %d partial sums written, %d expected", got, nb);
+            launch_reduce(part, nb, 1, 1, 0, out_dev + q, C->stream);
+            HIPCHECK(hipGetLastError());
+        }
+        return MMX_OK;
+    }
+
     double max_dm()
     {
         int nb = launch_maxdiff<T>(stg[1], y, N, partial.p, C->stream);
@@ -1809,6 +1860,7 @@ int mmx_create(mmx_ctx **out, const mmx_mesh_desc *mesh)
     c->mu0Ms.assign(N, 0.0);
     try {
         HIPCHECK(hipSetDevice(mesh->device));
+        if (scratch_init(mesh->device) != MMX_OK) return MMX_EHIP;
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (mesh->precision == MMX_F64) c->impl.reset(new Engine<double>(c.get()));
         else c->impl.reset(new Engine<float>(c.get()));
@@ -2457,6 +2509,19 @@ int mmx_compute_energies(mmx_ctx *c, double t, double *Eo)
         }
         return MMX_OK;
     });
+}
+
+int mmx_compute_energies_noalloc(mmx_ctx *c, double t, double *E_dev)
+{
+    if (!c || !E_dev) return fail(MMX_EINVAL, "null argument");
+    double *part;
+    {
+        std::lock_guard<std::mutex> l(scratch_mu);
+        part = scratch_ptr[c->mesh.device];
+    }
+    if (!part) return fail(MMX_ESTATE, "device %d has no scratch address (context not created by mmx_create)",
+                           c->mesh.device);
+    return with_engine(c, [&](auto &e) { return e.energies_noalloc(t, E_dev, part, NOALLOC_SLOTS); });
 }
 
 int mmx_set_alpha(mmx_ctx *c, const double *alpha)

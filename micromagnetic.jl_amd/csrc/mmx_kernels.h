@@ -122,6 +122,8 @@ template <class T>
 void launch_normalise(int64_t N, T *m, hipStream_t s);
 
 int max_local_blocks();
+// blocks of one launch of the last demag pass for this geometry
+int p5_grid(const DemagGeom &G);
 // streaming copy of n16 16-byte words (HBM ceiling probe)
 void launch_stream_copy(const void *src, void *dst, int64_t n16, int grid, hipStream_t s);
 // number of blocks (reduction slot groups) the last demag pass launches

@@ -13,6 +13,18 @@
 
 namespace mmx {
 
+#ifdef MMX_KT_MAIN_P5
+// blocks of one last-pass launch: RX rows per block, groups rounded up to the 8 XCDs
+int p5_grid(const DemagGeom &G)
+{
+    int g = 0;
+#define P5G_CALL(H) g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;
+    MMX_LG_SWITCH(G.lgH, P5G_CALL)
+#undef P5G_CALL
+    return g;
+}
+#endif
+
 template <class T>
 int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn, const Epi<T> &E,
               const T *m, int fast, uint32_t sig, const cplx<T> *tw, hipStream_t s)

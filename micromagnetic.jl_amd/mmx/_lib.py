@@ -65,6 +65,7 @@ SIGNATURES = {
     "mmx_add_sahe_torque": [_vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double],
     "mmx_add_magnetoelastic": [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double],
     "mmx_add_torque": [_vp, C.c_void_p, C.c_void_p],
+    "mmx_compute_energies_noalloc": [_vp, C.c_double, C.c_void_p],
     "mmx_add_dmi_int": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
     "mmx_set_alpha": [_vp, _dp],
     "mmx_num_terms": [_vp],
