@@ -222,8 +222,24 @@ int mmx_adjoint_record(mmx_ctx *ctx, int64_t nsteps, int64_t every);
 int mmx_adjoint_backward(mmx_ctx *ctx, const double *lambda_T, double *grad_m0);
 /* dL/dp per cell (N values) of the last backward pass for the term's scalar
  * parameter field: Ku (anisotropy), Kc (cubic), D (interfacial DMI), A
- * (spatial exchange). */
+ * (spatial exchange).  Bulk DMI (kernels.jl:306-422) gives 3N values, AoS
+ * (dL/dDx, dL/dDy, dL/dDz per cell): for a spatial vector D(x) the gradient of
+ * each cell's components; for a uniform D each cell's share of the gradient of
+ * the uniform components (their sum over cells). */
 int mmx_adjoint_param_grad(mmx_ctx *ctx, int term, double *grad);
+/* dL/dMs per cell (N values) of the last backward pass: Ms enters the local and
+ * stencil terms as 1/Ms and the demag term through M = Ms m (set_Ms,
+ * src/sim.jl:129-143); zero at cells with Ms = 0. */
+int mmx_adjoint_ms_grad(mmx_ctx *ctx, double *grad);
+/* The Jacobian of one llg_call_back evaluation (src/llg/llg.jl:45-58) at the
+ * state m (3N AoS; NULL: the current state) and time t, applied to v (JVP:
+ * out = J v) or transposed to a (VJP: out = J^T a), 3N AoS each.  The JVP is
+ * what ext/EnzymeExt.jl:32-59 (dynamic_matrix) builds column by column with
+ * Enzyme's forward mode; the field's Jacobian covers exchange, DMI, uniaxial
+ * and cubic anisotropy, static Zeeman and demag terms, uniform alpha.  The VJP,
+ * like the adjoint, takes m = 0 at cells with Ms = 0 (as init_m0 leaves them). */
+int mmx_llg_rhs_jvp(mmx_ctx *ctx, double t, const double *m, const double *v, double *out);
+int mmx_llg_rhs_vjp(mmx_ctx *ctx, double t, const double *m, const double *a, double *out);
 
 /* Batching form.  Many independent systems (thermal-ensemble members,
  * parameter sweeps) of any mesh sizes advanced together: every stage of a

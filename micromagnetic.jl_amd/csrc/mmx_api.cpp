@@ -1095,14 +1095,17 @@ struct Engine : CtxBase {
         bool recorded = false;
         int64_t nsteps = 0, every = 0, step0 = 0;
         double h = 0, t0 = 0;
-        DevBuf<T> ck, seg, work;
+        DevBuf<T> ck, seg, work, gms;
         std::vector<std::unique_ptr<DevBuf<T>>> g;
     } adj;
 
     static bool has_param_grad(int kind)
     {
-        return kind == K_ANIS || kind == K_ANIS_S || kind == K_CUBIC || kind == K_DMI_INT || kind == K_EXCH_S;
+        return kind == K_ANIS || kind == K_ANIS_S || kind == K_CUBIC || kind == K_DMI_INT || kind == K_EXCH_S ||
+               kind == K_DMI_BULK || kind == K_DMI_BULK_S;
     }
+    // bulk DMI gradients hold the three components (Dx, Dy, Dz planes)
+    static int64_t param_grad_planes(int kind) { return kind == K_DMI_BULK || kind == K_DMI_BULK_S ? 3 : 1; }
 
     int adj_record(int64_t nsteps, int64_t every)
     {
@@ -1157,10 +1160,13 @@ struct Engine : CtxBase {
         adj.g.resize(C->terms.size());
         for (size_t tt = 0; tt < C->terms.size(); tt++)
             if (has_param_grad(C->terms[tt].kind)) {
+                const int64_t n = param_grad_planes(C->terms[tt].kind) * N;
                 adj.g[tt] = std::make_unique<DevBuf<T>>();
-                adj.g[tt]->alloc(N);
-                HIPCHECK(hipMemsetAsync(adj.g[tt]->p, 0, N * sizeof(T), C->stream));
+                adj.g[tt]->alloc(n);
+                HIPCHECK(hipMemsetAsync(adj.g[tt]->p, 0, n * sizeof(T), C->stream));
             }
+        adj.gms.alloc(N);
+        HIPCHECK(hipMemsetAsync(adj.gms.p, 0, N * sizeof(T), C->stream));
         upload_soa(lamT, lam);
         const int64_t save_n = C->nsteps;
         const double save_t = C->t;
@@ -1214,6 +1220,7 @@ struct Engine : CtxBase {
                     A.adj.next = s > 0 ? as[s - 1] : nullptr;
                     A.adj.cnext = s == 3 ? h : 0.5 * h;
                     for (size_t tt = 0; tt < adj.g.size(); tt++) A.adj.g[tt] = adj.g[tt] ? adj.g[tt]->p : nullptr;
+                    A.adj.gms = adj.gms.p;
                     eval(U, A, true);
                 }
                 std::swap(lam, lam0);
@@ -1230,8 +1237,90 @@ struct Engine : CtxBase {
     {
         if (term < 0 || term >= (int)adj.g.size() || !adj.g[term])
             return fail(MMX_EINVAL, "no parameter gradient for term %d (run mmx_adjoint_backward; supported: "
-                                    "anisotropy Ku, cubic Kc, interfacial DMI D, spatial exchange A)", term);
-        download_scalar(adj.g[term]->p, out, N);
+                                    "anisotropy Ku, cubic Kc, interfacial and bulk DMI D, spatial exchange A)", term);
+        if (param_grad_planes(C->terms[term].kind) == 3) download_aos(adj.g[term]->p, out);
+        else download_scalar(adj.g[term]->p, out, N);
+        return MMX_OK;
+    }
+
+    // dL/dMs per cell: the mu0Ms gradient times mu0 (mu0Ms = mu0 Ms, src/sim.jl:138)
+    int adj_ms_grad(double *out)
+    {
+        if (!adj.gms.p) return fail(MMX_ESTATE, "no gradient yet (run mmx_adjoint_backward)");
+        download_scalar(adj.gms.p, out, N);
+        for (int64_t i = 0; i < N; i++) out[i] *= MU0;
+        return MMX_OK;
+    }
+
+    // ---- one RHS evaluation's Jacobian products -----------------------------
+    // JVP: J v = (dF/dm) v + (dF/dh) H'(v), H' the field's Jacobian at m (the
+    // terms linear in m evaluated on v, plus the cubic term linearised at m);
+    // VJP: J^T a through the adjoint pass.  Both at the state m (3N AoS, or the
+    // current state when null) and time t; the terms the adjoint covers only.
+    DevBuf<T> jbuf[5];
+    int jac_check() const
+    {
+        if (alphas.p) return fail(MMX_ESTATE, "the Jacobian products take a uniform alpha");
+        for (const auto &h : C->terms)
+            if (h.kind == K_SOT || h.kind == K_STT_SL || h.kind == K_STT_ZL || h.kind == K_ZEEMAN_T || h.kind == K_HEX ||
+                h.kind == K_EXCH_IL || h.kind == K_DMI_IL || h.kind == K_TWIN || h.kind == K_SAHE || h.kind == K_MEL_T ||
+                h.kind == K_MEL_C || h.kind == K_TORQUE_FN || h.kind == K_THERMAL || h.ufun)
+                return fail(MMX_EINVAL, "the Jacobian products cover the exchange, DMI, uniaxial and cubic anisotropy, "
+                                        "static Zeeman and demag terms");
+        return MMX_OK;
+    }
+    void jac_setup(const double *m)
+    {
+        for (auto &b : jbuf)
+            if (!b.p) b.alloc(3 * N);
+        if (m) upload_soa(m, jbuf[0].p);
+        else HIPCHECK(hipMemcpyAsync(jbuf[0].p, y, 3 * N * sizeof(T), hipMemcpyDeviceToDevice, C->stream));
+    }
+    void field_into(const T *src, T *out, uint32_t mask)
+    {
+        Epi<T> E = epi(EPI_FIELD);
+        E.term_mask = mask;
+        E.out = out;
+        eval(src, E, true);
+    }
+    int rhs_jvp(double t, const double *m, const double *v, double *out)
+    {
+        if (jac_check() != MMX_OK) return MMX_EINVAL;
+        jac_setup(m);
+        T *Y = jbuf[0].p, *V = jbuf[1].p, *hY = jbuf[2].p, *hv = jbuf[3].p;
+        upload_soa(v, V);
+        t_eval = t;
+        field_into(Y, hY, all_mask());
+        uint32_t lin = 0;  // terms linear in m (Zeeman is constant, the cubic term is linearised in k_llg_jvp)
+        for (size_t q = 0; q < C->terms.size(); q++) {
+            const int kd = C->terms[q].kind;
+            if (kd != K_ZEEMAN && kd != K_ZEEMAN_U && kd != K_CUBIC) lin |= 1u << q;
+        }
+        field_into(V, hv, lin);
+        launch_llg_jvp<T>(N, params_dev(), Y, hY, V, hv, hv, C->stream);
+        HIPCHECK(hipGetLastError());
+        download_aos(hv, out);
+        return MMX_OK;
+    }
+    int rhs_vjp(double t, const double *m, const double *a, double *out)
+    {
+        if (jac_check() != MMX_OK) return MMX_EINVAL;
+        jac_setup(m);
+        T *Y = jbuf[0].p, *Ab = jbuf[1].p, *hY = jbuf[2].p, *U = jbuf[3].p, *lam = jbuf[4].p;
+        upload_soa(a, Ab);
+        t_eval = t;
+        field_into(Y, hY, all_mask());
+        const FieldParams<T> *P = params_dev();
+        launch_adj_pre<T>(N, P, Y, Ab, U, C->stream);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemsetAsync(lam, 0, 3 * N * sizeof(T), C->stream));
+        Epi<T> A = epi(EPI_ADJ);
+        A.adj.Y = Y;
+        A.adj.hY = hY;
+        A.adj.a = Ab;
+        A.adj.lam = lam;
+        eval(U, A, true);
+        download_aos(lam, out);
         return MMX_OK;
     }
 
@@ -2787,6 +2876,26 @@ int mmx_adjoint_param_grad(mmx_ctx *c, int term, double *grad)
 {
     if (!grad) return fail(MMX_EINVAL, "null argument");
     return with_engine(c, [&](auto &e) { return e.adj_param_grad(term, grad); });
+}
+
+int mmx_adjoint_ms_grad(mmx_ctx *c, double *grad)
+{
+    if (!grad) return fail(MMX_EINVAL, "null argument");
+    return with_engine(c, [&](auto &e) { return e.adj_ms_grad(grad); });
+}
+
+int mmx_llg_rhs_jvp(mmx_ctx *c, double t, const double *m, const double *v, double *out)
+{
+    if (!c || !v || !out) return fail(MMX_EINVAL, "null argument");
+    if (!finite_all(v, 3 * c->N) || (m && !finite_all(m, 3 * c->N))) return fail(MMX_EINVAL, "NaN in the input");
+    return with_engine(c, [&](auto &e) { return e.rhs_jvp(t, m, v, out); });
+}
+
+int mmx_llg_rhs_vjp(mmx_ctx *c, double t, const double *m, const double *a, double *out)
+{
+    if (!c || !a || !out) return fail(MMX_EINVAL, "null argument");
+    if (!finite_all(a, 3 * c->N) || (m && !finite_all(m, 3 * c->N))) return fail(MMX_EINVAL, "NaN in the input");
+    return with_engine(c, [&](auto &e) { return e.rhs_vjp(t, m, a, out); });
 }
 
 int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n)

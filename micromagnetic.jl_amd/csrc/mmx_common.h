@@ -183,7 +183,7 @@ enum EpiMode : int {
 // (H_lin: the field's Jacobian at Y, which for the energy-derived terms is
 // diag(mu0Ms)^-1-symmetric, so J_H^T u = mu0Ms * J_H(u / mu0Ms)), adds mu to
 // lam and cnext * mu to next, and adds the terms' parameter gradients
-// u . dH/dp to g[term].
+// u . dH/dp to g[term] and u . dH/d(mu0Ms) to gms.
 template <class T>
 struct AdjIO {
     const T *Y;   // state the RHS was evaluated at
@@ -192,7 +192,8 @@ struct AdjIO {
     T *lam;       // += mu
     T *next;      // += cnext * mu (nullable)
     double cnext;
-    T *g[MAX_TERMS];  // per-cell parameter gradient of each term (nullable)
+    T *g[MAX_TERMS];  // per-cell parameter gradient of each term (nullable; 3 planes for bulk DMI)
+    T *gms;           // per-cell gradient with respect to mu0Ms (nullable)
 };
 
 template <class T>

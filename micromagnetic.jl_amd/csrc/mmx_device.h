@@ -988,6 +988,38 @@ __device__ __forceinline__ void llg_vjp_m(const MMX_CONST FieldParams<T> &P, T m
     rz *= a;
 }
 
+// Jacobian-vector product of llg_rhs at (m, h) along (v, w = dh):
+//   p = |m|^2 h - (m.h) m,  dp = 2 (m.v) h - (v.h) m - (m.h) v + |m|^2 w - (m.w) m,
+//   F = a (m x p - alpha p)  ->  dF = a (v x p + m x dp - alpha dp)
+// (the m x terms only with precession).
+template <class T>
+__device__ __forceinline__ void llg_jvp(const MMX_CONST FieldParams<T> &P, T mx, T my, T mz, T hx, T hy, T hz, T vx,
+                                        T vy, T vz, T wx, T wy, T wz, T &ox, T &oy, T &oz)
+{
+    const T alpha = P.alpha;
+    const T a = -P.gamma / (T(1) + alpha * alpha);
+    const T mm = mx * mx + my * my + mz * mz;
+    const T mh = mx * hx + my * hy + mz * hz;
+    const T mv = mx * vx + my * vy + mz * vz;
+    const T vh = vx * hx + vy * hy + vz * hz;
+    const T mw = mx * wx + my * wy + mz * wz;
+    const T px = mm * hx - mh * mx, py = mm * hy - mh * my, pz = mm * hz - mh * mz;
+    const T dx = T(2) * mv * hx - vh * mx - mh * vx + mm * wx - mw * mx;
+    const T dy = T(2) * mv * hy - vh * my - mh * vy + mm * wy - mw * my;
+    const T dz = T(2) * mv * hz - vh * mz - mh * vz + mm * wz - mw * mz;
+    ox = -alpha * dx;
+    oy = -alpha * dy;
+    oz = -alpha * dz;
+    if (P.precession) {
+        ox += (vy * pz - vz * py) + (my * dz - mz * dy);
+        oy += (vz * px - vx * pz) + (mz * dx - mx * dz);
+        oz += (vx * py - vy * px) + (mx * dy - my * dx);
+    }
+    ox *= a;
+    oy *= a;
+    oz *= a;
+}
+
 // e_q x v for the interfacial DMI directions e = (0,1,0), (0,-1,0), (-1,0,0),
 // (1,0,0) of the -x, +x, -y, +y neighbours (src/micro/kernels.jl:426-466)
 template <class T>
@@ -1062,10 +1094,71 @@ __device__ __forceinline__ void adj_cell(const MMX_CONST FieldParams<T> &P, cons
     if (!(Ms > T(0))) return;
     int64_t nb[6];
     bool nbd = false;
+    if (A.gms) {
+        // mu0Ms enters the stencil and local terms as 1/mu0Ms (dH/dmu = -H/mu at
+        // this cell) and the demag term through M = mu0Ms m of every cell: the
+        // demag value of this pass is D u (the operator is symmetric), so cell
+        // I gains Y_I . (D u)_I
+        uint32_t lm = 0;
+        for (int tt = 0; tt < P.nterms; tt++) {
+            const int kd = P.t[tt].kind;
+            if (kd == K_EXCH || kd == K_EXCH_S || kd == K_DMI_BULK || kd == K_DMI_BULK_S || kd == K_DMI_INT ||
+                kd == K_ANIS || kd == K_ANIS_S || kd == K_CUBIC)
+                lm |= ((E.term_mask >> tt) & 1u) << tt;
+        }
+        T lx = 0, ly = 0, lz = 0;
+        if (lm) cell_field(P, dyn, lm, I, i, j, k, A.Y, T(0), T(0), T(0), lx, ly, lz, (double *)nullptr);
+        const T gl = -(ux * lx + uy * ly + uz * lz);  // u . (-h/mu) with u = mu ut
+        const T gd = Yx * hdx + Yy * hdy + Yz * hdz;
+        A.gms[I] += gl + gd;
+    }
     for (int tt = 0; tt < P.nterms; tt++) {
         T *g = A.g[tt];
         if (!g) continue;
         const MMX_CONST TermDev<T> &D = P.t[tt];
+        if (D.kind == K_DMI_BULK || D.kind == K_DMI_BULK_S) {
+            // bulk DMI (kernels.jl:306-422): f = sum_q De_q s_q (e_q x Y_q) over the
+            // neighbours along axis e_q, s_q = +-1/d (-: the + neighbour).  Uniform D:
+            // the per-cell share ut . f_c / D_c of dL/dD_c (summed over cells it is
+            // the gradient of the uniform component); vector D(x): the harmonic mean
+            // De = 2 D_p D_n / (D_p + D_n) of both cells of each pair, differentiated
+            // for D_p as this cell's and as its neighbour's value.
+            if (!nbd) {
+                neighbours(P, i, j, k, nb);
+                nbd = true;
+            }
+            T gc[3] = {0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int64_t id = nb[q];
+                if (id < 0 || !(P.mu0Ms[id] > T(0))) continue;
+                const int ax = q >> 1;
+                const T sg = (q & 1) ? T(-1) : T(1);
+                const T jx = A.Y[id], jy = A.Y[N + id], jz = A.Y[2 * N + id];
+                // e x Y_j and e x Y_p for axis ax
+                T wx, wy, wz, zx, zy, zz;
+                if (ax == 0) { wx = 0; wy = -jz; wz = jy; zx = 0; zy = -Yz; zz = Yy; }
+                else if (ax == 1) { wx = jz; wy = 0; wz = -jx; zx = Yz; zy = 0; zz = -Yx; }
+                else { wx = -jy; wy = jx; wz = 0; zx = -Yy; zy = Yx; zz = 0; }
+                if (D.kind == K_DMI_BULK) {
+                    const T Dc = D.s[6 + ax];
+                    if (Dc != T(0)) gc[ax] += (D.s[ax] * sg) * (ux * wx + uy * wy + uz * wz) / Dc;
+                } else {
+                    const T *Dp = ax == 0 ? D.a : (ax == 1 ? D.b : D.c);
+                    const T DI = Dp[I], Dn = Dp[id];
+                    if (DI == T(0) || Dn == T(0)) continue;
+                    const T ssum = DI + Dn;
+                    const T dDe = T(2) * Dn * Dn / (ssum * ssum);
+                    const T vx = ut[id], vy = ut[N + id], vz = ut[2 * N + id];
+                    const T t = (ux * wx + uy * wy + uz * wz) - (vx * zx + vy * zy + vz * zz);
+                    gc[ax] += dDe * (D.s[ax] * sg) * t;
+                }
+            }
+            g[I] += gc[0];
+            g[N + I] += gc[1];
+            g[2 * N + I] += gc[2];
+            continue;
+        }
         T gv = 0;
         switch (D.kind) {
         case K_ANIS:

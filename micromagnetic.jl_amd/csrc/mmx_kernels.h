@@ -74,6 +74,9 @@ int launch_maxabs(const T *v, int64_t n, double *partial, hipStream_t s);
 // adjoint of the RK4 step: the EPI_ADJ input vector, and the pull-back of
 // the final combination + normalisation
 template <class T>
+void launch_llg_jvp(int64_t N, const FieldParams<T> *P, const T *Y, const T *hY, const T *v, const T *hv, T *out,
+                    hipStream_t s);
+template <class T>
 void launch_adj_pre(int64_t N, const FieldParams<T> *P, const T *Y, const T *a, T *ut, hipStream_t s);
 template <class T>
 void launch_adj_final(int64_t N, const T *y0, const T *const k[4], double c, const T *lam, T *lam0, T *const a[4],

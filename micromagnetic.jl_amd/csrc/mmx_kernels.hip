@@ -882,6 +882,49 @@ __global__ void k_adj_pre(const FieldParams<T> *__restrict__ Pp, const T *__rest
     }
 }
 
+// Jacobian-vector product of one RHS evaluation: out = (dF/dm) v + (dF/dh) w at
+// (Y, hY), where w = hv (the field of v under the terms that are linear in m)
+// plus the cubic anisotropy's Jacobian at Y applied to v; zero at pinned cells
+// (llg_rhs_kernel! leaves them at rest).  out may alias hv.
+template <class T>
+__global__ void k_llg_jvp(const FieldParams<T> *__restrict__ Pp, const T *__restrict__ Y, const T *__restrict__ hY,
+                          const T *__restrict__ v, const T *hv, T *out)
+{
+#pragma clang fp contract(off)
+    const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
+    const int64_t N = P.N;
+    for (int64_t I = (int64_t)blockIdx.x * BLOCK + threadIdx.x; I < N; I += (int64_t)gridDim.x * BLOCK) {
+        const T mx = Y[I], my = Y[N + I], mz = Y[2 * N + I];
+        const T vx = v[I], vy = v[N + I], vz = v[2 * N + I];
+        T wx = hv[I], wy = hv[N + I], wz = hv[2 * N + I];
+        const T Ms = P.mu0Ms[I];
+        if (Ms > T(0)) {
+            for (int tt = 0; tt < P.nterms; tt++) {
+                const MMX_CONST TermDev<T> &D = P.t[tt];
+                if (D.kind != K_CUBIC) continue;
+                // (12 Kc / mu0Ms) sum_r (Y.a_r)^2 (v.a_r) a_r  (cubic_anisotropy_kernel!, kernels.jl:110-137)
+                const T Kc = D.uniform ? D.s[9] : D.a[I];
+                const T c = T(12) * Kc / Ms;
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const T ax = D.s[3 * r], ay = D.s[3 * r + 1], az = D.s[3 * r + 2];
+                    const T yp = mx * ax + my * ay + mz * az;
+                    const T f = c * yp * yp * (vx * ax + vy * ay + vz * az);
+                    wx += f * ax;
+                    wy += f * ay;
+                    wz += f * az;
+                }
+            }
+        }
+        T ox = 0, oy = 0, oz = 0;
+        if (!(P.pins && P.pins[I]))
+            llg_jvp(P, mx, my, mz, hY[I], hY[N + I], hY[2 * N + I], vx, vy, vz, wx, wy, wz, ox, oy, oz);
+        out[I] = ox;
+        out[N + I] = oy;
+        out[2 * N + I] = oz;
+    }
+}
+
 // Pull-back of y1 = normalise(y0 + c (((k1 + 2 k2) + 2 k3) + k4)) (rk.jl:46,
 // helper.jl:255-271): lam_z = (lam - (lam.y1) y1) / |z|; writes lam0 = lam_z
 // and the stage cotangents a1 = a4 = c lam_z, a2 = a3 = 2 c lam_z.
@@ -1295,6 +1338,13 @@ void launch_adj_pre(int64_t N, const FieldParams<T> *P, const T *Y, const T *a, 
 }
 
 template <class T>
+void launch_llg_jvp(int64_t N, const FieldParams<T> *P, const T *Y, const T *hY, const T *v, const T *hv, T *out,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_llg_jvp<T>, dim3(grid_for(N)), dim3(BLOCK), 0, s, P, Y, hY, v, hv, out);
+}
+
+template <class T>
 void launch_adj_final(int64_t N, const T *y0, const T *const k[4], double c, const T *lam, T *lam0, T *const a[4],
                       hipStream_t s)
 {
@@ -1459,6 +1509,8 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
                                       const T *, const double *, int, const double[4], double, T *,          \
                                       const T *, const T *, const T *, const T *, T *, double *, hipStream_t); \
     template void launch_adj_pre<T>(int64_t, const FieldParams<T> *, const T *, const T *, T *, hipStream_t); \
+    template void launch_llg_jvp<T>(int64_t, const FieldParams<T> *, const T *, const T *, const T *, const T *,   \
+                                    T *, hipStream_t);                                                          \
     template void launch_adj_final<T>(int64_t, const T *, const T *const[4], double, const T *, T *,          \
                                       T *const[4], hipStream_t);
 

@@ -314,21 +314,25 @@ def add_dmi(sim, D, ft=None, name=None, type="bulk"):
             return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_INTERFACIAL, None, None, None, float(D), 0.0, 0.0), name)
         a = init_scalar(sim.mesh, D)
         return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_INTERFACIAL, dptr(a), None, None, 0.0, 0.0, 0.0), name)
+    if type not in ("bulk", "D2d"):
+        raise ValueError("Supported DMI type: interfacial bulk D2d")
     if type == "D2d":
         if np.ndim(D) != 0:
             raise ValueError("D2d only support uniform DMI!")
-        return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, -float(D), float(D), 0.0), name)
-    if type != "bulk":
-        raise ValueError("Supported DMI type: interfacial bulk D2d")
-    if np.ndim(D) == 0 and not callable(D):
-        return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, float(D), float(D), float(D)), name)
-    if isinstance(D, tuple) and len(D) == 3:
-        if all(np.ndim(d) == 0 and not callable(d) for d in D):
-            return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, *map(float, D)), name)
-        Dx, Dy, Dz = (init_scalar(sim.mesh, d) for d in D)
+        idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, -float(D), float(D), 0.0)
+    elif np.ndim(D) == 0 and not callable(D):
+        idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, float(D), float(D), float(D))
+    elif isinstance(D, tuple) and len(D) == 3 and all(np.ndim(d) == 0 and not callable(d) for d in D):
+        idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, None, None, None, *map(float, D))
     else:
-        Dx = Dy = Dz = init_scalar(sim.mesh, D)
-    return sim._add(L.mmx_add_dmi(sim._h, _lib.DMI_BULK, dptr(Dx), dptr(Dy), dptr(Dz), 0.0, 0.0, 0.0), name)
+        if isinstance(D, tuple) and len(D) == 3:
+            Dx, Dy, Dz = (init_scalar(sim.mesh, d) for d in D)
+        else:
+            Dx = Dy = Dz = init_scalar(sim.mesh, D)
+        idx = L.mmx_add_dmi(sim._h, _lib.DMI_BULK, dptr(Dx), dptr(Dy), dptr(Dz), 0.0, 0.0, 0.0)
+    t = sim._add(idx, name)
+    t.kind = "bulk_dmi"  # its parameter gradient has three components per cell
+    return t
 
 
 def add_anis(sim, Ku, axis=(0, 0, 1), name="anis"):
@@ -841,12 +845,14 @@ def set_term_param(sim, term, values):
 def rk4_trajectory_rrule(sim, nsteps, every=50):
     """Run `nsteps` RK4 steps from the current state and return (m_T, pullback).
 
-    The analogue of a ChainRules `rrule` for the map m0, {Ku, D, Kc, A} -> m(T):
-    `pullback(dL_dmT)` takes the cotangent of the final state (3N, AoS) and
-    returns {"m0": dL/dm0 (3N, AoS), term name: dL/dp per cell (N)} for every
-    term with a scalar parameter field (anisotropy Ku, cubic Kc, interfacial
-    DMI D, spatial exchange A).  The backward pass recomputes the trajectory
-    from checkpoints kept every `every` steps; it may be called more than once.
+    The analogue of a ChainRules `rrule` for the map m0, Ms, {Ku, D, Kc, A} ->
+    m(T): `pullback(dL_dmT)` takes the cotangent of the final state (3N, AoS)
+    and returns {"m0": dL/dm0 (3N, AoS), "Ms": dL/dMs (N), term name: dL/dp
+    per cell} for every term with a parameter field (anisotropy Ku, cubic Kc,
+    interfacial DMI D, spatial exchange A: N values; bulk DMI: 3N values AoS,
+    the Dx, Dy, Dz components, per-cell shares of the gradient when D is
+    uniform).  The backward pass recomputes the trajectory from checkpoints
+    kept every `every` steps; it may be called more than once.
     """
     L = _lib.load()
     check(L.mmx_adjoint_record(sim._h, int(nsteps), int(every)))
@@ -860,13 +866,85 @@ def rk4_trajectory_rrule(sim, nsteps, every=50):
         g0 = np.empty(3 * N)
         check(L.mmx_adjoint_backward(sim._h, dptr(lam), dptr(g0)))
         out = {"m0": g0}
+        gms = np.empty(N)
+        check(L.mmx_adjoint_ms_grad(sim._h, dptr(gms)))
+        out["Ms"] = gms
         for t in sim.interactions:
-            g = np.empty(N)
+            g = np.empty(3 * N)  # bulk DMI: (dL/dDx, dL/dDy, dL/dDz) per cell; N values otherwise
             if L.mmx_adjoint_param_grad(sim._h, t.index, dptr(g)) >= 0:
-                out[t.name] = g
+                out[t.name] = g if getattr(t, "kind", None) == "bulk_dmi" else g[:N].copy()
         return out
 
     return mT, pullback
+
+
+def llg_rhs_jvp(sim, v, spin=None, t=0.0):
+    """(d dm/dt / dm) v for one llg_call_back evaluation (src/llg/llg.jl:45-58)
+    at `spin` (or the current state): 3N AoS in, 3N AoS out."""
+    N = sim.n_total
+    v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+    if v.size != 3 * N:
+        raise ValueError(f"expected {3 * N} values, got {v.size}")
+    m = None if spin is None else np.ascontiguousarray(np.asarray(spin, dtype=np.float64).reshape(-1))
+    out = np.empty(3 * N)
+    check(_lib.load().mmx_llg_rhs_jvp(sim._h, float(t), dptr(m), dptr(v), dptr(out)))
+    return out
+
+
+def llg_rhs_vjp(sim, a, spin=None, t=0.0):
+    """(d dm/dt / dm)^T a for one llg_call_back evaluation at `spin` (or the
+    current state): 3N AoS in, 3N AoS out."""
+    N = sim.n_total
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+    if a.size != 3 * N:
+        raise ValueError(f"expected {3 * N} values, got {a.size}")
+    m = None if spin is None else np.ascontiguousarray(np.asarray(spin, dtype=np.float64).reshape(-1))
+    out = np.empty(3 * N)
+    check(_lib.load().mmx_llg_rhs_vjp(sim._h, float(t), dptr(m), dptr(a), dptr(out)))
+    return out
+
+
+def rotation_matrix(mx, my, mz):
+    """rotation_matrix: src/eigen/util.jl:117-139 (third column along m)."""
+    norm = np.sqrt(mx * mx + my * my + mz * mz)
+    if norm < np.finfo(float).eps:
+        return np.eye(3)
+    theta = np.arccos(mz / norm)
+    phi = np.arctan2(my, mx)
+    return np.array([[np.cos(phi) * np.cos(theta), -np.sin(phi), np.sin(theta) * np.cos(phi)],
+                     [np.sin(phi) * np.cos(theta), np.cos(phi), np.sin(theta) * np.sin(phi)],
+                     [-np.sin(theta), 0.0, np.cos(theta)]])
+
+
+def dynamic_matrix(sim, gamma=2.21e5, alpha=0.01):
+    """dynamic_matrix(sim; gamma, alpha): ext/EnzymeExt.jl:32-59.  The Jacobian A
+    of y = gamma m x H_eff(m) at the current state (H_eff without the constant
+    Zeeman field's derivative) built column by column, then for every pair of
+    cells the transverse 2x2 block of R_i^T A_ij R_j (2N x 2N).  The columns
+    come from the library's JVP of the LLG right-hand side with alpha = 0
+    (-gamma m x h there, so A = -J on the transverse directions); alpha is
+    unused, as in the reference."""
+    N = sim.n_total
+    m0 = sim.spin.copy()
+    d = sim.driver
+    saved = (d.alpha, d.gamma, d.precession)
+    try:
+        d.alpha, d.gamma, d.precession = 0.0, float(gamma), True
+        A = np.zeros((3 * N, 3 * N))
+        e = np.zeros(3 * N)
+        for i in range(3 * N):
+            e[:] = 0.0
+            e[i] = 1.0
+            A[:, i] = -llg_rhs_jvp(sim, e, spin=m0)
+    finally:
+        d.alpha, d.gamma, d.precession = saved
+    Rs = [rotation_matrix(*m0[3 * i:3 * i + 3]) for i in range(N)]
+    Cm = np.zeros((2 * N, 2 * N))
+    for i in range(N):
+        for j in range(N):
+            B = Rs[i].T @ A[3 * i:3 * i + 3, 3 * j:3 * j + 3] @ Rs[j]
+            Cm[2 * i:2 * i + 2, 2 * j:2 * j + 2] = B[:2, :2]
+    return Cm
 
 
 # ---- batching form -------------------------------------------------------------

@@ -66,6 +66,9 @@ SIGNATURES = {
     "mmx_add_magnetoelastic": [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double],
     "mmx_add_torque": [_vp, C.c_void_p, C.c_void_p],
     "mmx_compute_energies_noalloc": [_vp, C.c_double, C.c_void_p],
+    "mmx_adjoint_ms_grad": [_vp, _dp],
+    "mmx_llg_rhs_jvp": [_vp, C.c_double, _dp, _dp, _dp],
+    "mmx_llg_rhs_vjp": [_vp, C.c_double, _dp, _dp, _dp],
     "mmx_add_dmi_int": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
     "mmx_set_alpha": [_vp, _dp],
     "mmx_num_terms": [_vp],
