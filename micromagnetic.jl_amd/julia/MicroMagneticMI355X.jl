@@ -382,14 +382,79 @@ function evolve_rk4_pullback(sim, dL_dmT::AbstractVector)
     g0 = Vector{Float64}(undef, 3n)
     check(ccall((:mmx_adjoint_backward, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h,
                 Vector{Float64}(dL_dmT), g0))
-    grads = Dict{Int,Vector{Float64}}()
+    grads = Dict{Any,Vector{Float64}}()
     nterms = ccall((:mmx_num_terms, libmmx), Cint, (Ptr{Cvoid},), h)
     for t in 0:(nterms - 1)
-        g = Vector{Float64}(undef, n)
+        # bulk DMI terms give (dL/dDx, dL/dDy, dL/dDz) per cell (3n values), the others n
+        g = Vector{Float64}(undef, 3n)
         rc = ccall((:mmx_adjoint_param_grad, libmmx), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), h, t, g)
-        rc >= 0 && (grads[t + 1] = g)
+        term = sim.interactions[t + 1]
+        bulk = term isa MicroMagnetic.BulkDMI || term isa MicroMagnetic.SpatialVectorBulkDMI
+        rc >= 0 && (grads[t + 1] = bulk ? g : g[1:n])
     end
+    gms = Vector{Float64}(undef, n)
+    check(ccall((:mmx_adjoint_ms_grad, libmmx), Cint, (Ptr{Cvoid}, Ptr{Float64}), h, gms))
+    grads[:Ms] = gms
     return g0, grads
+end
+
+"""
+    llg_rhs_jvp(sim, v; spin=nothing, t=0.0), llg_rhs_vjp(sim, a; spin=nothing, t=0.0)
+
+The Jacobian of one llg_call_back evaluation at `spin` (the current state when
+nothing) applied to v, or transposed and applied to a (3N vectors).
+"""
+function llg_rhs_jvp(sim, v::AbstractVector; spin=nothing, t=0.0)
+    out = Vector{Float64}(undef, 3 * sim.n_total)
+    m = spin === nothing ? Ptr{Float64}(C_NULL) : Vector{Float64}(spin)
+    check(ccall((:mmx_llg_rhs_jvp, libmmx), Cint, (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                contexts[sim], t, m, Vector{Float64}(v), out))
+    return out
+end
+
+function llg_rhs_vjp(sim, a::AbstractVector; spin=nothing, t=0.0)
+    out = Vector{Float64}(undef, 3 * sim.n_total)
+    m = spin === nothing ? Ptr{Float64}(C_NULL) : Vector{Float64}(spin)
+    check(ccall((:mmx_llg_rhs_vjp, libmmx), Cint, (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                contexts[sim], t, m, Vector{Float64}(a), out))
+    return out
+end
+
+"""
+    dynamic_matrix_gpu(sim; gamma=2.21e5)
+
+ext/EnzymeExt.jl's dynamic_matrix from the library's JVP: the Jacobian of
+gamma m x H_eff(m) column by column (the JVP of the LLG right-hand side with
+alpha = 0 is -gamma m x h on the transverse directions), rotated into each
+cell's frame, transverse 2x2 blocks (2N x 2N).
+"""
+function dynamic_matrix_gpu(sim; gamma=2.21e5)
+    h = contexts[sim]
+    n = sim.n_total
+    pull_spin!(sim)
+    m0 = Array(sim.spin)
+    check(ccall((:mmx_set_llg, libmmx), Cint, (Ptr{Cvoid}, Float64, Float64, Cint), h, 0.0, gamma, 1))
+    A = zeros(3n, 3n)
+    try
+        e = zeros(3n)
+        for i in 1:(3n)
+            e .= 0
+            e[i] = 1
+            A[:, i] .= .-llg_rhs_jvp(sim, e; spin=m0)
+        end
+    finally  # the driver's own alpha, gamma and precession again
+        drv = sim.driver
+        check(ccall((:mmx_set_llg, libmmx), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Cint), h,
+                    drv isa MicroMagnetic.SpatialLLG ? 0.1 : Float64(drv.alpha), Float64(drv.gamma),
+                    Cint(drv.precession)))
+    end
+    C = zeros(2n, 2n)
+    Rs = [MicroMagnetic.rotation_matrix(m0[3i-2], m0[3i-1], m0[3i]) for i in 1:n]
+    for i in 1:n, j in 1:n
+        B = Rs[i]' * A[3i-2:3i, 3j-2:3j] * Rs[j]
+        C[2i-1:2i, 2j-1:2j] .= B[1:2, 1:2]
+    end
+    return C
 end
 
 if isdefined(Main, :ChainRulesCore)
