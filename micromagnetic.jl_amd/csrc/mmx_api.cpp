@@ -125,6 +125,9 @@ struct HostTerm {
     void (*ft3)(double, double *, void *) = nullptr;  // time Zeeman (fx, fy, fz)(t)
     // add_torque: fn(t, m, y, user) writes the torque y (3N AoS) for the state m (3N AoS)
     void (*tfn)(double, const double *, double *, void *) = nullptr;
+    // the host torque buffer handed to fn: zero when the term is added, then kept
+    // from call to call like the reference's torque.field (field.jl:282-293)
+    std::vector<double> tbuf;
 };
 
 struct CtxBase {
@@ -498,20 +501,31 @@ struct Engine : CtxBase {
     // add_torque terms: the state of this evaluation goes to the host, each
     // function writes its torque, which returns to the term's planes before the
     // passes read it (field.jl:282-293 calls torque_fun on the same stream order)
-    std::vector<double> tfn_m, tfn_y;
+    std::vector<double> tfn_m;
     void run_torque_fns(const T *m)
     {
         for (size_t q = 0; q < C->terms.size(); q++) {
-            const HostTerm &h = C->terms[q];
+            HostTerm &h = C->terms[q];
             if (!h.tfn) continue;
             if (tfn_m.empty()) {
                 tfn_m.resize(3 * N);
-                tfn_y.resize(3 * N);
                 download_aos(m, tfn_m.data());
             }
-            std::fill(tfn_y.begin(), tfn_y.end(), 0.0);
-            h.tfn(t_eval, tfn_m.data(), tfn_y.data(), h.uuser);
-            upload_soa(tfn_y.data(), const_cast<T *>(tA[q]));
+            h.tfn(t_eval, tfn_m.data(), h.tbuf.data(), h.uuser);
+            upload_soa(h.tbuf.data(), const_cast<T *>(tA[q]));
+            // torque_kernel! (kernels.jl:639-651) overwrites torque.field with
+            // H = (1/gamma) m x T in place, so the reference's next call of the
+            // function starts from that field: the host buffer follows it
+            const T a = T(1) / (T)C->gamma;
+            double *y = h.tbuf.data();
+            const double *mm = tfn_m.data();
+            for (int64_t i = 0; i < 3 * N; i += 3) {
+                const T mx = (T)mm[i], my = (T)mm[i + 1], mz = (T)mm[i + 2];
+                const T hx = (T)y[i], hy = (T)y[i + 1], hz = (T)y[i + 2];
+                y[i] = (double)(a * (-mz * hy + my * hz));
+                y[i + 1] = (double)(a * (mz * hx - mx * hz));
+                y[i + 2] = (double)(a * (-my * hx + mx * hy));
+            }
             HIPCHECK(hipStreamSynchronize(C->stream));
         }
         tfn_m.clear();
@@ -2376,6 +2390,7 @@ int mmx_add_torque(mmx_ctx *c, void (*fn)(double t, const double *m, double *y, 
         h.kind = K_TORQUE_FN;
         h.tfn = fn;
         h.uuser = user;
+        h.tbuf.assign(3 * c->N, 0.0);
         std::vector<double> zero(3 * c->N, 0.0);
         const TT *a = e.new_array(zero.data(), 3 * c->N, true);
         e.any_tfn = true;

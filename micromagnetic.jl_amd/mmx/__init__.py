@@ -381,7 +381,12 @@ def add_torque(sim, fun, name="torque"):
     n = 3 * sim.n_total
 
     def cb(t, m, y, user):
-        fun(np.ctypeslib.as_array(y, shape=(n,)), np.ctypeslib.as_array(m, shape=(n,)), t)
+        # an exception cannot cross the C frames: it is kept and raised by the
+        # check() of the library call that ran this evaluation
+        try:
+            fun(np.ctypeslib.as_array(y, shape=(n,)), np.ctypeslib.as_array(m, shape=(n,)), t)
+        except BaseException as e:  # noqa: BLE001
+            _lib.pending_errors.append(e)
 
     f = _TORQUEFN(cb)
     t = sim._add(_lib.load().mmx_add_torque(sim._h, C.cast(f, C.c_void_p), None), name)

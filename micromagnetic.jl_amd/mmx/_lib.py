@@ -126,7 +126,16 @@ def load(path=LIB_PATH):
     return L
 
 
+# exceptions raised inside host callbacks (add_torque functions) during a
+# library call; the next check() raises the first of them
+pending_errors = []
+
+
 def check(rc):
+    if pending_errors:
+        e = pending_errors[0]
+        pending_errors.clear()
+        raise e
     if rc < 0:
         raise MmxError(rc, load().mmx_last_error().decode(errors="replace"))
     return rc

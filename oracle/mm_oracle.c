@@ -1414,8 +1414,9 @@ int orc_add_torque(orc_sim *s, void (*fn)(double, const double *, double *, void
 /* TorqueField: field.jl:282-293 then torque_kernel! src/micro/kernels.jl:639-651 */
 static void field_torque_fn(orc_sim *s, term_t *t, const double *m)
 {
+    /* torque.field: zero when the term is added, then handed to the function
+     * holding the previous evaluation's field (torque_kernel! writes H into it) */
     double *y = t->a;
-    memset(y, 0, 3 * (size_t)s->N * sizeof(double));
     t->tfn(s->t_eval, m, y, t->uuser);
     const double a = 1 / s->gamma;
     for (int I = 0; I < s->N; I++) {
@@ -1426,6 +1427,7 @@ static void field_torque_fn(orc_sim *s, term_t *t, const double *m)
         t->field[j + 2] = a * cross_z(mx, my, mz, hx, hy, hz);
         t->energy[I] = 0;
     }
+    memcpy(y, t->field, 3 * (size_t)s->N * sizeof(double));
 }
 
 /* add_magnetoelastic(sim; model, lambda_s, B1, B2, sigma / strain): src/micro/add_field.jl:1367-1405;

@@ -105,3 +105,45 @@ def test_torque_fn_precession_and_refusals():
         mmx.rk4_trajectory_rrule(sim, 4, every=2)
     with pytest.raises(mmx.MmxError):  # the band's fields are not evaluated per stage state
         mmx.NEB(sim, [(1, 0, 0), (0, 0, 1)], [3])
+
+
+def test_torque_fn_buffer_and_errors(oracle):
+    """The host buffer handed to the function holds the previous evaluation's
+    field (as the reference's in-place torque_kernel! leaves it), so a function
+    that builds on it gives the oracle's fields call after call; an exception in
+    the function surfaces from the library call that ran it."""
+    note("add_torque persistent buffer and exceptions")
+
+    def acc(y, m, t):
+        y[:] = 0.5 * y + 1e9 * m
+
+    nx, ny, nz = 6, 5, 2
+    N = nx * ny * nz
+    m0 = np.random.default_rng(9).standard_normal(3 * N)
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz)
+    g = mmx.Sim(mesh)
+    o = oracle.OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9)
+    mmx.set_Ms(g, 7e5)
+    o.set_Ms(np.full(N, 7e5))
+    mmx.init_m0(g, m0)
+    o.init_m0(m0)
+    tg = mmx.add_torque(g, acc)
+    to = o.add_torque(acc)
+    for _ in range(3):
+        fg, _ = g._term_field(tg.index)
+        fo, _ = o.term_field(to)
+        assert relerr(fg, fo) <= 1e-12
+    boom = []
+
+    def bad(y, m, t):
+        boom.append(t)
+        raise ValueError("torque function failed")
+
+    mesh2 = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=4, ny=3, nz=1)
+    g2 = mmx.Sim(mesh2)
+    mmx.set_Ms(g2, 7e5)
+    mmx.init_m0(g2, (1, 0, 0))
+    mmx.add_torque(g2, bad)
+    with pytest.raises(ValueError, match="torque function failed"):
+        mmx.effective_field(g2)
+    assert boom

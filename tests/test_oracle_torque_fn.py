@@ -55,3 +55,28 @@ def test_precession_closed_form():
     m = o.spin
     phi = w * 100 * 1e-13
     assert abs(m[0] - math.cos(phi)) < 1e-9 and abs(m[1] - math.sin(phi)) < 1e-9
+
+
+def test_torque_buffer_persists_as_field():
+    """field.jl:282-293 hands torque.field to the function and torque_kernel!
+    (kernels.jl:639-651) then overwrites it in place with H = (1/gamma) m x T:
+    the first call sees zeros, every later call the previous field."""
+    seen = []
+
+    def acc(y, m, t):
+        seen.append(y.copy())
+        y[:] = y + 1e9  # builds on what it was handed
+
+    o = OracleSim(3, 2, 1, 1e-9, 1e-9, 1e-9)
+    o.set_Ms(8e5)
+    o.set_llg(0.1, 2.21e5, True)
+    m = np.random.default_rng(4).standard_normal(18)
+    o.init_m0(m)
+    k = o.add_torque(acc)
+    h1, _ = o.term_field(k)
+    h2, _ = o.term_field(k)
+    assert np.all(seen[0] == 0)
+    np.testing.assert_array_equal(seen[1], h1)
+    ms = o.spin.reshape(-1, 3)
+    np.testing.assert_allclose(h2.reshape(-1, 3), np.cross(ms, h1.reshape(-1, 3) + 1e9) / 2.21e5, rtol=1e-13,
+                               atol=1e-20)
