@@ -151,3 +151,61 @@ def test_batch_with_constant_spin_torques():
     mmx._set_time_fn(s, s.interactions[-1], lambda t: 2.0)
     with pytest.raises(mmx.MmxError):
         mmx.Batch([s])
+
+
+def _cfg_member(rng, precision):
+    """A stochastic-LLG member with the batching workload's ranges: mesh between
+    32x32x4 and 192x192x16, Ms in [4e5, 1.2e6], A in [5e-12, 2e-11], D in
+    [0, 4e-3], Ku in [0, 1e6], 300 K, alpha 0.1, Heun at 1e-14 s."""
+    nx, ny, nz = (int(v) for v in rng.integers([32, 32, 4], [193, 193, 17]))
+    p = dict(Ms=rng.uniform(4e5, 1.2e6), A=rng.uniform(5e-12, 2e-11), D=rng.uniform(0, 4e-3),
+             Ku=rng.uniform(0, 1e6), seed=int(rng.integers(1, 1 << 30)))
+    mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz, pbc="xy")
+    s = mmx.Sim(mesh, integrator="Heun", precision=precision)
+    N = nx * ny * nz
+    mmx.set_Ms(s, p["Ms"])
+    m0 = rng.standard_normal((N, 3)) + np.array([0, 0, 1.0])
+    mmx.init_m0(s, m0.reshape(-1))
+    mmx.add_exch(s, p["A"])
+    mmx.add_dmi(s, p["D"], type="interfacial")
+    mmx.add_anis(s, p["Ku"], axis=(0, 0, 1))
+    mmx.add_thermal_noise(s, 300.0, seed=p["seed"])
+    s.driver.alpha = 0.1
+    mmx.set_driver(s, integrator="Heun", step=1e-14)
+    return s, dict(mesh=(nx, ny, nz), m0=m0.reshape(-1), params=p)
+
+
+@pytest.mark.parametrize("precision", [mmx.F64, mmx.F32])
+def test_batch_workload_sizes(precision):
+    """Members drawn from the batching workload's size range (up to 192x192x16)
+    land exactly where each member advanced alone lands; the smallest FP64
+    member follows the oracle's stochastic Heun steps."""
+    print(f"[case] batch of workload-sized sLLG members FP{precision}", flush=True)
+    mem = [_cfg_member(np.random.default_rng(40 + q), precision) for q in range(6)]
+    ref = [_cfg_member(np.random.default_rng(40 + q), precision) for q in range(6)]
+    b = mmx.Batch([s for s, _ in mem])
+    b.advance(4)
+    for s, _ in ref:
+        mmx.run_step(s, n=4)
+    for (sb, _), (sa, _) in zip(mem, ref):
+        assert np.array_equal(sb.spin, sa.spin)
+    if precision != mmx.F64:
+        return
+    s, c = min(mem, key=lambda x: x[0].n_total)
+    nx, ny, nz = c["mesh"]
+    p = c["params"]
+    N = nx * ny * nz
+    o = OracleSim(nx, ny, nz, 2e-9, 2e-9, 2e-9, pbc="xy")
+    o.set_Ms(np.full(N, p["Ms"]))
+    o.init_m0(c["m0"])
+    o.add_exch(p["A"])
+    o.add_dmi(np.full(N, p["D"]), type="interfacial")
+    o.add_anis(np.full(N, p["Ku"]), axis=(0, 0, 1))
+    o.add_thermal(np.full(N, 300.0), 0.0, p["seed"])
+    o.set_llg(alpha=0.1, gamma=2.21e5, precession=True)
+    o.set_integrator(1, step=1e-14)
+    for _ in range(4):
+        o.advance_step()
+    err = np.abs(s.spin - o.spin).max()
+    print(f"member {nx}x{ny}x{nz} vs oracle after 4 stochastic Heun steps: {err:.2e}", flush=True)
+    assert err < 1e-12
