@@ -588,6 +588,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the copy ceiling, rocFFT and soft-film legs")
     ap.add_argument("--local-only", action="store_true",
                     help="time the same film without its demag term (local-field kernel only) and exit")
+    ap.add_argument("--term-list", action="store_true",
+                    help="time the same film through the term-list kernels (a per-cell damping of 0.1 everywhere, "
+                         "the SpatialLLG driver's path) and exit")
     ap.add_argument("--fd-cells", type=int, default=64, help="finite-difference probes per gradient quantity")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the library's streaming copy of 1 GiB three times (calibrates HBM counters)")
@@ -626,6 +629,15 @@ def main():
         ms = passes["LOCAL"][0]
         log(f"[bench] local-only FP64: {N * 4 * args.steps / el / 1e6:.1f} Mcell-evals/s, "
             f"local kernel {ms * 1e3:.1f} us per launch")
+        return
+    if args.term_list:
+        sim, _ = build_system(mmx, nx, ny, nz, mmx.F64, images=args.images)
+        mmx.set_driver(sim, driver="SpatialLLG", integrator="RungeKutta", step=1e-14)  # alpha 0.1 per cell
+        el = time_steps(sim, mmx, args.steps, args.warmup)
+        passes = profile_passes(sim, mmx, max(5, min(args.steps, 20)))
+        log(f"[bench] term-list FP64: {N * 4 * args.steps / el / 1e6:.1f} Mcell-evals/s, "
+            f"{el / args.steps * 1e3:.3f} ms/step; " +
+            ", ".join(f"{k} {v[0] * 1e3:.1f}us" for k, v in passes.items() if v[0] is not None))
         return
     errors = {}
     log(f"[bench] FP64 set-up {nx}x{ny}x{nz}, images {args.images}")

@@ -176,6 +176,14 @@ enum EpiMode : int {
     EPI_ADJ,         // adjoint of one RHS evaluation (AdjIO below)
 };
 
+// Sets of epilogue modes a term-list kernel is compiled for (bit = EpiMode):
+// each instantiation carries only the code of the modes it serves, which keeps
+// its registers within the file and off the scratch segment.
+constexpr uint32_t MM_ADJ = 1u << EPI_ADJ;
+constexpr uint32_t MM_FIELD = 1u << EPI_FIELD;
+constexpr uint32_t MM_STAGES = (1u << EPI_STAGE) | (1u << EPI_STAGE_SCALED) | (1u << EPI_RK4_FINAL);
+constexpr uint32_t MM_GENERIC = ~MM_ADJ;  // every mode but the adjoint
+
 // Adjoint of k = f(Y) = F(Y, H(Y)) applied to a cotangent a (EPI_ADJ).  The
 // pass's input vector is ut = (dF/dh)^T a / mu0Ms (zero where Ms = 0 or the
 // cell is pinned); the epilogue forms

@@ -1237,7 +1237,9 @@ __device__ __forceinline__ void adj_cell(const MMX_CONST FieldParams<T> &P, cons
 
 // The fused last step of one RHS evaluation for cell I.  `red` accumulates the
 // thread's reduction value (ERR: running max; ENERGY: sum of the masked terms).
-template <class T>
+// MM: the modes this instantiation serves (MM_* of mmx_common.h); a launch
+// picks the instantiation whose set holds E.mode.
+template <class T, uint32_t MM = MM_GENERIC>
 __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
                                             int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
                                             double &red)
@@ -1245,17 +1247,20 @@ __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, c
 #pragma clang fp contract(off)
     const int64_t N = P.N;
     T Hx, Hy, Hz;
-    if (E.mode == EPI_ADJ) {
-        adj_cell<T>(P, dyn, E, I, i, j, k, m, hdx, hdy, hdz);
-        return;
+    if constexpr ((MM & MM_ADJ) != 0) {
+        if (E.mode == EPI_ADJ) {
+            adj_cell<T>(P, dyn, E, I, i, j, k, m, hdx, hdy, hdz);
+            return;
+        }
     }
-    if (E.mode == EPI_ENERGY) {
+    if constexpr (MM == MM_ADJ) return;
+    if ((MM & (1u << EPI_ENERGY)) && E.mode == EPI_ENERGY) {
         double es = 0;
         cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, &es);
         red += es;
         return;
     }
-    if (E.mode == EPI_FIELD) {
+    if ((MM & MM_FIELD) && E.mode == EPI_FIELD) {
         if (E.e_out) {
             double es = 0;
             cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, &es);
@@ -1268,11 +1273,13 @@ __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, c
         E.out[2 * N + I] = Hz;
         return;
     }
+    // nothing below for an instantiation without the RHS, stage and error modes
+    if constexpr ((MM & ~(MM_FIELD | (1u << EPI_ENERGY) | MM_ADJ)) == 0) return;
     cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
     const T mx = m[I], my = m[N + I], mz = m[2 * N + I];
     T kx, ky, kz;
     llg_rhs(P, I, mx, my, mz, Hx, Hy, Hz, kx, ky, kz);
-    if (E.mode == EPI_RHS) {
+    if ((MM & (1u << EPI_RHS)) && E.mode == EPI_RHS) {
         E.out[I] = kx;
         E.out[N + I] = ky;
         E.out[2 * N + I] = kz;
@@ -1289,7 +1296,7 @@ __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, c
         E.out[2 * N + I] = Hz;
     }
     const T kk[3] = {kx, ky, kz};
-    if (E.mode == EPI_ERR) {  // vector_add6b / BS23 error combination, then |err|/(1+|y|)
+    if ((MM & (1u << EPI_ERR)) && E.mode == EPI_ERR) {  // vector_add6b / BS23 error combination, then |err|/(1+|y|)
         double mval = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) {

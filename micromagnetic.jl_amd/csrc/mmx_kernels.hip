@@ -430,9 +430,12 @@ __global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ 
     for (int it = lo + r; it < hi; it += Gx) {
         const int2 w = items[it];
         const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Ps[w.x];
-        const Epi<T> E = st[w.x].E;
+        // the member's epilogue and per-launch values are read in place (a private
+        // copy of these structs would live in scratch: their arrays are indexed
+        // by term)
+        const Epi<T> &E = st[w.x].E;
         const T *m = st[w.x].m;
-        const Dyn d = dyns[w.x];
+        const Dyn &d = dyns[w.x];
         const unsigned nx = P.nx, ny = P.ny;
         const int N = (int)P.N;
         double red = 0;
@@ -443,7 +446,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ 
                 const int i = (int)((unsigned)I - q * nx);
                 const unsigned k = q / ny;
                 const int j = (int)(q - k * ny);
-                epilogue<T, V>(P, d, E, I, i, j, (int)k, m, T(0), T(0), T(0), red);
+                epilogue<T, V, 0u, MM_STAGES>(P, d, E, I, i, j, (int)k, m, T(0), T(0), T(0), red);
             }
         }
     }
@@ -1179,6 +1182,7 @@ int launch_local(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const Epi<T
     else if (v == 1) hipLaunchKernelGGL((k_local<T, 1>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else if (v == 2) hipLaunchKernelGGL((k_local<T, 2>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else if (v == 3) hipLaunchKernelGGL((k_local<T, 3>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
+    else if (v == 5) hipLaunchKernelGGL((k_local<T, 5>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     else hipLaunchKernelGGL((k_local<T, 0>), dim3(g), dim3(BLOCK), 0, s, P, dyn, E, m);
     return g;
 }

@@ -46,6 +46,8 @@ int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, con
             hipLaunchKernelGGL((k_p5<T, H, 2>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
         else if (v == 3)                                                                             \
             hipLaunchKernelGGL((k_p5<T, H, 3>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
+        else if (v == 5)                                                                             \
+            hipLaunchKernelGGL((k_p5<T, H, 5>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
         else                                                                                         \
             hipLaunchKernelGGL((k_p5<T, H, 0>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
     }

@@ -50,7 +50,7 @@ int launch_p5_sig(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P,
                   const T *m, uint32_t sig, const cplx<T> *tw, hipStream_t s)
 {
     const int v = variant(E.mode, 1, E.np);
-    if (v == 0) return -1;
+    if (v == 0 || v == 5) return -1;
     switch (sig) {
     case SIG_CHIRAL: return p5_h<T, SIG_CHIRAL>(v, G, A, P, dyn, E, m, tw, s);
     case SIG_SOFT: return p5_h<T, SIG_SOFT>(v, G, A, P, dyn, E, m, tw, s);
@@ -64,7 +64,7 @@ int launch_local_sig(int64_t N, const FieldParams<T> *P, const Dyn &dyn, const E
                      int grid, hipStream_t s)
 {
     const int v = variant(E.mode, 1, E.np);
-    if (v == 0 || sig != SIG_CHIRAL_LOCAL) return -1;
+    if (v == 0 || v == 5 || sig != SIG_CHIRAL_LOCAL) return -1;
     const dim3 g(grid), b(BLOCK);
     if (v == 4) hipLaunchKernelGGL((k_local<T, 4, SIG_CHIRAL_LOCAL>), g, b, 0, s, P, dyn, E, m);
     else if (v == 1) hipLaunchKernelGGL((k_local<T, 1, SIG_CHIRAL_LOCAL>), g, b, 0, s, P, dyn, E, m);

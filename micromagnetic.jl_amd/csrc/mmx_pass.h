@@ -48,12 +48,15 @@ struct VMode {
     static constexpr bool nokp = V == 4;
 };
 
-template <class T, int V, uint32_t SIG = 0>
+// V: 0 the term list with the modes of MM, 5 the adjoint (term list), 1-4 the
+// slot-form integrator stages (VMode)
+template <class T, int V, uint32_t SIG = 0, uint32_t MM = MM_GENERIC>
 __device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
                                          int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red,
                                          T *ynext = nullptr)
 {
-    if constexpr (V == 0) finish_cell<T>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
+    if constexpr (V == 0) finish_cell<T, MM>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
+    else if constexpr (V == 5) finish_cell<T, MM_ADJ>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
     else finish_fast<T, VMode<V>::mode, VMode<V>::nokp, SIG>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red, ynext);
 }
 
@@ -109,7 +112,9 @@ struct RowWave {
 #ifndef MMX_P5_WPE
 #define MMX_P5_WPE 3
 #endif
-#define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu(MMX_P5_WPE)))
+// the term-list variants (V = 0, 5) hold more live values: 2 waves per SIMD
+// leave them their registers without scratch spills
+#define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu((V == 0 || V == 5) ? 2 : MMX_P5_WPE)))
 // IMG: the launch covers a stack of images (grid y = image), each with its own
 // spectra, spin and field planes; fields and per-cell energies only.
 template <class T, int H, int V, uint32_t SIG = 0, bool IMG = false>
@@ -213,7 +218,8 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
                 for (int c = 0; c < 3; c++) ldsr[2 * (c * RX + rr) * HS + x] = yn[c] * ms;
             }
         } else {
-            epilogue<T, V, SIG>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz, red);
+            epilogue<T, V, SIG, IMG ? MM_FIELD : MM_GENERIC>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz,
+                                                           red);
         }
         x += blockDim.x;
         while (x >= nx) {
@@ -291,9 +297,11 @@ __global__ void __launch_bounds__(BLOCK) k_local(const FieldParams<T> *__restric
     default: break;                                                            \
     }
 
-// epilogue variant for a mode: the slot-form kernels take the integrator stages
+// epilogue variant for a mode: the slot-form kernels take the integrator stages,
+// the adjoint its own term-list instantiation
 inline int variant(int mode, int fast, int np)
 {
+    if (mode == EPI_ADJ) return 5;
     if (!fast) return 0;
     if (mode == EPI_STAGE) return np == 0 ? 4 : 1;
     if (mode == EPI_RK4_FINAL) return 2;

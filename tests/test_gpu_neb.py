@@ -112,9 +112,10 @@ def test_neb_macrospin_barrier_reference():
     """test/test_neb.jl:56-108 on the GPU: NEB(sim, (m0, m1), (1,)) puts the
     one free image at (1,1,0)/sqrt(2); the macrospin band (Kx = 5e4 along x,
     Kz = -2e4 along z, 11 free images) relaxes to the barrier Kx V =
-    5e4 (5e-9)^3 within 1e-3, and its band energies follow the oracle's band
-    relaxed the same way to 1e-3 of the barrier (at stopping_dmdt = 0.001 the
-    images still slide along the band, so the two runs stop a few steps apart;
+    5e4 (5e-9)^3 within 1e-3, and its saddle energy equals that of the
+    oracle's band relaxed the same way to 1e-5 of the barrier (at stopping_dmdt
+    = 0.001 the images still slide along the band, so the two runs stop a few
+    steps apart with the intermediate images at slightly different places;
     step-for-step agreement is checked by test_neb_matches_oracle)."""
     from test_oracle_neb import macrospin_oracle
 
@@ -145,4 +146,5 @@ def test_neb_macrospin_barrier_reference():
     o = macrospin_oracle()
     ref = OracleNEB(o, [np.array([1, 0, 0.0]), np.array([0, 1, 1.0]), np.array([-1, 0, 0.0])], [5, 5])
     ref.relax(max_steps=500, stopping_dmdt=0.001)
-    assert np.max(np.abs(neb.energy - ref.energy)) <= 1e-3 * expected
+    assert abs(neb.energy.max() - ref.energy.max()) <= 1e-5 * expected
+    assert neb.energy[0] == ref.energy[0] and neb.energy[-1] == ref.energy[-1]
