@@ -6,7 +6,8 @@
 // global memory in the same pattern.  Stages are radix E (a final smaller
 // radix when log2 L is not a multiple of log2 E); values move between stages through an
 // LDS exchange buffer of L elements per sequence.  Twiddles come from an FP64-
-// accurate table tw[t] = exp(-2 pi i t / 2^lgLmax) indexed at t << (lgLmax - lgL).
+// accurate table tw[t] = exp(-2 pi i t / 2^lgLmax) indexed at t << (lgLmax - lgL):
+// each butterfly loads W^k and forms W^(2k) .. W^((R-1)k) by multiplication.
 #pragma once
 #include "mmx_common.h"
 
@@ -144,11 +145,14 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int c
         const int kk = jb & (Ns - 1);
         cplx<T> w[Rs];
         if constexpr (Ns > 1) {
+            // one table load per butterfly; the other twiddles are products of
+            // powers (w^r = w^(r/2) w^(r - r/2), at most 3 roundings deep),
+            // which cuts the stage's vector-memory instructions from Rs-1 to 1
+            cplx<T> w1 = tw[(kk * TWSTEP) << twsh];
+            if (INV) w1.y = -w1.y;
+            w[1] = w1;
 #pragma unroll
-            for (int r = 1; r < Rs; r++) {
-                w[r] = tw[(r * kk * TWSTEP) << twsh];
-                if (INV) w[r].y = -w[r].y;
-            }
+            for (int r = 2; r < Rs; r++) w[r] = cmul(w[r >> 1], w[r - (r >> 1)]);
         }
 #pragma unroll
         for (int c = 0; c < NC; c++) {
