@@ -359,7 +359,6 @@ struct Engine : CtxBase {
             P.slot[sk] = i;
             P.ord[i] = sk;
         }
-        if (alphas.p) P.fast = 0;  // per-cell damping: term-list kernels
         // packed switches of the slot form
         {
             uint32_t f = (P.px ? FL_PX : 0u) | (P.py ? FL_PY : 0u) | (P.pz ? FL_PZ : 0u);

@@ -28,6 +28,23 @@ __device__ __forceinline__ X gld(const X *p, int64_t i)
 {
     return ((const __attribute__((address_space(1))) X *)p)[i];
 }
+// Load of the two values p[2i], p[2i+1] (p 2*sizeof(X)-aligned) as one vector
+// access through the global address space.
+template <class X>
+__device__ __forceinline__ void gld2(const X *p, int64_t i, X &a, X &b)
+{
+    typedef X v2 __attribute__((vector_size(2 * sizeof(X))));
+    const v2 v = ((const __attribute__((address_space(1))) v2 *)p)[i];
+    a = v[0];
+    b = v[1];
+}
+// Loads from LDS through a pointer the compiler cannot trace to a __shared__
+// array (LDS address space: ds_read, not flat).
+template <class X>
+__device__ __forceinline__ X lld(const X *p, int i)
+{
+    return ((const __attribute__((address_space(3))) X *)p)[i];
+}
 constexpr int BLOCK = 256;
 
 enum TermKind : int {
@@ -257,6 +274,26 @@ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b)
 {
     return {a.x - b.x, a.y - b.y};
 }
+
+// Quad permute across lanes (DPP, a VALU move: no LDS or memory access):
+// lane 4a+q returns the value of lane 4a+((CTRL >> 2q) & 3).  Every lane of
+// the quad must be active.
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+constexpr int QP_XOR1 = 0xB1;                                        // [1,0,3,2]
+template <int Q>
+constexpr int QP_BCAST = Q | (Q << 2) | (Q << 4) | (Q << 6);         // [Q,Q,Q,Q]
 
 // Demag convolution geometry.  Padded lengths are powers of two >= 2n-1
 // (any such length gives the same linear convolution as the reference's

@@ -566,12 +566,23 @@ __device__ __forceinline__ void gst(X *p, int64_t i, X v)
     ((__attribute__((address_space(1))) X *)p)[i] = v;
 }
 
+// m of a block's x rows staged in LDS by the last demag pass: component c of
+// the cell's own row at p[c*cs + x], the block's row r' = r + d at p + d*rs
+// for 0 <= r' < nrows (the rows are consecutive z at one y).
+template <class T>
+struct RowTile {
+    const T *p;
+    int cs, rs, r, nrows;
+};
+
 // Field of cell I in slot form (see SlotCfg); returns m of the cell and its
-// neighbour / pin mask.
-template <class T, uint32_t SIG>
+// neighbour / pin mask.  TILE: m of the cell, of its x neighbours and of the
+// z neighbours inside the block come from the LDS row tile tl.
+template <class T, uint32_t SIG, bool TILE = false>
 __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, int I, int i, int j,
                                                 int k, const T *__restrict__ m, T hdx, T hdy, T hdz, T &mx, T &my,
-                                                T &mz, T &Hx, T &Hy, T &Hz, uint32_t &msk)
+                                                T &mz, T &Hx, T &Hy, T &Hz, uint32_t &msk,
+                                                const RowTile<T> &tl = RowTile<T>{})
 {
 #pragma clang fp contract(off)
     const SlotCfg<T> C = slot_cfg(P);
@@ -584,20 +595,32 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
     // ---- loads ----
     msk = gld(C.nbm, I);
     const T Ms = gld(C.mu0Ms, I);
-    mx = gld(m, I);
-    my = gld(m, N + I);
-    mz = gld(m, 2 * N + I);
+    if constexpr (TILE) {
+        mx = lld(tl.p, i);
+        my = lld(tl.p, tl.cs + i);
+        mz = lld(tl.p, 2 * tl.cs + i);
+    } else {
+        mx = gld(m, I);
+        my = gld(m, N + I);
+        mz = gld(m, 2 * N + I);
+    }
     const bool stencil = ex || bd || idm;
     int nb[6];
+    // TILE: x index of each x neighbour in the row; block-row offset of each z
+    // neighbour and whether that row is one of the block's
+    int tx[2] = {i, i}, tz[2] = {0, 0};
+    bool tzin[2] = {false, false};
     {
         const int nx = C.nx, ny = C.ny, nz = C.nz, sxy = nx * ny;
         int t;
         t = i - 1;
         if ((fl & FL_PX) && t < 0) t += nx;
         nb[0] = (t >= 0) ? I - i + t : I;
+        tx[0] = (t >= 0) ? t : i;
         t = i + 1;
         if ((fl & FL_PX) && t >= nx) t -= nx;
         nb[1] = (t < nx) ? I - i + t : I;
+        tx[1] = (t < nx) ? t : i;
         t = j - 1;
         if ((fl & FL_PY) && t < 0) t += ny;
         nb[2] = (t >= 0) ? I + (t - j) * nx : I;
@@ -607,9 +630,17 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
         t = k - 1;
         if ((fl & FL_PZ) && t < 0) t += nz;
         nb[4] = (t >= 0) ? I + (t - k) * sxy : I;
+        if constexpr (TILE) {
+            tz[0] = (t >= 0) ? t - k : 0;
+            tzin[0] = tl.r + tz[0] >= 0 && tl.r + tz[0] < tl.nrows;
+        }
         t = k + 1;
         if ((fl & FL_PZ) && t >= nz) t -= nz;
         nb[5] = (t < nz) ? I + (t - k) * sxy : I;
+        if constexpr (TILE) {
+            tz[1] = (t < nz) ? t - k : 0;
+            tzin[1] = tl.r + tz[1] >= 0 && tl.r + tz[1] < tl.nrows;
+        }
     }
     T nmx[6], nmy[6], nmz[6];
 #pragma unroll
@@ -617,6 +648,21 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
     if (stencil) {
 #pragma unroll
         for (int q = 0; q < 6; q++) {
+            if constexpr (TILE) {
+                if (q < 2) {
+                    nmx[q] = lld(tl.p, tx[q]);
+                    nmy[q] = lld(tl.p, tl.cs + tx[q]);
+                    nmz[q] = lld(tl.p, 2 * tl.cs + tx[q]);
+                    continue;
+                }
+                if (q >= 4 && tzin[q - 4]) {
+                    const int o = tz[q - 4] * tl.rs + i;
+                    nmx[q] = lld(tl.p, o);
+                    nmy[q] = lld(tl.p, tl.cs + o);
+                    nmz[q] = lld(tl.p, 2 * tl.cs + o);
+                    continue;
+                }
+            }
             nmx[q] = gld(m, nb[q]);
             nmy[q] = gld(m, N + nb[q]);
             nmz[q] = gld(m, 2 * N + nb[q]);
@@ -797,10 +843,10 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
 
 // Epilogue of the specialised kernels: MODE is EPI_STAGE, EPI_RK4_FINAL or EPI_ERR.
 // ynext (nullable): receives the stored new state of the cell (STAGE / RK4_FINAL).
-template <class T, int MODE, bool NOKP, uint32_t SIG>
+template <class T, int MODE, bool NOKP, uint32_t SIG, bool TILE = false>
 __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int I,
                                             int i, int j, int k, const T *__restrict__ m, T hdx, T hdy, T hdz,
-                                            double &red, T *ynext = nullptr)
+                                            double &red, T *ynext = nullptr, const RowTile<T> &tl = RowTile<T>{})
 {
 #pragma clang fp contract(off)
     const int N = (int)P.N;
@@ -820,13 +866,17 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
     }
     T mx, my, mz, Hx, Hy, Hz;
     uint32_t msk;
-    cell_field_fast<T, SIG>(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk);
+    cell_field_fast<T, SIG, TILE>(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk, tl);
     // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
     T kk[3];
     if ((msk >> 6) & 1u) {
         kk[0] = kk[1] = kk[2] = T(0);
     } else {
-        const T alpha = alpha_, a = llg_a_;
+        T alpha = alpha_, a = llg_a_;
+        if (P.alphas) {  // spatial_llg_rhs_kernel! (llg.jl:62-97): alpha per cell
+            alpha = gld(P.alphas, I);
+            a = -P.gamma / (T(1) + alpha * alpha);
+        }
         const T mm = mx * mx + my * my + mz * mz;
         const T mh = mx * Hx + my * Hy + mz * Hz;
         const T h1 = mm * Hx - mh * mx, h2 = mm * Hy - mh * my, h3 = mm * Hz - mh * mz;

@@ -305,6 +305,13 @@ struct P3W {
 #ifndef MMX_P3W_PF
 #define MMX_P3W_PF 1
 #endif
+// FP64 tensor loads in kx pairs: the two lanes of a kx pair (t even/odd) load
+// 16 bytes each of three components for both kx and swap halves with a DPP
+// move, and a quad of lanes (same j) loads the four Lline[kzf] once and
+// broadcasts them: 12 + 1 vector loads per lane instead of 24 + 4
+#ifndef MMX_P3W_PAIR
+#define MMX_P3W_PAIR 1
+#endif
 template <class T, int L>
 __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict__ B, const T *__restrict__ Kt,
                                                const T *__restrict__ Lline, int n_in, int nzb,
@@ -319,7 +326,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     constexpr int NB = MMX_P3_NBW;
     constexpr int WS = TW * LS;
     __shared__ cplx<T> lds[4 * NB * WS];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int t = lane % TW, j = lane / TW;
     constexpr int WX = P3W<T>::template wx<L>();
     const int half = WX == 2 ? wave >> 1 : 0;
@@ -348,21 +355,63 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     const int64_t cs = (int64_t)G.Py2 * KS * Pz2;
     const T *kb = Kt + (((int64_t)kyf * (KS >> 2) + (kxs >> 2)) * Pz2) * 4 + (kxs & 3);
     T kxx[EPT], kyy[EPT], kzz[EPT], kxy[EPT], kxz[EPT], kyz[EPT];
-    auto load_tensor = [&]() {
+    constexpr bool PAIR = MMX_P3W_PAIR && sizeof(T) == 8 && TW % 4 == 0 && EPT == 4;
+    auto load_tensor_pairs = [&]() {
         const T lyz = Lline[Pz2 + G.Py2 + kxs];
         const T lxz = Lline[Pz2 + kyf];
+        // lane (j, t) loads Lline at its quad's kz = j + (t&3)*TPS; m is taken from lane m of the quad
+        const int kzq = j + (t & 3) * TPS;
+        const T lq = Lline[kzq <= L / 2 ? kzq : L - kzq];
+        T lz[4];
+        lz[0] = quad_perm<QP_BCAST<0>>(lq);
+        lz[1] = quad_perm<QP_BCAST<1>>(lq);
+        lz[2] = quad_perm<QP_BCAST<2>>(lq);
+        lz[3] = quad_perm<QP_BCAST<3>>(lq);
+        // kx pair (2pk, 2pk+1), clamped into the padded rows (lanes past them hold no data)
+        const int e = t & 1;
+        const int pk = min(kx >> 1, (KS >> 1) - 1);
+        const T *kbp = Kt + (((int64_t)kyf * (KS >> 2) + (pk >> 1)) * Pz2) * 4 + 2 * (pk & 1) + e * cs;
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
             const int kz = j + m * TPS;
             const int kzf = (kz <= L / 2) ? kz : L - kz;
             const T sz = (kz <= L / 2) ? T(1) : T(-1);
-            const T *kp = kb + kzf * 4;
-            kxx[m] = kp[0];
-            kyy[m] = kp[cs];
-            kzz[m] = kp[2 * cs];
-            kxy[m] = sy * kp[3 * cs] + Lline[kzf];
-            kxz[m] = sz * kp[4 * cs] + lxz;
-            kyz[m] = sy * sz * kp[5 * cs] + lyz;
+            const T *kp = kbp + kzf * 4;
+            // even lanes load components 0, 2, 4 (xx, zz, xz), odd lanes 1, 3, 5 (yy, xy, yz)
+            const cplx<T> a = *reinterpret_cast<const cplx<T> *>(kp);
+            const cplx<T> b = *reinterpret_cast<const cplx<T> *>(kp + 2 * cs);
+            const cplx<T> c = *reinterpret_cast<const cplx<T> *>(kp + 4 * cs);
+            const T ra = quad_perm<QP_XOR1>(e ? a.x : a.y);
+            const T rb = quad_perm<QP_XOR1>(e ? b.x : b.y);
+            const T rc = quad_perm<QP_XOR1>(e ? c.x : c.y);
+            const T oa = e ? a.y : a.x, ob = e ? b.y : b.x, oc = e ? c.y : c.x;
+            kxx[m] = e ? ra : oa;
+            kyy[m] = e ? oa : ra;
+            kzz[m] = e ? rb : ob;
+            kxy[m] = sy * (e ? ob : rb) + lz[m];
+            kxz[m] = sz * (e ? rc : oc) + lxz;
+            kyz[m] = sy * sz * (e ? oc : rc) + lyz;
+        }
+    };
+    auto load_tensor = [&]() {
+        if constexpr (PAIR) {
+            load_tensor_pairs();
+        } else {
+            const T lyz = Lline[Pz2 + G.Py2 + kxs];
+            const T lxz = Lline[Pz2 + kyf];
+#pragma unroll
+            for (int m = 0; m < EPT; m++) {
+                const int kz = j + m * TPS;
+                const int kzf = (kz <= L / 2) ? kz : L - kz;
+                const T sz = (kz <= L / 2) ? T(1) : T(-1);
+                const T *kp = kb + kzf * 4;
+                kxx[m] = kp[0];
+                kyy[m] = kp[cs];
+                kzz[m] = kp[2 * cs];
+                kxy[m] = sy * kp[3 * cs] + Lline[kzf];
+                kxz[m] = sz * kp[4 * cs] + lxz;
+                kyz[m] = sy * sz * kp[5 * cs] + lyz;
+            }
         }
     };
     if constexpr (MMX_P3W_PF) load_tensor();

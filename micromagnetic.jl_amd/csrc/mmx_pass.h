@@ -50,14 +50,19 @@ struct VMode {
 
 // V: 0 the term list with the modes of MM, 5 the adjoint (term list), 1-4 the
 // slot-form integrator stages (VMode)
-template <class T, int V, uint32_t SIG = 0, uint32_t MM = MM_GENERIC>
+// TILE (slot form only): m of the cell's row and of the block's other rows
+// comes from the LDS row tile tl.
+template <class T, int V, uint32_t SIG = 0, uint32_t MM = MM_GENERIC, bool TILE = false>
 __device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, const Dyn &dyn, const Epi<T> &E, int64_t I,
                                          int i, int j, int k, const T *__restrict__ m, T hx, T hy, T hz, double &red,
-                                         T *ynext = nullptr)
+                                         T *ynext = nullptr, const RowTile<T> &tl = RowTile<T>{})
 {
+    static_assert(!TILE || (V >= 1 && V <= 4), "the row tile serves the slot-form epilogues");
     if constexpr (V == 0) finish_cell<T, MM>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
     else if constexpr (V == 5) finish_cell<T, MM_ADJ>(P, dyn, E, I, i, j, k, m, hx, hy, hz, red);
-    else finish_fast<T, VMode<V>::mode, VMode<V>::nokp, SIG>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red, ynext);
+    else
+        finish_fast<T, VMode<V>::mode, VMode<V>::nokp, SIG, TILE>(P, dyn, E, (int)I, i, j, k, m, hx, hy, hz, red,
+                                                                  ynext, tl);
 }
 
 // Real-to-complex x transform tail shared by the first and the last pass: v
@@ -107,6 +112,13 @@ struct RowWave {
     static constexpr bool on = MMX_ROWWAVE && FftShape<H>::TPS <= 64;
 };
 
+// Slot-form last pass: the threads that transformed a row also stage m of
+// that row in the unused upper half of the row's LDS buffer, so the cell loop
+// reads m of the cell, of its x neighbours and of its in-block z neighbours
+// from LDS (one 16-byte load per two cells instead of up to 12 loads per cell)
+#ifndef MMX_P5_TILE
+#define MMX_P5_TILE 1
+#endif
 // at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
 // latency-bound, and the register budget this sets costs no spills
 #ifndef MMX_P5_WPE
@@ -186,8 +198,41 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
     }
     if constexpr (MIR) exchange_sync<true>();
     fft_reg<T, H, true, 8, RowWave<H>::on>(v, j, buf, tw, G.lgLmax);
+    // Row buffer (c, r) as 2*HS reals: the field of cell x at x (x < nx) and,
+    // with the tile, m of the row at mo + x (mo + nx <= 2H + 2 since nx <= H).
+    // The row's threads are the ones that just ended its transform, so no
+    // barrier is needed before they overwrite its exchange buffer.
+    constexpr bool TL = MMX_P5_TILE && V >= 1 && V <= 4 && RowWave<H>::on;
+    const int mo = (nx + 1) & ~1;
 #pragma unroll
-    for (int q = 0; q < EPT; q++) buf[j + q * TPS] = v[q];
+    for (int q = 0; q < EPT; q++) {
+        const int k = j + q * TPS;
+        if (!TL || 2 * k < nx) buf[k] = v[q];
+    }
+    if constexpr (TL) {
+        if (active) {
+            const T *mrow = m + (int64_t)c * nx * ny * nz + ((int64_t)(z0 + r) * ny + y0) * nx;
+            T *dst = reinterpret_cast<T *>(buf) + mo;
+            if ((nx & 1) == 0 && ((uintptr_t)mrow & (2 * sizeof(T) - 1)) == 0) {
+#pragma unroll
+                for (int q = 0; q < EPT; q++) {
+                    const int k = j + q * TPS;
+                    if (2 * k < nx) {
+                        cplx<T> w;
+                        gld2(mrow, k, w.x, w.y);
+                        reinterpret_cast<cplx<T> *>(dst)[k] = w;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < EPT; q++) {
+                    const int k = j + q * TPS;
+                    if (2 * k < nx) dst[2 * k] = gld(mrow, 2 * k);
+                    if (2 * k + 1 < nx) dst[2 * k + 1] = gld(mrow, 2 * k + 1);
+                }
+            }
+        }
+    }
     __syncthreads();
     const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
     double red = 0;
@@ -208,18 +253,20 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         const T hx = (x & 1) ? h0.y : h0.x;
         const T hy = (x & 1) ? h1.y : h1.x;
         const T hz = (x & 1) ? h2.y : h2.x;
+        RowTile<T> tl{};
+        if constexpr (TL) tl = RowTile<T>{ldsr + 2 * rr * HS + mo, 2 * RX * HS, 2 * HS, rr, nrows};
         if constexpr (NXOK) {
             T yn[3];
             const int I = rw * nx + x;
-            epilogue<T, V, SIG>(P, dyn, E, (int64_t)I, x, y, z, m, hx, hy, hz, red, yn);
+            epilogue<T, V, SIG, MM_GENERIC, TL>(P, dyn, E, (int64_t)I, x, y, z, m, hx, hy, hz, red, yn, tl);
             if (nx_on) {
                 const T ms = gld(P.mu0Ms, I);
 #pragma unroll
                 for (int c = 0; c < 3; c++) ldsr[2 * (c * RX + rr) * HS + x] = yn[c] * ms;
             }
         } else {
-            epilogue<T, V, SIG, IMG ? MM_FIELD : MM_GENERIC>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy, hz,
-                                                           red);
+            epilogue<T, V, SIG, IMG ? MM_FIELD : MM_GENERIC, TL>(P, dyn, E, (int64_t)rw * nx + x, x, y, z, m, hx, hy,
+                                                               hz, red, nullptr, tl);
         }
         x += blockDim.x;
         while (x >= nx) {

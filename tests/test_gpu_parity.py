@@ -492,16 +492,20 @@ def test_spatial_llg_driver_set_alpha():
         mmx.add_thermal_noise(sim, 300.0)
 
 
+@pytest.mark.parametrize("form", ["slot", "list"])
 @pytest.mark.parametrize("integrator", ["RungeKutta", "DormandPrince"])
-def test_spatial_alpha_trajectory(oracle, integrator):
-    """Per-cell damping (spatial_llg_rhs_kernel!, llg.jl:62-97) through the
-    term-list kernels, with demag, against the oracle."""
+def test_spatial_alpha_trajectory(oracle, integrator, form):
+    """Per-cell damping (spatial_llg_rhs_kernel!, llg.jl:62-97) with demag
+    against the oracle, through the slot-form kernels and (a kind of term
+    added twice) through the term-list kernels."""
     import mmx
 
-    note(f"spatial alpha {integrator}")
+    note(f"spatial alpha {integrator} {form}")
     nx, ny, nz = 12, 9, 3
     N = nx * ny * nz
     terms = [("exch", dict(A=1.3e-11)), ("dmi", dict(D=2e-3, type="interfacial")), ("demag", {})]
+    if form == "list":
+        terms += [("zeeman", dict(H=(1e4, 0.0, 2e4))), ("zeeman", dict(H=(0.0, -3e4, 0.0)))]
     g, o = build(oracle, nx, ny, nz, pbc="y", Ms=np.full(N, 6e5), terms=terms, seed=21)
     alphas = np.random.default_rng(22).uniform(0.01, 0.8, N)
     if integrator == "RungeKutta":
