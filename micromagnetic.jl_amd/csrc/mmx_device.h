@@ -655,11 +655,15 @@ __device__ __forceinline__ void cell_field_fast(const MMX_CONST FieldParams<T> &
                     nmz[q] = lld(tl.p, 2 * tl.cs + tx[q]);
                     continue;
                 }
-                if (q >= 4 && tzin[q - 4]) {
-                    const int o = tz[q - 4] * tl.rs + i;
-                    nmx[q] = lld(tl.p, o);
-                    nmy[q] = lld(tl.p, tl.cs + o);
-                    nmz[q] = lld(tl.p, 2 * tl.cs + o);
+                if (q >= 4) {
+                    // one generic (flat) access reads the tile or HBM per lane:
+                    // no branch, so these loads issue with the others
+                    const bool in = tzin[q - 4];
+                    const T *px = in ? tl.p + tz[q - 4] * tl.rs + i : m + nb[q];
+                    const int64_t st = in ? (int64_t)tl.cs : (int64_t)N;
+                    nmx[q] = px[0];
+                    nmy[q] = px[st];
+                    nmz[q] = px[2 * st];
                     continue;
                 }
             }
@@ -864,6 +868,7 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
 #pragma unroll
             for (int c = 0; c < 3; c++) kpv[q][c] = gld(E.kp[q], c * N + I);
     }
+    const T alpha_c = P.alphas ? gld(P.alphas, I) : alpha_;
     T mx, my, mz, Hx, Hy, Hz;
     uint32_t msk;
     cell_field_fast<T, SIG, TILE>(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk, tl);
@@ -874,7 +879,7 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
     } else {
         T alpha = alpha_, a = llg_a_;
         if (P.alphas) {  // spatial_llg_rhs_kernel! (llg.jl:62-97): alpha per cell
-            alpha = gld(P.alphas, I);
+            alpha = alpha_c;
             a = -P.gamma / (T(1) + alpha * alpha);
         }
         const T mm = mx * mx + my * my + mz * mz;

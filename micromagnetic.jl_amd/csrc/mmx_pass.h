@@ -93,7 +93,9 @@ __device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T>
         cplx<T> Fe = {(Zk.x + Zc.x) * T(0.5), (Zk.y + Zc.y) * T(0.5)};
         cplx<T> d = csub(Zk, Zc);
         cplx<T> Fo = {d.y * T(0.5), -d.x * T(0.5)};
-        dst[k] = cadd(Fe, cmul(tw[k << sh], Fo));
+        cplx<T> w;
+        gld2(reinterpret_cast<const T *>(tw), (int64_t)(k << sh), w.x, w.y);  // global, not flat, after the asm
+        dst[k] = cadd(Fe, cmul(w, Fo));
         if (k == 0) dst[H] = csub(Fe, Fo);  // W_Px^H = -1
     }
 }
@@ -210,26 +212,23 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         if (!TL || 2 * k < nx) buf[k] = v[q];
     }
     if constexpr (TL) {
+        // indices past the row are clamped to its last element, whose value
+        // they store again: no per-element branch, so the loads issue together
         if (active) {
             const T *mrow = m + (int64_t)c * nx * ny * nz + ((int64_t)(z0 + r) * ny + y0) * nx;
             T *dst = reinterpret_cast<T *>(buf) + mo;
             if ((nx & 1) == 0 && ((uintptr_t)mrow & (2 * sizeof(T) - 1)) == 0) {
+                cplx<T> w[EPT];
 #pragma unroll
-                for (int q = 0; q < EPT; q++) {
-                    const int k = j + q * TPS;
-                    if (2 * k < nx) {
-                        cplx<T> w;
-                        gld2(mrow, k, w.x, w.y);
-                        reinterpret_cast<cplx<T> *>(dst)[k] = w;
-                    }
-                }
+                for (int q = 0; q < EPT; q++) gld2(mrow, min(j + q * TPS, nx / 2 - 1), w[q].x, w[q].y);
+#pragma unroll
+                for (int q = 0; q < EPT; q++) reinterpret_cast<cplx<T> *>(dst)[min(j + q * TPS, nx / 2 - 1)] = w[q];
             } else {
+                T w[2 * EPT];
 #pragma unroll
-                for (int q = 0; q < EPT; q++) {
-                    const int k = j + q * TPS;
-                    if (2 * k < nx) dst[2 * k] = gld(mrow, 2 * k);
-                    if (2 * k + 1 < nx) dst[2 * k + 1] = gld(mrow, 2 * k + 1);
-                }
+                for (int q = 0; q < 2 * EPT; q++) w[q] = gld(mrow, min(2 * j + (q >> 1) * 2 * TPS + (q & 1), nx - 1));
+#pragma unroll
+                for (int q = 0; q < 2 * EPT; q++) dst[min(2 * j + (q >> 1) * 2 * TPS + (q & 1), nx - 1)] = w[q];
             }
         }
     }
@@ -258,9 +257,10 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         if constexpr (NXOK) {
             T yn[3];
             const int I = rw * nx + x;
+            // read with the cell's other loads (the field reads the same value)
+            const T ms = gld(P.mu0Ms, I);
             epilogue<T, V, SIG, MM_GENERIC, TL>(P, dyn, E, (int64_t)I, x, y, z, m, hx, hy, hz, red, yn, tl);
             if (nx_on) {
-                const T ms = gld(P.mu0Ms, I);
 #pragma unroll
                 for (int c = 0; c < 3; c++) ldsr[2 * (c * RX + rr) * HS + x] = yn[c] * ms;
             }
