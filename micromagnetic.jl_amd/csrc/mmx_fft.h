@@ -7,7 +7,8 @@
 // radix when log2 L is not a multiple of log2 E); values move between stages through an
 // LDS exchange buffer of L elements per sequence.  Twiddles come from an FP64-
 // accurate table tw[t] = exp(-2 pi i t / 2^lgLmax) indexed at t << (lgLmax - lgL):
-// each butterfly loads W^k and forms W^(2k) .. W^((R-1)k) by multiplication.
+// each butterfly loads W^k (all stages' loads issue before the first stage)
+// and forms W^(2k) .. W^((R-1)k) by multiplication.
 #pragma once
 #include "mmx_common.h"
 
@@ -123,6 +124,37 @@ __device__ __forceinline__ void exchange_sync()
     else __syncthreads();
 }
 
+// Twiddle bases of every stage of a length-L transform for thread j, loaded
+// together before the first stage so that their latency overlaps it instead
+// of stalling each later stage.  The stage at span Ns keeps W^(kk*L/(Ns*Rs))
+// (kk = jb mod Ns) for each of its NQ butterflies; the other twiddles of a
+// butterfly are products of its powers (w^r = w^(r/2) w^(r - r/2), at most
+// 3 roundings deep), so a stage costs one table load per butterfly.
+template <class T, int L, int E, int Ns = 1>
+struct TwSet {
+    static constexpr int EPT = FftShape<L, E>::EPT, TPS = FftShape<L, E>::TPS;
+    static constexpr int Rs = (L / Ns) >= EPT ? EPT : L / Ns;
+    static constexpr int NQ = EPT / Rs;
+    static constexpr int TWSTEP = L / (Ns * Rs);
+    cplx<T> w[NQ];
+    TwSet<T, L, E, Ns * Rs> next;
+    __device__ __forceinline__ void load(int j, const cplx<T> *__restrict__ tw, int twsh)
+    {
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int kk = (j + q * TPS) & (Ns - 1);
+                gld2(reinterpret_cast<const T *>(tw), (int64_t)((kk * TWSTEP) << twsh), w[q].x, w[q].y);
+            }
+        }
+        next.load(j, tw, twsh);
+    }
+};
+template <class T, int L, int E>
+struct TwSet<T, L, E, L> {
+    __device__ __forceinline__ void load(int, const cplx<T> *__restrict__, int) {}
+};
+
 // One Stockham stage of radix Rs at span Ns over NC sequences held by the same
 // threads (v[c*EPT + m]; sequence c exchanges through lds + c*cs).  Butterfly
 // jb = j + q*TPS (q < EPT/Rs) reads v[q + r*EPT/Rs] = x[jb + r*L/Rs], applies
@@ -131,13 +163,11 @@ __device__ __forceinline__ void exchange_sync()
 // NC sequences share each exchange's barriers and twiddles.  WAVE: all threads
 // of each sequence are lanes of one wave (exchange without block barriers).
 template <class T, int L, int E, int Rs, int Ns, bool INV, int NC, bool WAVE = false>
-__device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
-                                          int twsh)
+__device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *wb)
 {
     constexpr int EPT = FftShape<L, E>::EPT, TPS = FftShape<L, E>::TPS;
     constexpr int NQ = EPT / Rs;
     constexpr bool LAST = (Ns * Rs == L);
-    constexpr int TWSTEP = L / (Ns * Rs);
     cplx<T> o[LAST ? NC * EPT : 1];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
@@ -145,10 +175,7 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int c
         const int kk = jb & (Ns - 1);
         cplx<T> w[Rs];
         if constexpr (Ns > 1) {
-            // one table load per butterfly; the other twiddles are products of
-            // powers (w^r = w^(r/2) w^(r - r/2), at most 3 roundings deep),
-            // which cuts the stage's vector-memory instructions from Rs-1 to 1
-            cplx<T> w1 = tw[(kk * TWSTEP) << twsh];
+            cplx<T> w1 = wb[q];
             if (INV) w1.y = -w1.y;
             w[1] = w1;
 #pragma unroll
@@ -188,16 +215,23 @@ __device__ __forceinline__ void fft_stage(cplx<T> *v, int j, cplx<T> *lds, int c
 }
 
 template <class T, int L, int E, int Ns, bool INV, int NC, bool WAVE = false>
-__device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
-                                           int twsh)
+__device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int cs, const TwSet<T, L, E, Ns> &ts)
 {
     if constexpr (Ns < L) {
-        constexpr int rem = L / Ns;
-        constexpr int EPT = FftShape<L, E>::EPT;
-        constexpr int Rs = rem >= EPT ? EPT : rem;
-        fft_stage<T, L, E, Rs, Ns, INV, NC, WAVE>(v, j, lds, cs, tw, twsh);
-        fft_stages<T, L, E, Ns * Rs, INV, NC, WAVE>(v, j, lds, cs, tw, twsh);
+        constexpr int Rs = TwSet<T, L, E, Ns>::Rs;
+        fft_stage<T, L, E, Rs, Ns, INV, NC, WAVE>(v, j, lds, cs, ts.w);
+        fft_stages<T, L, E, Ns * Rs, INV, NC, WAVE>(v, j, lds, cs, ts.next);
     }
+}
+
+// Twiddle bases of thread j for a length-L transform (shared by the forward
+// and the inverse transform of the same length).
+template <class T, int L, int E = 8>
+__device__ __forceinline__ TwSet<T, L, E> fft_twiddles(int j, const cplx<T> *__restrict__ tw, int lgLmax)
+{
+    TwSet<T, L, E> ts;
+    ts.load(j, tw, lgLmax - FftShape<L, E>::LG);
+    return ts;
 }
 
 // Transform of one sequence.  Threads of the same block must all call this
@@ -206,7 +240,7 @@ __device__ __forceinline__ void fft_stages(cplx<T> *v, int j, cplx<T> *lds, int 
 template <class T, int L, bool INV, int E = 8, bool WAVE = false>
 __device__ __forceinline__ void fft_reg(cplx<T> *v, int j, cplx<T> *lds, const cplx<T> *__restrict__ tw, int lgLmax)
 {
-    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, 1, WAVE>(v, j, lds, 0, tw, lgLmax - FftShape<L, E>::LG);
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, 1, WAVE>(v, j, lds, 0, fft_twiddles<T, L, E>(j, tw, lgLmax));
 }
 
 // NC sequences per thread (v[c*EPT + m]), exchanging through lds + c*cs.
@@ -214,17 +248,16 @@ template <class T, int L, bool INV, int E, int NC>
 __device__ __forceinline__ void fft_reg_multi(cplx<T> *v, int j, cplx<T> *lds, int cs, const cplx<T> *__restrict__ tw,
                                               int lgLmax)
 {
-    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC>(v, j, lds, cs, tw, lgLmax - FftShape<L, E>::LG);
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC>(v, j, lds, cs, fft_twiddles<T, L, E>(j, tw, lgLmax));
 }
 
 // NC sequences per thread whose threads all sit in one wave: the exchanges
-// synchronise the wave only.
+// synchronise the wave only.  ts: the twiddle bases from fft_twiddles.
 template <class T, int L, bool INV, int E, int NC>
-__device__ __forceinline__ void fft_wave_multi(cplx<T> *v, int j, cplx<T> *lds, int cs,
-                                               const cplx<T> *__restrict__ tw, int lgLmax)
+__device__ __forceinline__ void fft_wave_multi(cplx<T> *v, int j, cplx<T> *lds, int cs, const TwSet<T, L, E> &ts)
 {
     static_assert(FftShape<L, E>::TPS <= 64, "a sequence must fit in one wave");
-    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC, true>(v, j, lds, cs, tw, lgLmax - FftShape<L, E>::LG);
+    if constexpr (L > 1) fft_stages<T, L, E, 1, INV, NC, true>(v, j, lds, cs, ts);
 }
 
 }  // namespace mmx

@@ -415,12 +415,14 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
         }
     };
     if constexpr (MMX_P3W_PF) load_tensor();
+    // one set of twiddle bases serves the forward and the inverse transform
+    const TwSet<T, L, P3W<T>::E> ts = fft_twiddles<T, L, P3W<T>::E>(j, tw, G.lgLmax);
     cplx<T> *buf = lds + wave * NB * WS + t * LS;
     if constexpr (NB == 3) {
-        fft_wave_multi<T, L, false, P3W<T>::E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
+        fft_wave_multi<T, L, false, P3W<T>::E, 3>(&v[0][0], j, buf, WS, ts);
     } else {
 #pragma unroll
-        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, false, P3W<T>::E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
+        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, false, P3W<T>::E, 1>(v[c], j, buf, 0, ts);
     }
     if constexpr (!MMX_P3W_PF) load_tensor();
 #pragma unroll
@@ -432,10 +434,10 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
         v[2][m] = {xz * Mx.x + yz * My.x + zz * Mz.x, xz * Mx.y + yz * My.y + zz * Mz.y};
     }
     if constexpr (NB == 3) {
-        fft_wave_multi<T, L, true, P3W<T>::E, 3>(&v[0][0], j, buf, WS, tw, G.lgLmax);
+        fft_wave_multi<T, L, true, P3W<T>::E, 3>(&v[0][0], j, buf, WS, ts);
     } else {
 #pragma unroll
-        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, true, P3W<T>::E, 1>(v[c], j, buf, 0, tw, G.lgLmax);
+        for (int c = 0; c < 3; c++) fft_wave_multi<T, L, true, P3W<T>::E, 1>(v[c], j, buf, 0, ts);
     }
     if (!ok) return;
     const int nz = G.nz;

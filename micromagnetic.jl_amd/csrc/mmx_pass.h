@@ -288,15 +288,13 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         const int jt = tid % TPS;
         cplx<T> *bt = lds + (tid / TPS) * HS;  // (c*RX + r)*HS
         // row r of component c: z[n] = (M[2n], M[2n+1]) for 2n < nx, zero beyond
+        // k < H lies inside the buffer: read unconditionally, select the zeros
 #pragma unroll
         for (int q = 0; q < EPT; q++) {
             const int k = jt + q * TPS;
-            cplx<T> Z = {0, 0};
-            if (active && 2 * k < nx) {
-                Z = bt[k];
-                if (2 * k + 1 >= nx) Z.y = 0;
-            }
-            v[q] = Z;
+            const cplx<T> Z = bt[k];
+            const bool in = active && 2 * k < nx;
+            v[q] = {in ? Z.x : T(0), (in && 2 * k + 1 < nx) ? Z.y : T(0)};
         }
         exchange_sync<RowWave<H>::on>();
         fft_reg<T, H, false, 8, RowWave<H>::on>(v, jt, bt, tw, G.lgLmax);
