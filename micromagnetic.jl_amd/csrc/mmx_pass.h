@@ -121,6 +121,10 @@ struct RowWave {
 #ifndef MMX_P5_TILE
 #define MMX_P5_TILE 1
 #endif
+// 1: the tile's loads issue with the spectrum's, before the row transform
+#ifndef MMX_P5_TILE_EARLY
+#define MMX_P5_TILE_EARLY 0
+#endif
 // at least MMX_P5_WPE waves per SIMD for the last pass: its cell loop is
 // latency-bound, and the register budget this sets costs no spills
 #ifndef MMX_P5_WPE
@@ -174,63 +178,84 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
     // with wave-local rows the mirror values X[H-k] come from LDS: each
     // spectrum value is read from HBM once
     constexpr bool MIR = RowWave<H>::on && MMX_P5_MIRROR_LDS;
+    // Row buffer (c, r) as 2*HS reals: the field of cell x at x (x < nx) and,
+    // with the tile, m of the row at mo + x (mo + nx <= 2H + 2 since nx <= H).
+    // The row's threads write the tile after they end the row's transform,
+    // so no barrier is needed before they overwrite its exchange buffer.
+    // Pairs past the row are clamped to its last pair (stored again, same
+    // value); the loop stops at a uniform bound, so no load waits on a branch.
+    constexpr bool TL = MMX_P5_TILE && V >= 1 && V <= 4 && RowWave<H>::on;
+    const int mo = (nx + 1) & ~1;
+    const T *mrow = m + (int64_t)c * nx * ny * nz + ((int64_t)(z0 + (active ? r : 0)) * ny + y0) * nx;
+    const bool tpair = (nx & 1) == 0 && ((uintptr_t)mrow & (2 * sizeof(T) - 1)) == 0;
+    cplx<T> tv[TL ? EPT : 1];
+    auto tile_load = [&]() {
+        const int nh = (nx + 1) >> 1;
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            if (q * TPS >= nh) break;
+            const int k = min(j + q * TPS, nh - 1);
+            if (tpair) {
+                gld2(mrow, k, tv[q].x, tv[q].y);
+            } else {
+                tv[q].x = gld(mrow, 2 * k);
+                tv[q].y = gld(mrow, min(2 * k + 1, nx - 1));
+            }
+        }
+    };
+    auto tile_store = [&]() {
+        if (!active) return;
+        const int nh = (nx + 1) >> 1;
+        T *dst = reinterpret_cast<T *>(buf) + mo;
+#pragma unroll
+        for (int q = 0; q < EPT; q++) {
+            if (q * TPS >= nh) break;
+            const int k = min(j + q * TPS, nh - 1);
+            dst[2 * k] = tv[q].x;
+            if (2 * k + 1 < nx) dst[2 * k + 1] = tv[q].y;
+        }
+    };
+    if constexpr (TL && MMX_P5_TILE_EARLY) tile_load();
+    // (row points at a valid row also for inactive threads: their loads are
+    // harmless and their results are replaced by zeros, so no load sits
+    // behind a branch and they all issue together)
+    cplx<T> wq[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; q++) gld2(reinterpret_cast<const T *>(tw), (int64_t)((j + q * TPS) << sh), wq[q].x, wq[q].y);
     if constexpr (MIR) {
+        cplx<T> xr[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; q++) xr[q] = row[j + q * TPS];
+        const cplx<T> xh = row[H];
         if (active) {
 #pragma unroll
-            for (int q = 0; q < EPT; q++) buf[j + q * TPS] = row[j + q * TPS];
-            if (j == 0) buf[H] = row[H];
+            for (int q = 0; q < EPT; q++) buf[j + q * TPS] = xr[q];
+            if (j == 0) buf[H] = xh;
         }
         exchange_sync<true>();
     }
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
         const int k = j + q * TPS;
-        cplx<T> Z = {0, 0};
-        if (active) {
-            cplx<T> Xk = MIR ? buf[k] : row[k];
-            cplx<T> Xc = MIR ? buf[H - k] : row[H - k];
-            Xc.y = -Xc.y;
-            cplx<T> Fe = cadd(Xk, Xc);
-            cplx<T> w = tw[k << sh];
-            w.y = -w.y;  // W_Px^{-k}
-            cplx<T> Fo = cmul(csub(Xk, Xc), w);
-            Z = {Fe.x - Fo.y, Fe.y + Fo.x};  // Fe + i Fo
-        }
-        v[q] = Z;
+        cplx<T> Xk = MIR ? buf[k] : row[k];
+        cplx<T> Xc = MIR ? buf[H - k] : row[H - k];
+        Xc.y = -Xc.y;
+        cplx<T> Fe = cadd(Xk, Xc);
+        cplx<T> w = wq[q];
+        w.y = -w.y;  // W_Px^{-k}
+        cplx<T> Fo = cmul(csub(Xk, Xc), w);
+        v[q] = active ? cplx<T>{Fe.x - Fo.y, Fe.y + Fo.x} : cplx<T>{0, 0};  // Fe + i Fo
     }
     if constexpr (MIR) exchange_sync<true>();
     fft_reg<T, H, true, 8, RowWave<H>::on>(v, j, buf, tw, G.lgLmax);
-    // Row buffer (c, r) as 2*HS reals: the field of cell x at x (x < nx) and,
-    // with the tile, m of the row at mo + x (mo + nx <= 2H + 2 since nx <= H).
-    // The row's threads are the ones that just ended its transform, so no
-    // barrier is needed before they overwrite its exchange buffer.
-    constexpr bool TL = MMX_P5_TILE && V >= 1 && V <= 4 && RowWave<H>::on;
-    const int mo = (nx + 1) & ~1;
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
         const int k = j + q * TPS;
         if (!TL || 2 * k < nx) buf[k] = v[q];
     }
     if constexpr (TL) {
-        // indices past the row are clamped to its last element, whose value
-        // they store again: no per-element branch, so the loads issue together
-        if (active) {
-            const T *mrow = m + (int64_t)c * nx * ny * nz + ((int64_t)(z0 + r) * ny + y0) * nx;
-            T *dst = reinterpret_cast<T *>(buf) + mo;
-            if ((nx & 1) == 0 && ((uintptr_t)mrow & (2 * sizeof(T) - 1)) == 0) {
-                cplx<T> w[EPT];
-#pragma unroll
-                for (int q = 0; q < EPT; q++) gld2(mrow, min(j + q * TPS, nx / 2 - 1), w[q].x, w[q].y);
-#pragma unroll
-                for (int q = 0; q < EPT; q++) reinterpret_cast<cplx<T> *>(dst)[min(j + q * TPS, nx / 2 - 1)] = w[q];
-            } else {
-                T w[2 * EPT];
-#pragma unroll
-                for (int q = 0; q < 2 * EPT; q++) w[q] = gld(mrow, min(2 * j + (q >> 1) * 2 * TPS + (q & 1), nx - 1));
-#pragma unroll
-                for (int q = 0; q < 2 * EPT; q++) dst[min(2 * j + (q >> 1) * 2 * TPS + (q & 1), nx - 1)] = w[q];
-            }
-        }
+        if constexpr (!MMX_P5_TILE_EARLY) tile_load();
+        tile_store();
     }
     __syncthreads();
     const MMX_CONST FieldParams<T> &P = *(const MMX_CONST FieldParams<T> *)Pp;
