@@ -50,28 +50,33 @@ __global__ void __launch_bounds__(Shape<H>::BX) k_p1(DemagGeom G, const T *__res
     // even rows whose start is a multiple of 2 elements: one 2-element load per
     // pair (x[2k], x[2k+1]) for the input and for the scale
     const bool pairs = (n_in % 2 == 0) && (cstride % 2 == 0);
+    // indices past the row are clamped to its last element and the value is
+    // replaced by zero: no load waits behind a branch
+    const int nh = (n_in + 1) >> 1;
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
         const int k = j + m * TPS;
-        const int x0 = 2 * k, x1 = 2 * k + 1;
-        T a = 0, b = 0;
+        if (m * TPS >= nh) {  // uniform: the whole element is padding
+            v[m] = {0, 0};
+            continue;
+        }
+        const bool in0 = active && 2 * k < n_in, in1 = active && 2 * k + 1 < n_in;
+        T a, b;
         if (pairs) {
-            if (active && x0 < n_in) {
-                const cplx<T> xv = *(const cplx<T> *)(src + x0);
-                if (scale) {
-                    const cplx<T> sv = *(const cplx<T> *)(scale + rb + x0);
-                    a = xv.x * sv.x;
-                    b = xv.y * sv.y;
-                } else {
-                    a = xv.x;
-                    b = xv.y;
-                }
+            const int kc = min(k, nh - 1);
+            gld2(src, kc, a, b);
+            if (scale) {
+                T sa, sb;
+                gld2(scale + rb, kc, sa, sb);
+                a = a * sa;
+                b = b * sb;
             }
         } else {
-            if (active && x0 < n_in) a = scale ? src[x0] * scale[rb + x0] : src[x0];
-            if (active && x1 < n_in) b = scale ? src[x1] * scale[rb + x1] : src[x1];
+            const int x0 = min(2 * k, n_in - 1), x1 = min(2 * k + 1, n_in - 1);
+            a = scale ? src[x0] * scale[rb + x0] : src[x0];
+            b = scale ? src[x1] * scale[rb + x1] : src[x1];
         }
-        v[m] = {a, b};
+        v[m] = {in0 ? a : T(0), in1 ? b : T(0)};
     }
     cplx<T> *buf = lds + ls * HS;
     fft_reg<T, H, false, 8, RowWave<H>::on>(v, j, buf, tw, G.lgLmax);
@@ -356,31 +361,45 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     const T *kb = Kt + (((int64_t)kyf * (KS >> 2) + (kxs >> 2)) * Pz2) * 4 + (kxs & 3);
     T kxx[EPT], kyy[EPT], kzz[EPT], kxy[EPT], kxz[EPT], kyz[EPT];
     constexpr bool PAIR = MMX_P3W_PAIR && sizeof(T) == 8 && TW % 4 == 0 && EPT == 4;
+    // raw 16-byte loads (issued before the forward transform) and the lane
+    // exchanges that complete them (after it, so the transform does not wait
+    // for the loads: the wave barriers of its exchanges keep the DPP moves
+    // from being scheduled above it)
+    cplx<T> pa[PAIR ? EPT : 1], pb[PAIR ? EPT : 1], pc[PAIR ? EPT : 1];
+    T lq = 0, lyz = 0, lxz = 0;
+    const int e = t & 1;
     auto load_tensor_pairs = [&]() {
-        const T lyz = Lline[Pz2 + G.Py2 + kxs];
-        const T lxz = Lline[Pz2 + kyf];
-        // lane (j, t) loads Lline at its quad's kz = j + (t&3)*TPS; m is taken from lane m of the quad
+        lyz = Lline[Pz2 + G.Py2 + kxs];
+        lxz = Lline[Pz2 + kyf];
+        // lane (j, t) loads Lline at its quad's kz = j + (t&3)*TPS
         const int kzq = j + (t & 3) * TPS;
-        const T lq = Lline[kzq <= L / 2 ? kzq : L - kzq];
-        T lz[4];
-        lz[0] = quad_perm<QP_BCAST<0>>(lq);
-        lz[1] = quad_perm<QP_BCAST<1>>(lq);
-        lz[2] = quad_perm<QP_BCAST<2>>(lq);
-        lz[3] = quad_perm<QP_BCAST<3>>(lq);
+        lq = Lline[kzq <= L / 2 ? kzq : L - kzq];
         // kx pair (2pk, 2pk+1), clamped into the padded rows (lanes past them hold no data)
-        const int e = t & 1;
         const int pk = min(kx >> 1, (KS >> 1) - 1);
         const T *kbp = Kt + (((int64_t)kyf * (KS >> 2) + (pk >> 1)) * Pz2) * 4 + 2 * (pk & 1) + e * cs;
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
             const int kz = j + m * TPS;
             const int kzf = (kz <= L / 2) ? kz : L - kz;
-            const T sz = (kz <= L / 2) ? T(1) : T(-1);
             const T *kp = kbp + kzf * 4;
             // even lanes load components 0, 2, 4 (xx, zz, xz), odd lanes 1, 3, 5 (yy, xy, yz)
-            const cplx<T> a = *reinterpret_cast<const cplx<T> *>(kp);
-            const cplx<T> b = *reinterpret_cast<const cplx<T> *>(kp + 2 * cs);
-            const cplx<T> c = *reinterpret_cast<const cplx<T> *>(kp + 4 * cs);
+            gld2(kp, 0, pa[m].x, pa[m].y);
+            gld2(kp + 2 * cs, 0, pb[m].x, pb[m].y);
+            gld2(kp + 4 * cs, 0, pc[m].x, pc[m].y);
+        }
+    };
+    auto finish_tensor_pairs = [&]() {
+        // m is taken from lane m of the quad
+        T lz[4];
+        lz[0] = quad_perm<QP_BCAST<0>>(lq);
+        lz[1] = quad_perm<QP_BCAST<1>>(lq);
+        lz[2] = quad_perm<QP_BCAST<2>>(lq);
+        lz[3] = quad_perm<QP_BCAST<3>>(lq);
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int kz = j + m * TPS;
+            const T sz = (kz <= L / 2) ? T(1) : T(-1);
+            const cplx<T> a = pa[m], b = pb[m], c = pc[m];
             const T ra = quad_perm<QP_XOR1>(e ? a.x : a.y);
             const T rb = quad_perm<QP_XOR1>(e ? b.x : b.y);
             const T rc = quad_perm<QP_XOR1>(e ? c.x : c.y);
@@ -414,9 +433,11 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
             }
         }
     };
-    if constexpr (MMX_P3W_PF) load_tensor();
-    // one set of twiddle bases serves the forward and the inverse transform
+    // one set of twiddle bases serves the forward and the inverse transform;
+    // loaded before the tensor (loads complete in order, and the forward
+    // transform needs the twiddles, not the tensor)
     const TwSet<T, L, P3W<T>::E> ts = fft_twiddles<T, L, P3W<T>::E>(j, tw, G.lgLmax);
+    if constexpr (MMX_P3W_PF) load_tensor();
     cplx<T> *buf = lds + wave * NB * WS + t * LS;
     if constexpr (NB == 3) {
         fft_wave_multi<T, L, false, P3W<T>::E, 3>(&v[0][0], j, buf, WS, ts);
@@ -425,6 +446,7 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
         for (int c = 0; c < 3; c++) fft_wave_multi<T, L, false, P3W<T>::E, 1>(v[c], j, buf, 0, ts);
     }
     if constexpr (!MMX_P3W_PF) load_tensor();
+    if constexpr (PAIR) finish_tensor_pairs();
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
         const T xx = kxx[m], yy = kyy[m], zz = kzz[m], xy = kxy[m], xz = kxz[m], yz = kyz[m];
