@@ -310,10 +310,11 @@ struct P3W {
 #ifndef MMX_P3W_PF
 #define MMX_P3W_PF 1
 #endif
-// FP64 tensor loads in kx pairs: the two lanes of a kx pair (t even/odd) load
-// 16 bytes each of three components for both kx and swap halves with a DPP
-// move, and a quad of lanes (same j) loads the four Lline[kzf] once and
-// broadcasts them: 12 + 1 vector loads per lane instead of 24 + 4
+// Tensor loads in kx pairs: the two lanes of a kx pair (t even/odd) load
+// both kx of three components each (one 2-value load per component) and swap
+// halves with a DPP move, and a quad of lanes (same j) loads four Lline[kzf]
+// at once and broadcasts them: per element 3 + 1/4 vector loads per lane
+// instead of 6 + 1
 #ifndef MMX_P3W_PAIR
 #define MMX_P3W_PAIR 1
 #endif
@@ -360,20 +361,23 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     const int64_t cs = (int64_t)G.Py2 * KS * Pz2;
     const T *kb = Kt + (((int64_t)kyf * (KS >> 2) + (kxs >> 2)) * Pz2) * 4 + (kxs & 3);
     T kxx[EPT], kyy[EPT], kzz[EPT], kxy[EPT], kxz[EPT], kyz[EPT];
-    constexpr bool PAIR = MMX_P3W_PAIR && sizeof(T) == 8 && TW % 4 == 0 && EPT == 4;
+    constexpr bool PAIR = MMX_P3W_PAIR && TW % 4 == 0 && EPT % 4 == 0;
     // raw 16-byte loads (issued before the forward transform) and the lane
     // exchanges that complete them (after it, so the transform does not wait
     // for the loads: the wave barriers of its exchanges keep the DPP moves
     // from being scheduled above it)
     cplx<T> pa[PAIR ? EPT : 1], pb[PAIR ? EPT : 1], pc[PAIR ? EPT : 1];
-    T lq = 0, lyz = 0, lxz = 0;
+    T lq[PAIR ? EPT / 4 : 1], lyz = 0, lxz = 0;
     const int e = t & 1;
     auto load_tensor_pairs = [&]() {
         lyz = Lline[Pz2 + G.Py2 + kxs];
         lxz = Lline[Pz2 + kyf];
-        // lane (j, t) loads Lline at its quad's kz = j + (t&3)*TPS
-        const int kzq = j + (t & 3) * TPS;
-        lq = Lline[kzq <= L / 2 ? kzq : L - kzq];
+        // lane (j, t) loads Lline at kz = j + (4h + (t&3))*TPS for h < EPT/4
+#pragma unroll
+        for (int h = 0; h < EPT / 4; h++) {
+            const int kzq = j + (4 * h + (t & 3)) * TPS;
+            lq[h] = Lline[kzq <= L / 2 ? kzq : L - kzq];
+        }
         // kx pair (2pk, 2pk+1), clamped into the padded rows (lanes past them hold no data)
         const int pk = min(kx >> 1, (KS >> 1) - 1);
         const T *kbp = Kt + (((int64_t)kyf * (KS >> 2) + (pk >> 1)) * Pz2) * 4 + 2 * (pk & 1) + e * cs;
@@ -389,12 +393,15 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
         }
     };
     auto finish_tensor_pairs = [&]() {
-        // m is taken from lane m of the quad
-        T lz[4];
-        lz[0] = quad_perm<QP_BCAST<0>>(lq);
-        lz[1] = quad_perm<QP_BCAST<1>>(lq);
-        lz[2] = quad_perm<QP_BCAST<2>>(lq);
-        lz[3] = quad_perm<QP_BCAST<3>>(lq);
+        // element m is taken from lane m%4 of the quad (load m/4)
+        T lz[EPT];
+#pragma unroll
+        for (int h = 0; h < EPT / 4; h++) {
+            lz[4 * h + 0] = quad_perm<QP_BCAST<0>>(lq[h]);
+            lz[4 * h + 1] = quad_perm<QP_BCAST<1>>(lq[h]);
+            lz[4 * h + 2] = quad_perm<QP_BCAST<2>>(lq[h]);
+            lz[4 * h + 3] = quad_perm<QP_BCAST<3>>(lq[h]);
+        }
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
             const int kz = j + m * TPS;
