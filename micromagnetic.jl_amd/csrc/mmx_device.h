@@ -113,17 +113,18 @@ __device__ __forceinline__ void cell_field(const MMX_CONST FieldParams<T> &P, co
     neighbours(P, i, j, k, nb);
     T nmx[6], nmy[6], nmz[6];
     bool ok[6];
+    // a missing neighbour reads the cell itself and is masked: no load sits
+    // behind a branch, so the 24 loads issue together
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-        int64_t id = nb[q];
-        ok[q] = false;
-        nmx[q] = nmy[q] = nmz[q] = T(0);
-        if (id >= 0) {
-            ok[q] = P.mu0Ms[id] > T(0);
-            nmx[q] = m[id];
-            nmy[q] = m[N + id];
-            nmz[q] = m[2 * N + id];
-        }
+        const bool ex = nb[q] >= 0;
+        const int64_t id = ex ? nb[q] : I;
+        const T msn = gld(P.mu0Ms, id);
+        const T ax = gld(m, id), ay = gld(m, N + id), az = gld(m, 2 * N + id);
+        ok[q] = ex && msn > T(0);
+        nmx[q] = ex ? ax : T(0);
+        nmy[q] = ex ? ay : T(0);
+        nmz[q] = ex ? az : T(0);
     }
     Hx = Hy = Hz = T(0);
     for (int tt = 0; tt < P.nterms; tt++) {
@@ -872,6 +873,18 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
     T mx, my, mz, Hx, Hy, Hz;
     uint32_t msk;
     cell_field_fast<T, SIG, TILE>(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk, tl);
+    // the earlier stages' k of a DOPRI54 / BS23 / Heun combination or error
+    // estimate: all issued here together (uniform guards, no use inside),
+    // after the field so that they do not add to its register peak
+    constexpr bool KPL = (MODE == EPI_STAGE && !NOKP) || MODE == EPI_ERR;
+    T kq[KPL ? 6 : 1][3];
+    if constexpr (KPL) {
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+            if (q < E.np)
+#pragma unroll
+                for (int c = 0; c < 3; c++) kq[q][c] = gld(E.kp[q], c * N + I);
+    }
     // llg_rhs_kernel!: src/llg/llg.jl:6-40 (a = -gamma/(1+alpha^2) precomputed)
     T kk[3];
     if ((msk >> 6) & 1u) {
@@ -909,8 +922,8 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
 #pragma unroll
             for (int q = 0; q < 6; q++)
                 if (q < E.np) {
-                    const double kq = (double)gld(E.kp[q], c * N + I);
-                    err = (q == 0) ? E.cp[0] * kq : err + E.cp[q] * kq;
+                    const double kd = (double)kq[q][c];
+                    err = (q == 0) ? E.cp[0] * kd : err + E.cp[q] * kd;
                 }
             err = err + E.cself * (double)kk[c];
             mval = pmax(mval, fabs(err / (1.0 + fabs((double)mv[c]))));
@@ -928,7 +941,7 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
                 if constexpr (!NOKP) {
 #pragma unroll
                     for (int q = 0; q < 6; q++)
-                        if (q < E.np) s = s + (T)E.cp[q] * gld(E.kp[q], c * N + I);
+                        if (q < E.np) s = s + (T)E.cp[q] * kq[q][c];
                 }
                 y[c] = s + (T)E.cself * kk[c];
             }
@@ -1331,6 +1344,19 @@ __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, c
     // nothing below for an instantiation without the RHS, stage and error modes
     if constexpr ((MM & ~(MM_FIELD | (1u << EPI_ENERGY) | MM_ADJ)) == 0) return;
     cell_field(P, dyn, E.term_mask, I, i, j, k, m, hdx, hdy, hdz, Hx, Hy, Hz, (double *)nullptr);
+    // operands of the stage combination / error estimate, issued together
+    // after the field (uniform guards, no use inside)
+    const int nkp = E.mode == EPI_RK4_FINAL ? 3 : (E.mode == EPI_RHS ? 0 : E.np);
+    const bool uy0 = E.mode != EPI_RHS && E.mode != EPI_ERR;
+    T kq[6][3], y0v[3];
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+        if (q < nkp)
+#pragma unroll
+            for (int c = 0; c < 3; c++) kq[q][c] = gld(E.kp[q], c * N + I);
+    if (uy0)
+#pragma unroll
+        for (int c = 0; c < 3; c++) y0v[c] = gld(E.y0, c * N + I);
     const T mx = m[I], my = m[N + I], mz = m[2 * N + I];
     T kx, ky, kz;
     llg_rhs(P, I, mx, my, mz, Hx, Hy, Hz, kx, ky, kz);
@@ -1359,7 +1385,7 @@ __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, c
             double err = 0;
 #pragma unroll
             for (int q = 0; q < 6; q++)
-                if (q < E.np) err = (q == 0) ? E.cp[0] * (double)E.kp[0][o] : err + E.cp[q] * (double)E.kp[q][o];
+                if (q < E.np) err = (q == 0) ? E.cp[0] * (double)kq[0][c] : err + E.cp[q] * (double)kq[q][c];
             err = err + E.cself * (double)kk[c];
             const double v = fabs(err / (1.0 + fabs((double)m[o])));
             mval = pmax(mval, v);
@@ -1370,22 +1396,21 @@ __device__ __forceinline__ void finish_cell(const MMX_CONST FieldParams<T> &P, c
     T y[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const int64_t o = c * N + I;
         if (E.mode == EPI_RK4_FINAL) {  // src/integrator/rk.jl:46
-            T s = E.kp[0][o] + T(2) * E.kp[1][o] + T(2) * E.kp[2][o] + kk[c];
-            y[c] = E.y0[o] + (T)E.cself * s;
+            T s = kq[0][c] + T(2) * kq[1][c] + T(2) * kq[2][c] + kk[c];
+            y[c] = y0v[c] + (T)E.cself * s;
         } else if (E.mode == EPI_STAGE_SCALED) {  // adaptive_rk_steps.jl:35
             T s = T(0);
 #pragma unroll
             for (int q = 0; q < 6; q++)
-                if (q < E.np) s = (q == 0) ? (T)E.cp[0] * E.kp[0][o] : s + (T)E.cp[q] * E.kp[q][o];
+                if (q < E.np) s = (q == 0) ? (T)E.cp[0] * kq[0][c] : s + (T)E.cp[q] * kq[q][c];
             s = s + (T)E.cself * kk[c];
-            y[c] = E.y0[o] + (T)E.outer * s;
+            y[c] = y0v[c] + (T)E.outer * s;
         } else {  // vector_add2..6: a1 + c2 a2 + ... (util.jl:86-166)
-            T s = E.y0[o];
+            T s = y0v[c];
 #pragma unroll
             for (int q = 0; q < 6; q++)
-                if (q < E.np) s = s + (T)E.cp[q] * E.kp[q][o];
+                if (q < E.np) s = s + (T)E.cp[q] * kq[q][c];
             y[c] = s + (T)E.cself * kk[c];
         }
     }
