@@ -215,10 +215,10 @@ def grf(rng, nx, ny, dx, corr, mean, rel):
     return mean * (1 + rel * g)
 
 
-def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
-    """Batching form: stochastic-LLG ensemble members of uneven size (Heun,
-    300 K, alpha 0.1, seeded Philox noise) advanced by one launch per stage over
-    all members, against the same members advanced one Sim at a time."""
+def ensemble_specs(n_members=48, seed=4004):
+    """Member meshes and parameters of the stochastic-LLG ensemble, drawn from
+    a seed: meshes 32..192 x 32..192 x 4..16, Ms, A, D, Ku in the ranges of the
+    batching workload."""
     rng = np.random.default_rng(seed)
     specs = []
     for q in range(n_members):
@@ -226,21 +226,34 @@ def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
         nz = int(rng.integers(4, 17))
         specs.append(dict(nx=nx, ny=ny, nz=nz, Ms=rng.uniform(4e5, 1.2e6), A=rng.uniform(5e-12, 2e-11),
                           D=rng.uniform(0, 4e-3), Ku=rng.uniform(0, 1e6), seed=int(rng.integers(1, 1 << 30))))
+    return specs
+
+
+def ensemble_sims(mmx, precision, specs):
+    """One Sim per member: Heun dt 1e-14, 300 K seeded thermal field, alpha 0.1."""
+    sims = []
+    for sp in specs:
+        mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=sp["nx"], ny=sp["ny"], nz=sp["nz"], pbc="xy")
+        s = mmx.Sim(mesh, integrator="Heun", precision=precision)
+        mmx.set_Ms(s, sp["Ms"])
+        mmx.init_m0(s, (0, 0, 1))
+        mmx.add_exch(s, sp["A"])
+        mmx.add_dmi(s, sp["D"], type="interfacial")
+        mmx.add_anis(s, sp["Ku"], axis=(0, 0, 1))
+        mmx.add_thermal_noise(s, 300.0, seed=sp["seed"])
+        mmx.set_driver(s, integrator="Heun", step=1e-14)
+        sims.append(s)
+    return sims
+
+
+def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
+    """Batching form: stochastic-LLG ensemble members of uneven size (Heun,
+    300 K, alpha 0.1, seeded Philox noise) advanced by one launch per stage over
+    all members, against the same members advanced one Sim at a time."""
+    specs = ensemble_specs(n_members, seed)
 
     def make():
-        sims = []
-        for sp in specs:
-            mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=sp["nx"], ny=sp["ny"], nz=sp["nz"], pbc="xy")
-            s = mmx.Sim(mesh, integrator="Heun", precision=precision)
-            mmx.set_Ms(s, sp["Ms"])
-            mmx.init_m0(s, (0, 0, 1))
-            mmx.add_exch(s, sp["A"])
-            mmx.add_dmi(s, sp["D"], type="interfacial")
-            mmx.add_anis(s, sp["Ku"], axis=(0, 0, 1))
-            mmx.add_thermal_noise(s, 300.0, seed=sp["seed"])
-            mmx.set_driver(s, integrator="Heun", step=1e-14)
-            sims.append(s)
-        return sims
+        return ensemble_sims(mmx, precision, specs)
 
     cells = sum(sp["nx"] * sp["ny"] * sp["nz"] for sp in specs)
     L = mmx._lib.load()
@@ -277,9 +290,31 @@ def bench_neb(mmx, precision, steps=20, warmup=3, chains=2, frames=22, seed=6006
     film between a seeded skyrmion and the uniform state, stepped by the NEB
     driver (BS23, tol 1e-5); the field of every free image is one launch per
     step, the band's tangent, force and stage updates one launch each per stage."""
+    negs = neb_chains(mmx, precision, chains, frames, seed)
+    N = 128 * 128 * 4
+    for _, nb in negs:
+        nb.advance(warmup)
+    t0 = time.perf_counter()
+    for _, nb in negs:
+        nb.advance(steps)  # synchronous: one host sync per BS23 try
+    t = time.perf_counter() - t0
+    n_img = negs[0][1].n_images
+    st = [nb.state for _, nb in negs]
+    work = chains * n_img * N * steps
+    return {"workload": f"{chains} NEB chains of {n_img + 2} images on a 128x128x4 film (seed {seed}), "
+                        f"exch+iDMI+Ku+Zeeman, BS23 tol 1e-5",
+            "value": round(work / t / 1e6, 1), "unit": "Mcell-image-steps/s",
+            "ms_per_chain_step": round(t / (chains * steps) * 1e3, 4),
+            "fevals_per_step": round(sum(x["nfevals"] for x in st) / sum(x["nsteps"] for x in st), 3)}
+
+
+def neb_chains(mmx, precision, chains=2, frames=22, seed=6006):
+    """NEB chains of frames + 2 images on a 128x128x4 film (2 nm cells, periodic
+    x/y): exchange, interfacial DMI, anisotropy along z and a 0.1 T field, each
+    chain with its own seeded parameters, between a seeded skyrmion and the
+    uniform state."""
     rng = np.random.default_rng(seed)
     nx, ny, nz, d = 128, 128, 4, 2e-9
-    N = nx * ny * nz
     negs = []
     for _ in range(chains):
         mesh = mmx.FDMesh(dx=d, dy=d, dz=d, nx=nx, ny=ny, nz=nz, pbc="xy")
@@ -299,20 +334,79 @@ def bench_neb(mmx, precision, steps=20, warmup=3, chains=2, frames=22, seed=6006
         left = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=-1)
         left = np.tile(left.reshape(-1, 3), (nz, 1)).reshape(-1)
         negs.append((s, mmx.NEB(s, [left, (0.0, 0.0, 1.0)], [frames])))
-    for _, nb in negs:
-        nb.advance(warmup)
+    return negs
+
+
+def bench_all96(mmx, precision, ens_steps=100, neb_steps=40, warmup=3):
+    """The whole batching workload: the 48 ensemble members and the 2 NEB
+    chains of 24 images (44 free) at once.  The ensemble advances with one launch
+    per stage over all members (a HIP graph on its own stream) while each
+    chain steps its band on its own stream; the three are driven from three
+    host threads (the library calls release the GIL), so the chains' small
+    launches and host-side BS23 accept/reject fill the device around the
+    ensemble's.  Reported: field+LLG cell evaluations of all 96 systems per
+    second, against the same work run back to back."""
+    import threading
+
+    specs = ensemble_specs()
+    ens_cells = sum(sp["nx"] * sp["ny"] * sp["nz"] for sp in specs)
+    N = 128 * 128 * 4
+
+    def setup():
+        sims = ensemble_sims(mmx, precision, specs)
+        b = mmx.Batch(sims)
+        negs = neb_chains(mmx, precision)
+        b.advance(warmup)
+        for _, nb in negs:
+            nb.advance(warmup)
+        return sims, b, negs
+
+    def fevals(negs):
+        return [nb.state["nfevals"] for _, nb in negs]
+
+    def work(negs, f0):
+        n_img = negs[0][1].n_images
+        neb = sum((f1 - a) * n_img * N for f1, a in zip(fevals(negs), f0))
+        return ens_cells * 2 * ens_steps + neb
+
+    # back to back
+    sims, b, negs = setup()
+    f0 = fevals(negs)
     t0 = time.perf_counter()
+    b.advance(ens_steps)
     for _, nb in negs:
-        nb.advance(steps)  # synchronous: one host sync per BS23 try
-    t = time.perf_counter() - t0
-    n_img = negs[0][1].n_images
-    st = [nb.state for _, nb in negs]
-    work = chains * n_img * N * steps
-    return {"workload": f"{chains} NEB chains of {n_img + 2} images on a 128x128x4 film (seed {seed}), "
-                        f"exch+iDMI+Ku+Zeeman, BS23 tol 1e-5",
-            "value": round(work / t / 1e6, 1), "unit": "Mcell-image-steps/s",
-            "ms_per_chain_step": round(t / (chains * steps) * 1e3, 4),
-            "fevals_per_step": round(sum(x["nfevals"] for x in st) / sum(x["nsteps"] for x in st), 3)}
+        nb.advance(neb_steps)
+    t_seq = time.perf_counter() - t0
+    w_seq = work(negs, f0)
+    del b, negs, sims
+    # concurrent
+    sims, b, negs = setup()
+    f0 = fevals(negs)
+    errs = []
+
+    def run(fn):
+        try:
+            fn()
+        except Exception as e:  # re-raised below
+            errs.append(e)
+
+    th = [threading.Thread(target=run, args=(lambda: b.advance(ens_steps),))]
+    th += [threading.Thread(target=run, args=(lambda nb=nb: nb.advance(neb_steps),)) for _, nb in negs]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    t_con = time.perf_counter() - t0
+    if errs:
+        raise errs[0]
+    w_con = work(negs, f0)
+    del b, negs, sims
+    return {"workload": f"96 systems: 48 sLLG members ({ens_cells} cells, Heun dt 1e-14, 300 K, {ens_steps} steps) "
+                        f"+ 2 NEB chains of 24 images on 128x128x4 ({neb_steps} BS23 steps each)",
+            "value": round(w_con / t_con / 1e6, 1), "unit": "Mcell-evals/s",
+            "back_to_back_value": round(w_seq / t_seq / 1e6, 1),
+            "seconds": round(t_con, 4), "back_to_back_seconds": round(t_seq, 4)}
 
 
 def bench_adjoint(mmx, precision, nsteps=2000, every=50, n_fd=8, seed=5005):
@@ -557,6 +651,12 @@ def run_forms(args, out_path):
     for k in ("neb_fp64", "neb_fp32"):
         if "value" in res.get(k, {}):
             log(f"[bench] {k}: {res[k]['value']} {res[k]['unit']}, {res[k]['ms_per_chain_step']} ms per chain step")
+    leg("all96_fp64", lambda: bench_all96(mmx, mmx.F64))
+    leg("all96_fp32", lambda: bench_all96(mmx, mmx.F32))
+    for k in ("all96_fp64", "all96_fp32"):
+        if "value" in res.get(k, {}):
+            log(f"[bench] {k}: {res[k]['value']} Mcell-evals/s over the 96 systems "
+                f"(back to back {res[k]['back_to_back_value']})")
     if ADJOINT_ENABLED:
         log("[bench] differentiation form (FP64 with finite-difference probes, FP32)")
         leg("adjoint_fp64", lambda: bench_adjoint(mmx, mmx.F64, n_fd=args.fd_cells))
