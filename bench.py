@@ -308,6 +308,63 @@ def bench_neb(mmx, precision, steps=20, warmup=3, chains=2, frames=22, seed=6006
             "fevals_per_step": round(sum(x["nfevals"] for x in st) / sum(x["nsteps"] for x in st), 3)}
 
 
+def bench_batch_demag(mmx, precision, steps=50, warmup=3, per_group=8, seed=4104):
+    """Batching form with demag members: three geometry groups of per_group
+    members each (128x128x4 periodic x/y with 1x1 images, 96x96x8 open,
+    64x64x16 periodic x/y), every member with its own Ms, A, D, Ku, exchange +
+    interfacial DMI + anisotropy + demag, Heun dt 1e-14 at 300 K; against the
+    same members advanced one Sim at a time."""
+    geoms = [((128, 128, 4), "xy", (1, 1, 0)), ((96, 96, 8), "", (0, 0, 0)), ((64, 64, 16), "xy", (0, 0, 0))]
+    rng = np.random.default_rng(seed)
+    specs = [dict(geom=g, Ms=rng.uniform(4e5, 1.2e6), A=rng.uniform(5e-12, 2e-11), D=rng.uniform(0, 4e-3),
+                  Ku=rng.uniform(0, 1e6), seed=int(rng.integers(1, 1 << 30))) for g in geoms for _ in range(per_group)]
+
+    def make():
+        sims = []
+        for sp in specs:
+            (nx, ny, nz), pbc, img = sp["geom"]
+            mesh = mmx.FDMesh(dx=2e-9, dy=2e-9, dz=2e-9, nx=nx, ny=ny, nz=nz, pbc=pbc)
+            s = mmx.Sim(mesh, integrator="Heun", precision=precision)
+            mmx.set_Ms(s, sp["Ms"])
+            mmx.init_m0(s, (0, 0, 1))
+            mmx.add_exch(s, sp["A"])
+            mmx.add_dmi(s, sp["D"], type="interfacial")
+            mmx.add_anis(s, sp["Ku"], axis=(0, 0, 1))
+            mmx.add_demag(s, Nx=img[0], Ny=img[1], Nz=img[2])
+            mmx.add_thermal_noise(s, 300.0, seed=sp["seed"])
+            mmx.set_driver(s, integrator="Heun", step=1e-14)
+            sims.append(s)
+        return sims
+
+    cells = sum(int(np.prod(sp["geom"][0])) for sp in specs)
+    L = mmx._lib.load()
+    sims = make()
+    b = mmx.Batch(sims)
+    b.advance(warmup)
+    t0 = time.perf_counter()
+    b.advance(steps)
+    tb = time.perf_counter() - t0
+    batched_spins = [s.spin for s in sims[::per_group]]
+    del b, sims
+    sims = make()
+    for s in sims:
+        mmx.run_step(s, n=warmup)
+    for s in sims:
+        mmx._lib.check(L.mmx_synchronize(s._h))
+    t0 = time.perf_counter()
+    for s in sims:
+        mmx._lib.check(L.mmx_advance_step(s._h, steps))
+    tp = time.perf_counter() - t0
+    same = all(np.array_equal(a, s.spin) for a, s in zip(batched_spins, sims[::per_group]))
+    del sims
+    ev = cells * 2 * steps
+    return {"workload": f"{len(specs)} members with demag in 3 geometry groups (128x128x4 pbc xy 1x1 images, "
+                        f"96x96x8, 64x64x16 pbc xy), {cells} cells, Heun dt 1e-14, 300 K",
+            "value": round(ev / tb / 1e6, 1), "unit": "Mcell-evals/s",
+            "plain_value": round(ev / tp / 1e6, 1), "speedup_vs_plain": round(tp / tb, 3),
+            "identical_to_plain": same}
+
+
 def neb_chains(mmx, precision, chains=2, frames=22, seed=6006):
     """NEB chains of frames + 2 images on a 128x128x4 film (2 nm cells, periodic
     x/y): exchange, interfacial DMI, anisotropy along z and a 0.1 T field, each
@@ -651,6 +708,12 @@ def run_forms(args, out_path):
     for k in ("neb_fp64", "neb_fp32"):
         if "value" in res.get(k, {}):
             log(f"[bench] {k}: {res[k]['value']} {res[k]['unit']}, {res[k]['ms_per_chain_step']} ms per chain step")
+    leg("batch_demag_fp64", lambda: bench_batch_demag(mmx, mmx.F64))
+    leg("batch_demag_fp32", lambda: bench_batch_demag(mmx, mmx.F32))
+    for k in ("batch_demag_fp64", "batch_demag_fp32"):
+        if "value" in res.get(k, {}):
+            log(f"[bench] {k}: {res[k]['value']} Mcell-evals/s vs plain {res[k]['plain_value']} "
+                f"(identical: {res[k]['identical_to_plain']})")
     leg("all96_fp64", lambda: bench_all96(mmx, mmx.F64))
     leg("all96_fp32", lambda: bench_all96(mmx, mmx.F32))
     for k in ("all96_fp64", "all96_fp32"):

@@ -243,12 +243,15 @@ int mmx_llg_rhs_vjp(mmx_ctx *ctx, double t, const double *m, const double *a, do
 
 /* Batching form.  Many independent systems (thermal-ensemble members,
  * parameter sweeps) of any mesh sizes advanced together: every stage of a
- * step is one launch over all members, and two steps are replayed from a HIP
- * graph.  The reference runs such systems one Sim at a time (one
- * run_sim / advance_step per Sim, src/sim.jl:813-857); each member keeps its
- * own terms, parameters, state and counters, exactly as if advanced alone.
- * Members: same precision and device, no demag term, all RungeKutta or all
- * Heun with a step set.  The members stay usable on their own. */
+ * step is one launch over all members without demag, plus, for each group of
+ * members with demag that share a geometry (mesh, cell size, periodicity and
+ * demag images, hence one tensor), one launch per demag pass over the group;
+ * two steps are replayed from a HIP graph.  The reference runs such systems
+ * one Sim at a time (one run_sim / advance_step per Sim, src/sim.jl:813-857);
+ * each member keeps its own terms, parameters, state and counters, exactly as
+ * if advanced alone.  Members: same precision and device, all RungeKutta or
+ * all Heun with a step set, constant terms.  The members stay usable on their
+ * own. */
 typedef struct mmx_batch mmx_batch;
 int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n);
 int mmx_batch_destroy(mmx_batch *batch);

@@ -150,6 +150,7 @@ struct mmx_ctx {
     int64_t nsteps = 0, nfevals = 0;
     int succeed = 0;
     int demag_term = -1;
+    int demag_img[3] = {0, 0, 0};  // macro-PBC image counts of the demag term
     bool profile = false;
     // SD driver (src/driver.jl:3-12): tau, min/max tau, step count
     double sd_tau = 0, sd_min_tau = 1e-10, sd_max_tau = 1.0;
@@ -1629,7 +1630,12 @@ struct NebEngine : CtxBase {
 // ---------------------------------------------------------------------------
 // Batches: many independent systems (ensemble members, parameter sweeps) of
 // any sizes advanced together.  Each stage of a step is ONE launch over every
-// member (k_batch_local); two steps are captured in a HIP graph and replayed.
+// member without demag (k_batch_local); members with demag are grouped by
+// geometry (mesh, cells, periodicity, demag images: one tensor), and each
+// group takes one launch per demag pass and stage, its members' spectra
+// stacked side by side (the y and z passes over 3 x members components, the
+// last pass k_p5_batch with each member's own parameters and epilogue).  Two
+// steps are captured in a HIP graph and replayed.
 // The state of member m alternates between its y buffer (A) and its stg[1]
 // buffer (B); an odd remainder step is copied back so the state always ends
 // in A.  The launch shape (cells per thread, grid) is picked at creation by
@@ -1651,6 +1657,18 @@ struct BatchEngine : CtxBase {
     std::vector<T *> bufA, bufB;  // the two state buffers of each member
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // members with demag, one group per geometry
+    struct Group {
+        std::vector<int> mem;  // member indices
+        DevBuf<int> gidx;
+        DevBuf<cplx<T>> A, Bs;  // stacked spectra (A: P1/P4 layout, Bs: P2/P3 layout)
+        int64_t astride = 0;
+        bool fast = true;       // slot-form last pass: chains the next spectrum, P1 once per advance
+        int vg[4] = {0, 0, 0, 0};
+    };
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<int> local;                   // members without demag
+    std::vector<BatchStage<T>> hst;           // host copy of the stage descriptors
 
     explicit BatchEngine(mmx_batch *b) : B(b) {}
     ~BatchEngine() override
@@ -1671,6 +1689,31 @@ struct BatchEngine : CtxBase {
             bufA.push_back(E[m]->y);
             bufB.push_back(E[m]->stg[1]);
         }
+        // group the members with demag by geometry
+        for (int m = 0; m < M; m++) {
+            mmx_ctx *c = B->members[m];
+            if (!E[m]->demag) {
+                local.push_back(m);
+                continue;
+            }
+            Group *g = nullptr;
+            for (auto &q : groups) {
+                const mmx_ctx *c0 = B->members[q->mem[0]];
+                const mmx_mesh_desc &a = c0->mesh, &b = c->mesh;
+                if (a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dx == b.dx && a.dy == b.dy && a.dz == b.dz &&
+                    a.pbc == b.pbc &&
+                    c0->demag_img[0] == c->demag_img[0] && c0->demag_img[1] == c->demag_img[1] &&
+                    c0->demag_img[2] == c->demag_img[2]) {
+                    g = q.get();
+                    break;
+                }
+            }
+            if (!g) {
+                groups.push_back(std::make_unique<Group>());
+                g = groups.back().get();
+            }
+            g->mem.push_back(m);
+        }
         std::vector<int2> it;
         items_for(4, it);
         std::vector<const FieldParams<T> *> ps(M);
@@ -1678,7 +1721,20 @@ struct BatchEngine : CtxBase {
         for (int m = 0; m < M; m++) {
             E[m]->ensure_k();
             ps[m] = E[m]->params_dev();
-            fast = fast && E[m]->Plast.fast;
+        }
+        for (int m : local) fast = fast && E[m]->Plast.fast;
+        for (auto &g : groups) {
+            const Engine<T> &e0 = *E[g->mem[0]];
+            const int n = (int)g->mem.size();
+            for (int m : g->mem) g->fast = g->fast && E[m]->Plast.fast;
+            g->astride = e0.Abuf.n;
+            g->A.alloc((size_t)n * e0.Abuf.n);
+            g->Bs.alloc((size_t)n * e0.Bbuf.n);
+            // padded entries that the passes read but never write stay zero
+            HIPCHECK(hipMemsetAsync(g->A.p, 0, g->A.n * sizeof(cplx<T>), B->stream));
+            HIPCHECK(hipMemsetAsync(g->Bs.p, 0, g->Bs.n * sizeof(cplx<T>), B->stream));
+            g->gidx.alloc(n);
+            HIPCHECK(hipMemcpy(g->gidx.p, g->mem.data(), n * sizeof(int), hipMemcpyHostToDevice));
         }
         Ps.alloc(M);
         HIPCHECK(hipMemcpy(Ps.p, ps.data(), M * sizeof(ps[0]), hipMemcpyHostToDevice));
@@ -1735,11 +1791,21 @@ struct BatchEngine : CtxBase {
                     d[1 * M + m] = {f, a};
                 }
             }
+        // group members' last passes also write the next stage's x-spectrum
+        // (every stage's output is the next stage's input)
+        for (auto &g : groups) {
+            if (!g->fast) continue;
+            for (size_t q = 0; q < g->mem.size(); q++)
+                for (int par = 0; par < 2; par++)
+                    for (int s = 0; s < S; s++) h[((size_t)par * S + s) * M + g->mem[q]].E.anext = g->A.p + q * g->astride;
+        }
+        hst = h;
         st.alloc(h.size());
         HIPCHECK(hipMemcpy(st.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
         for (int s = 0; s < S; s++) {
             const Epi<T> &x = h[(size_t)s * M].E;  // every member's stage s has the same mode and np
             V[s] = batch_variant(x.mode, fast ? 1 : 0, x.np);
+            for (auto &g : groups) g->vg[s] = batch_variant(x.mode, g->fast ? 1 : 0, x.np);
         }
         dyns.alloc(M);
         upload_dyn();
@@ -1749,11 +1815,46 @@ struct BatchEngine : CtxBase {
     void items_for(int c, std::vector<int2> &it)
     {
         it.clear();
-        for (int m = 0; m < M; m++)
+        for (int m : local)
             for (int64_t c0 = 0; c0 < E[m]->N; c0 += (int64_t)BLOCK * c) it.push_back(make_int2(m, (int)c0));
         nitems = (int)it.size();
+        if (it.empty()) return;
         items.alloc(it.size());
         HIPCHECK(hipMemcpy(items.p, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+
+    // x-spectra of the groups' members at the stage-0 input of parity par
+    // (the first pass; later stages take them from the previous last pass)
+    void prime(int par)
+    {
+        for (auto &g : groups)
+            for (size_t q = 0; q < g->mem.size(); q++) {
+                Engine<T> &e = *E[g->mem[q]];
+                const T *in = hst[(size_t)par * S * M + g->mem[q]].m;
+                launch_p1<T>(e.G, in, e.N, e.mu0Ms.p, e.G.nx, e.G.ny, e.G.nz, 3, g->A.p + q * g->astride, e.tw.p,
+                             B->stream);
+            }
+        HIPCHECK(hipGetLastError());
+    }
+
+    void stage_groups(int par, int s)
+    {
+        const BatchStage<T> *sts = st.p + ((size_t)par * S + s) * M;
+        for (auto &g : groups) {
+            const Engine<T> &e0 = *E[g->mem[0]];
+            const DemagGeom &G = e0.G;
+            const int n = (int)g->mem.size(), ny = G.ny, nz = G.nz;
+            if (!g->fast)  // term-list last pass: no chained spectrum, the first pass runs every stage
+                for (int q = 0; q < n; q++) {
+                    const Engine<T> &e = *E[g->mem[q]];
+                    launch_p1<T>(G, hst[((size_t)par * S + s) * M + g->mem[q]].m, e.N, e.mu0Ms.p, G.nx, ny, nz, 3,
+                                 g->A.p + q * g->astride, e0.tw.p, B->stream);
+                }
+            launch_coly<T>(G, g->A.p, g->Bs.p, ny, ny, G.Py, G.Py, nz, 3 * n, 0, e0.tw.p, B->stream);
+            launch_convz<T>(G, g->Bs.p, e0.Kt.p, e0.Lline.p, nz, nz, 3, 0, e0.tw.p, B->stream, n);
+            launch_coly<T>(G, g->Bs.p, g->A.p, G.Py, G.Py, ny, ny, nz, 3 * n, 1, e0.tw.p, B->stream);
+            launch_p5_batch<T>(G, g->A.p, g->astride, g->gidx.p, n, Ps.p, sts, dyns.p, g->vg[s], e0.tw.p, B->stream);
+        }
     }
 
     void upload_dyn()
@@ -1763,10 +1864,17 @@ struct BatchEngine : CtxBase {
         HIPCHECK(hipMemcpyAsync(dyns.p, d.data(), M * sizeof(Dyn), hipMemcpyHostToDevice, B->stream));
     }
 
+    void stage_local(int par, int s)
+    {
+        if (nitems)
+            launch_batch_local<T>(items.p, nitems, Ps.p, st.p + ((size_t)par * S + s) * M, dyns.p, V[s], cpt, grid,
+                                  B->stream);
+    }
+
     void stage(int par, int s)
     {
-        launch_batch_local<T>(items.p, nitems, Ps.p, st.p + ((size_t)par * S + s) * M, dyns.p, V[s], cpt, grid,
-                              B->stream);
+        stage_local(par, s);
+        stage_groups(par, s);
     }
 
     void step(int par)
@@ -1775,9 +1883,11 @@ struct BatchEngine : CtxBase {
         launch_batch_tick(dyns.p, M, B->stream);
     }
 
-    // time the first stage (it writes only stage buffers) for each launch shape
+    // time the first stage of the local members (it writes only stage
+    // buffers) for each launch shape
     void tune()
     {
+        if (local.empty()) return;
         const int cands_c[3] = {1, 2, 4};
         const int cands_g[4] = {1024, 2048, 4096, 8192};
         hipEvent_t a, b;
@@ -1792,9 +1902,9 @@ struct BatchEngine : CtxBase {
             for (int g : cands_g) {
                 cpt = c;
                 grid = g;
-                stage(0, 0);
+                stage_local(0, 0);
                 HIPCHECK(hipEventRecord(a, B->stream));
-                for (int r = 0; r < 3; r++) stage(0, 0);
+                for (int r = 0; r < 3; r++) stage_local(0, 0);
                 HIPCHECK(hipEventRecord(b, B->stream));
                 HIPCHECK(hipEventSynchronize(b));
                 float ms = 0;
@@ -1838,6 +1948,7 @@ struct BatchEngine : CtxBase {
             }
         }
         upload_dyn();
+        prime(0);
         capture();
         for (int64_t i = 0; i + 1 < n; i += 2) HIPCHECK(hipGraphLaunch(gexec, B->stream));
         if (n & 1) {
@@ -2252,7 +2363,12 @@ int mmx_add_demag(mmx_ctx *c, int Nx, int Ny, int Nz)
                       h.rtensor.data(), 0);
         e.setup_demag(h.rtensor);
         int r = add_term(c, e, h, nullptr, nullptr, nullptr);
-        if (r >= 0) c->demag_term = r;
+        if (r >= 0) {
+            c->demag_term = r;
+            c->demag_img[0] = Nx;
+            c->demag_img[1] = Ny;
+            c->demag_img[2] = Nz;
+        }
         return r;
     });
 }
@@ -2922,7 +3038,6 @@ int mmx_batch_create(mmx_batch **out, mmx_ctx *const *members, int n)
         if (!c) return fail(MMX_EINVAL, "member %d is null", i);
         if (c->mesh.precision != prec || c->mesh.device != members[0]->mesh.device)
             return fail(MMX_EINVAL, "members must share precision and device");
-        if (c->demag_term >= 0) return fail(MMX_EINVAL, "member %d has a demag term; batches take local terms only", i);
         if (c->ikind != MMX_RK4 && c->ikind != MMX_HEUN)
             return fail(MMX_EINVAL, "member %d: batches run the fixed-step RungeKutta or Heun integrators", i);
         if (c->ikind != members[0]->ikind) return fail(MMX_EINVAL, "members must use the same integrator");

@@ -40,6 +40,13 @@ int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, con
 template <class T>
 void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, const Dyn &dyn,
                       const Epi<T> &E, const T *m, int nimg, const cplx<T> *tw, hipStream_t s);
+// last pass of a group of batch members sharing one geometry: member q's
+// spectra at A + q*astride, its tables entries gidx[q]; v is the variant of
+// the group's stage mode (4/1/2 slot form, 0 term list)
+template <class T>
+void launch_p5_batch(const DemagGeom &G, const cplx<T> *A, int64_t astride, const int *gidx, int nmem,
+                     const FieldParams<T> *const *Ps, const BatchStage<T> *st, const Dyn *dyns, int v,
+                     const cplx<T> *tw, hipStream_t s);
 template <class T>
 void launch_sum_images(const T *v, int64_t N, int n, int nb, double *partial, double *out, hipStream_t s);
 // kernels compiled for a term signature (mmx_kernels_sig.hip); -1 when none matches

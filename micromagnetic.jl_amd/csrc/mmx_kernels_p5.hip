@@ -70,8 +70,33 @@ void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T>
 #undef P5I_CALL
 }
 
+template <class T>
+void launch_p5_batch(const DemagGeom &G, const cplx<T> *A, int64_t astride, const int *gidx, int nmem,
+                     const FieldParams<T> *const *Ps, const BatchStage<T> *st, const Dyn *dyns, int v,
+                     const cplx<T> *tw, hipStream_t s)
+{
+#define P5B_CALL(H)                                                                                  \
+    {                                                                                                \
+        const dim3 g((G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8, nmem);         \
+        const dim3 b(Shape<H>::B5);                                                                  \
+        if (v == 4)                                                                                  \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 4>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+        else if (v == 1)                                                                             \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 1>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+        else if (v == 2)                                                                             \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 2>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 0>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+    }
+    MMX_LG_SWITCH(G.lgH, P5B_CALL)
+#undef P5B_CALL
+}
+
 template int launch_p5<MMX_KT>(const DemagGeom &, const cplx<MMX_KT> *, const FieldParams<MMX_KT> *, const Dyn &,
                                const Epi<MMX_KT> &, const MMX_KT *, int, uint32_t, const cplx<MMX_KT> *, hipStream_t);
+template void launch_p5_batch<MMX_KT>(const DemagGeom &, const cplx<MMX_KT> *, int64_t, const int *, int,
+                                      const FieldParams<MMX_KT> *const *, const BatchStage<MMX_KT> *, const Dyn *,
+                                      int, const cplx<MMX_KT> *, hipStream_t);
 template void launch_p5_images<MMX_KT>(const DemagGeom &, const cplx<MMX_KT> *, const FieldParams<MMX_KT> *,
                                        const Dyn &, const Epi<MMX_KT> &, const MMX_KT *, int, const cplx<MMX_KT> *,
                                        hipStream_t);

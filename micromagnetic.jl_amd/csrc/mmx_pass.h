@@ -135,18 +135,14 @@ struct RowWave {
 #define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu((V == 0 || V == 5) ? 2 : MMX_P5_WPE)))
 // IMG: the launch covers a stack of images (grid y = image), each with its own
 // spectra, spin and field planes; fields and per-cell energies only.
-template <class T, int H, int V, uint32_t SIG = 0, bool IMG = false>
-__global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
-                                                      const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
-                                                      const T *__restrict__ m, const cplx<T> *__restrict__ tw)
+// The last pass of one system (one block's rows); the kernels below supply
+// its operands: the launch's own arguments (k_p5) or a batch member's entries
+// of the batch tables (k_p5_batch).
+template <class T, int H, int V, uint32_t SIG, bool IMG>
+__device__ __forceinline__ void p5_body(const DemagGeom &G, const cplx<T> *__restrict__ A,
+                                        const FieldParams<T> *__restrict__ Pp, const Dyn &dyn, const Epi<T> &E,
+                                        const T *__restrict__ m, const cplx<T> *__restrict__ tw)
 {
-    if constexpr (IMG) {
-        const int64_t q = blockIdx.y, Nc = (int64_t)G.nx * G.ny * G.nz;
-        A += q * 3 * (int64_t)G.ny * G.nz * G.KS;
-        m += q * 3 * Nc;
-        E.out += q * 3 * Nc;
-        if (E.e_out) E.e_out += q * Nc;
-    }
     using S = Shape<H>;
     constexpr int EPT = S::EPT, TPS = S::TPS, RX = S::RX;
     constexpr int HS = FftShape<H>::XS + 1;
@@ -326,6 +322,37 @@ __global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, co
         cplx<T> *dst = E.anext + ((int64_t)c * nrows_tot + (int64_t)(z0 + (active ? r : 0)) * ny + y0) * G.KS;
         r2c_split_store<T, H, RowWave<H>::on>(v, jt, bt, active, dst, tw, G.lgLmax);
     }
+}
+
+template <class T, int H, int V, uint32_t SIG = 0, bool IMG = false>
+__global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR k_p5(DemagGeom G, const cplx<T> *__restrict__ A,
+                                                      const FieldParams<T> *__restrict__ Pp, Dyn dyn, Epi<T> E,
+                                                      const T *__restrict__ m, const cplx<T> *__restrict__ tw)
+{
+    if constexpr (IMG) {
+        const int64_t q = blockIdx.y, Nc = (int64_t)G.nx * G.ny * G.nz;
+        A += q * 3 * (int64_t)G.ny * G.nz * G.KS;
+        m += q * 3 * Nc;
+        E.out += q * 3 * Nc;
+        if (E.e_out) E.e_out += q * Nc;
+    }
+    p5_body<T, H, V, SIG, IMG>(G, A, Pp, dyn, E, m, tw);
+}
+
+// Batch of systems of one geometry (one demag tensor): grid y = member q of
+// the group, whose spectra lie at A + q*astride; its parameters, dynamic
+// scalars and stage descriptor (epilogue and input vector) are entries gidx[q]
+// of the batch tables.  Every member takes the same stage mode.
+template <class T, int H, int V>
+__global__ void __launch_bounds__(Shape<H>::B5) MMX_P5_ATTR
+    k_p5_batch(DemagGeom G, const cplx<T> *__restrict__ A, int64_t astride, const int *__restrict__ gidx,
+               const FieldParams<T> *const *__restrict__ Ps, const BatchStage<T> *__restrict__ st,
+               const Dyn *__restrict__ dyns, const cplx<T> *__restrict__ tw)
+{
+    const int q = blockIdx.y;
+    const int mi = gidx[q];
+    const BatchStage<T> &bs = st[mi];
+    p5_body<T, H, V, 0u, false>(G, A + q * astride, Ps[mi], dyns[mi], bs.E, bs.m, tw);
 }
 
 // ---------------------------------------------------------------------------

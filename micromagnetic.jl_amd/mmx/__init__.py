@@ -955,10 +955,11 @@ def dynamic_matrix(sim, gamma=2.21e5, alpha=0.01):
 # ---- batching form -------------------------------------------------------------
 class Batch:
     """Many independent Sims advanced together (one launch per RK stage over all
-    members, replayed from a HIP graph).  Each member must use the RungeKutta or
-    Heun integrator with a step set, share precision and device with the others
-    and have no demag term; after `advance(n)` each member is exactly where
-    `run_step(sim, n=n)` would have left it."""
+    members without demag; members with demag grouped by geometry, one launch
+    per demag pass and stage for each group; replayed from a HIP graph).  Each
+    member must use the RungeKutta or Heun integrator with a step set and share
+    precision and device with the others; after `advance(n)` each member is
+    exactly where `run_step(sim, n=n)` would have left it."""
 
     def __init__(self, sims):
         self.sims = list(sims)
