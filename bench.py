@@ -571,8 +571,9 @@ def read_traffic(kernel_pass):
     path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
     if not os.path.exists(path):
         return None
+    # | pass | kernel(s) | launches | traffic (B/launch) | ...
     for line in open(path):
-        m = re.match(r"\|\s*" + kernel_pass + r"\s*\|\s*([0-9.eE+]+)\s*\|", line)
+        m = re.match(r"\|\s*" + kernel_pass + r"\s*\|[^|]*\|\s*\d+\s*\|\s*([0-9.eE+]+)\s*\|", line)
         if m:
             return float(m.group(1))
     return None
