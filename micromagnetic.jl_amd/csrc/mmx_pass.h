@@ -65,6 +65,19 @@ __device__ __forceinline__ void epilogue(const MMX_CONST FieldParams<T> &P, cons
                                                                   ynext, tl);
 }
 
+// Split-step twiddles W_Px^(j + q*TPS), q < EPT, of thread j: one per-lane
+// table load of W^j times the wave-uniform W^(q*TPS) (scalar loads), so the
+// split step costs one vector load per thread instead of EPT.
+template <class T, int EPT, int TPS>
+__device__ __forceinline__ void split_twiddles(cplx<T> *w, int j, const cplx<T> *__restrict__ tw, int sh)
+{
+    cplx<T> wj;
+    gld2(reinterpret_cast<const T *>(tw), (int64_t)(j << sh), wj.x, wj.y);
+    w[0] = wj;
+#pragma unroll
+    for (int q = 1; q < EPT; q++) w[q] = cmul(wj, tw[(q * TPS) << sh]);
+}
+
 // Real-to-complex x transform tail shared by the first and the last pass: v
 // holds this thread's outputs Z[j + q*TPS] of the half-length complex FFT of
 // one row packed as z[n] = (x[2n], x[2n+1]); buf is that sequence's exchange
@@ -84,6 +97,8 @@ __device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T>
     // their registers would add to its peak)
     asm volatile("" : "+s"(tw));
     const int sh = lgLmax - (FftShape<H>::LG + 1);  // W_Px^k = tw[k << sh]
+    cplx<T> ws[EPT];
+    split_twiddles<T, EPT, TPS>(ws, j, tw, sh);
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
         const int k = j + m * TPS;
@@ -93,9 +108,7 @@ __device__ __forceinline__ void r2c_split_store(const cplx<T> *v, int j, cplx<T>
         cplx<T> Fe = {(Zk.x + Zc.x) * T(0.5), (Zk.y + Zc.y) * T(0.5)};
         cplx<T> d = csub(Zk, Zc);
         cplx<T> Fo = {d.y * T(0.5), -d.x * T(0.5)};
-        cplx<T> w;
-        gld2(reinterpret_cast<const T *>(tw), (int64_t)(k << sh), w.x, w.y);  // global, not flat, after the asm
-        dst[k] = cadd(Fe, cmul(w, Fo));
+        dst[k] = cadd(Fe, cmul(ws[m], Fo));
         if (k == 0) dst[H] = csub(Fe, Fo);  // W_Px^H = -1
     }
 }
@@ -216,8 +229,7 @@ __device__ __forceinline__ void p5_body(const DemagGeom &G, const cplx<T> *__res
     // harmless and their results are replaced by zeros, so no load sits
     // behind a branch and they all issue together)
     cplx<T> wq[EPT];
-#pragma unroll
-    for (int q = 0; q < EPT; q++) gld2(reinterpret_cast<const T *>(tw), (int64_t)((j + q * TPS) << sh), wq[q].x, wq[q].y);
+    split_twiddles<T, EPT, TPS>(wq, j, tw, sh);
     if constexpr (MIR) {
         cplx<T> xr[EPT];
 #pragma unroll
