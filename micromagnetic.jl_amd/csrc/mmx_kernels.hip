@@ -361,22 +361,32 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
     const int64_t cs = (int64_t)G.Py2 * KS * Pz2;
     const T *kb = Kt + (((int64_t)kyf * (KS >> 2) + (kxs >> 2)) * Pz2) * 4 + (kxs & 3);
     T kxx[EPT], kyy[EPT], kzz[EPT], kxy[EPT], kxz[EPT], kyz[EPT];
-    constexpr bool PAIR = MMX_P3W_PAIR && TW % 4 == 0 && EPT % 4 == 0;
+    constexpr bool PAIR = MMX_P3W_PAIR && TW % 2 == 0;
+    // the kz line table by lane quads needs 4 kx lanes per j and 4 | EPT
+    constexpr bool LQ = TW % 4 == 0 && EPT % 4 == 0;
     // raw 16-byte loads (issued before the forward transform) and the lane
     // exchanges that complete them (after it, so the transform does not wait
     // for the loads: the wave barriers of its exchanges keep the DPP moves
     // from being scheduled above it)
     cplx<T> pa[PAIR ? EPT : 1], pb[PAIR ? EPT : 1], pc[PAIR ? EPT : 1];
-    T lq[PAIR ? EPT / 4 : 1], lyz = 0, lxz = 0;
+    T lq[PAIR ? (LQ ? EPT / 4 : EPT) : 1], lyz = 0, lxz = 0;
     const int e = t & 1;
     auto load_tensor_pairs = [&]() {
         lyz = Lline[Pz2 + G.Py2 + kxs];
         lxz = Lline[Pz2 + kyf];
-        // lane (j, t) loads Lline at kz = j + (4h + (t&3))*TPS for h < EPT/4
+        if constexpr (LQ) {
+            // lane (j, t) loads Lline at kz = j + (4h + (t&3))*TPS for h < EPT/4
 #pragma unroll
-        for (int h = 0; h < EPT / 4; h++) {
-            const int kzq = j + (4 * h + (t & 3)) * TPS;
-            lq[h] = Lline[kzq <= L / 2 ? kzq : L - kzq];
+            for (int h = 0; h < EPT / 4; h++) {
+                const int kzq = j + (4 * h + (t & 3)) * TPS;
+                lq[h] = Lline[kzq <= L / 2 ? kzq : L - kzq];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < EPT; m++) {
+                const int kz = j + m * TPS;
+                lq[m] = Lline[kz <= L / 2 ? kz : L - kz];
+            }
         }
         // kx pair (2pk, 2pk+1), clamped into the padded rows (lanes past them hold no data)
         const int pk = min(kx >> 1, (KS >> 1) - 1);
@@ -393,14 +403,19 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
         }
     };
     auto finish_tensor_pairs = [&]() {
-        // element m is taken from lane m%4 of the quad (load m/4)
         T lz[EPT];
+        if constexpr (LQ) {
+            // element m is taken from lane m%4 of the quad (load m/4)
 #pragma unroll
-        for (int h = 0; h < EPT / 4; h++) {
-            lz[4 * h + 0] = quad_perm<QP_BCAST<0>>(lq[h]);
-            lz[4 * h + 1] = quad_perm<QP_BCAST<1>>(lq[h]);
-            lz[4 * h + 2] = quad_perm<QP_BCAST<2>>(lq[h]);
-            lz[4 * h + 3] = quad_perm<QP_BCAST<3>>(lq[h]);
+            for (int h = 0; h < EPT / 4; h++) {
+                lz[4 * h + 0] = quad_perm<QP_BCAST<0>>(lq[h]);
+                lz[4 * h + 1] = quad_perm<QP_BCAST<1>>(lq[h]);
+                lz[4 * h + 2] = quad_perm<QP_BCAST<2>>(lq[h]);
+                lz[4 * h + 3] = quad_perm<QP_BCAST<3>>(lq[h]);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < EPT; m++) lz[m] = lq[m];
         }
 #pragma unroll
         for (int m = 0; m < EPT; m++) {

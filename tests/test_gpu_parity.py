@@ -145,6 +145,11 @@ CASES = [
       ("demag", {})]),
     ("no demag local only", (40, 30, 3), (2e-9, 2e-9, 2e-9), "xy",
      [("exch", dict(A=1.5e-11)), ("dmi", dict(D=3e-3, type="interfacial")), ("anis", dict(Ku=8e5, axis=(1, 1, 1)))]),
+    # long z transforms: Pz = 128 (two kx per wave, paired tensor loads) and 256 (one kx per wave)
+    ("tall column nz=40", (6, 5, 40), (2e-9, 2e-9, 2e-9), "xy",
+     [("exch", dict(A=1.5e-11)), ("dmi", dict(D=3e-3, type="interfacial")), ("demag", dict(Nx=1, Ny=1))]),
+    ("tall column nz=80", (4, 3, 80), (2e-9, 2e-9, 1e-9), "",
+     [("exch", dict(A=1.3e-11)), ("anis", dict(Ku=5e5)), ("demag", {})]),
 ]
 
 
