@@ -264,6 +264,7 @@ def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
     b.advance(steps)
     tb = time.perf_counter() - t0
     cpt, grid, items, tune = b.info()
+    shape = b.shape()
     batched_spins = [s.spin for s in sims[:4]]
     del b, sims
     sims = make()
@@ -282,7 +283,8 @@ def bench_batch(mmx, precision, steps, warmup, n_members=48, seed=4004):
                         f"{cells} cells, Heun dt 1e-14, 300 K",
             "value": round(ev / tb / 1e6, 1), "unit": "Mcell-evals/s",
             "plain_value": round(ev / tp / 1e6, 1), "speedup_vs_plain": round(tp / tb, 3),
-            "identical_to_plain": same, "tuned": {"cells_per_thread": cpt, "grid": grid, "work_items": items}}
+            "identical_to_plain": same, "tuned": {"block_threads": shape["block_threads"], "cells_per_thread": cpt,
+                                                  "grid": grid, "work_items": items}}
 
 
 def bench_neb(mmx, precision, steps=20, warmup=3, chains=2, frames=22, seed=6006):

@@ -261,6 +261,12 @@ int mmx_batch_advance(mmx_batch *batch, int64_t nsteps);
  * the stage launch, number of work items) and the candidates' times in ms */
 int mmx_batch_info(mmx_batch *batch, int32_t *cells_per_thread, int32_t *grid, int32_t *work_items,
                    float *tune_ms, int32_t n_tune);
+/* The whole launch shape the set-up picked (threads per block, cells per
+ * thread, grid of the members without demag) and the number of geometry
+ * groups of members with demag.  No counterpart in the reference (it has no
+ * batched launch); for callers that report or pin the tuned shape. */
+int mmx_batch_shape(mmx_batch *batch, int32_t *block_threads, int32_t *cells_per_thread, int32_t *grid,
+                    int32_t *n_groups);
 
 /* Nudged elastic band with the LLG driver: NEB(sim, init_images,
  * frames_between_images; spring_constant, clib_image), src/neb/neb.jl:84-153.

@@ -1652,7 +1652,7 @@ struct BatchEngine : CtxBase {
     DevBuf<BatchStage<T>> st;  // [parity][stage][member]
     DevBuf<Dyn> dyns;
     int V[4] = {0, 0, 0, 0};
-    int cpt = 1, grid = 2048;
+    int cpt = 1, bs = 256, grid = 2048;
     std::vector<float> tune_ms;
     std::vector<T *> bufA, bufB;  // the two state buffers of each member
     hipGraph_t graph = nullptr;
@@ -1715,7 +1715,7 @@ struct BatchEngine : CtxBase {
             g->mem.push_back(m);
         }
         std::vector<int2> it;
-        items_for(4, it);
+        items_for(256, 4, it);
         std::vector<const FieldParams<T> *> ps(M);
         bool fast = true;
         for (int m = 0; m < M; m++) {
@@ -1812,11 +1812,11 @@ struct BatchEngine : CtxBase {
         tune();
     }
 
-    void items_for(int c, std::vector<int2> &it)
+    void items_for(int b, int c, std::vector<int2> &it)
     {
         it.clear();
         for (int m : local)
-            for (int64_t c0 = 0; c0 < E[m]->N; c0 += (int64_t)BLOCK * c) it.push_back(make_int2(m, (int)c0));
+            for (int64_t c0 = 0; c0 < E[m]->N; c0 += (int64_t)b * c) it.push_back(make_int2(m, (int)c0));
         nitems = (int)it.size();
         if (it.empty()) return;
         items.alloc(it.size());
@@ -1867,8 +1867,8 @@ struct BatchEngine : CtxBase {
     void stage_local(int par, int s)
     {
         if (nitems)
-            launch_batch_local<T>(items.p, nitems, Ps.p, st.p + ((size_t)par * S + s) * M, dyns.p, V[s], cpt, grid,
-                                  B->stream);
+            launch_batch_local<T>(items.p, nitems, Ps.p, st.p + ((size_t)par * S + s) * M, dyns.p, V[s], cpt,
+                                  V[s] == 0 ? 256 : bs, grid, B->stream);
     }
 
     void stage(int par, int s)
@@ -1884,44 +1884,52 @@ struct BatchEngine : CtxBase {
     }
 
     // time the first stage of the local members (it writes only stage
-    // buffers) for each launch shape
+    // buffers) for each launch shape: threads per block (the term-list
+    // variant runs 256 only), cells per thread of a work item, grid
     void tune()
     {
         if (local.empty()) return;
+        bool any_list = false;
+        for (int s = 0; s < S; s++) any_list = any_list || V[s] == 0;
+        const std::vector<int> cands_b = any_list ? std::vector<int>{256} : std::vector<int>{128, 256, 512};
         const int cands_c[3] = {1, 2, 4};
         const int cands_g[4] = {1024, 2048, 4096, 8192};
         hipEvent_t a, b;
         HIPCHECK(hipEventCreate(&a));
         HIPCHECK(hipEventCreate(&b));
         float best = 1e30f;
-        int bc = 1, bg = 2048;
+        int bb = 256, bc = 1, bg = 2048;
         std::vector<int2> it;
         tune_ms.clear();
-        for (int c : cands_c) {
-            items_for(c, it);
-            for (int g : cands_g) {
-                cpt = c;
-                grid = g;
-                stage_local(0, 0);
-                HIPCHECK(hipEventRecord(a, B->stream));
-                for (int r = 0; r < 3; r++) stage_local(0, 0);
-                HIPCHECK(hipEventRecord(b, B->stream));
-                HIPCHECK(hipEventSynchronize(b));
-                float ms = 0;
-                HIPCHECK(hipEventElapsedTime(&ms, a, b));
-                tune_ms.push_back(ms / 3);
-                if (ms < best) {
-                    best = ms;
-                    bc = c;
-                    bg = g;
+        for (int t : cands_b)
+            for (int c : cands_c) {
+                items_for(t, c, it);
+                for (int g : cands_g) {
+                    bs = t;
+                    cpt = c;
+                    grid = g;
+                    stage_local(0, 0);
+                    HIPCHECK(hipEventRecord(a, B->stream));
+                    for (int r = 0; r < 3; r++) stage_local(0, 0);
+                    HIPCHECK(hipEventRecord(b, B->stream));
+                    HIPCHECK(hipEventSynchronize(b));
+                    float ms = 0;
+                    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+                    tune_ms.push_back(ms / 3);
+                    if (ms < best) {
+                        best = ms;
+                        bb = t;
+                        bc = c;
+                        bg = g;
+                    }
                 }
             }
-        }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
+        bs = bb;
         cpt = bc;
         grid = bg;
-        items_for(cpt, it);
+        items_for(bs, cpt, it);
     }
 
     void capture()
@@ -3095,6 +3103,18 @@ int mmx_batch_advance(mmx_batch *b, int64_t nsteps)
 {
     if (nsteps < 0) return fail(MMX_EINVAL, "nsteps must be >= 0");
     return with_batch(b, [&](auto &e) { return e.advance(nsteps); });
+}
+
+int mmx_batch_shape(mmx_batch *b, int32_t *block_threads, int32_t *cells_per_thread, int32_t *grid,
+                    int32_t *n_groups)
+{
+    return with_batch(b, [&](auto &e) {
+        if (block_threads) *block_threads = e.bs;
+        if (cells_per_thread) *cells_per_thread = e.cpt;
+        if (grid) *grid = e.grid;
+        if (n_groups) *n_groups = (int32_t)e.groups.size();
+        return MMX_OK;
+    });
 }
 
 int mmx_batch_info(mmx_batch *b, int32_t *cells_per_thread, int32_t *grid, int32_t *work_items, float *tune_ms,

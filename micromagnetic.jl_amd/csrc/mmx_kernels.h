@@ -89,11 +89,12 @@ template <class T>
 void launch_adj_final(int64_t N, const T *y0, const T *const k[4], double c, const T *lam, T *lam0, T *const a[4],
                       hipStream_t s);
 
-// batches without demag: one launch per stage over every member; V is the
-// epilogue variant (batch_variant), cpt the cells per thread of a work item
+// batch members without demag: one launch per stage over all of them; V is
+// the epilogue variant (batch_variant), cpt the cells per thread of a work
+// item, bs the threads per block (128, 256 or 512; 256 for the term list)
 template <class T>
 void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *const *Ps, const BatchStage<T> *st,
-                        const Dyn *dyns, int V, int cpt, int grid, hipStream_t s);
+                        const Dyn *dyns, int V, int cpt, int bs, int grid, hipStream_t s);
 void launch_batch_tick(Dyn *dyns, int M, hipStream_t s);
 int batch_variant(int mode, int fast, int np);
 

@@ -511,8 +511,8 @@ __global__ void __launch_bounds__(256) k_convw(DemagGeom G, cplx<T> *__restrict_
 // range x*ceil(n/8) ... so that the y/z neighbour rows of a member's item
 // were fetched by the same XCD's L2 shortly before.
 // ---------------------------------------------------------------------------
-template <class T, int V, int CPT>
-__global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ items, int nitems,
+template <class T, int V, int CPT, int BS>
+__global__ void __launch_bounds__(BS) k_batch_local(const int2 *__restrict__ items, int nitems,
                                                         const FieldParams<T> *const *__restrict__ Ps,
                                                         const BatchStage<T> *__restrict__ st,
                                                         const Dyn *__restrict__ dyns)
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_local(const int2 *__restrict__ 
         const int N = (int)P.N;
         double red = 0;
         for (int c = 0; c < CPT; c++) {
-            const int I = w.y + c * BLOCK + threadIdx.x;
+            const int I = w.y + c * BS + threadIdx.x;
             if (I < N) {
                 const unsigned q = (unsigned)I / nx;
                 const int i = (int)((unsigned)I - q * nx);
@@ -1453,17 +1453,23 @@ void launch_adj_final(int64_t N, const T *y0, const T *const k[4], double c, con
 
 template <class T>
 void launch_batch_local(const int2 *items, int nitems, const FieldParams<T> *const *Ps, const BatchStage<T> *st,
-                        const Dyn *dyns, int V, int cpt, int grid, hipStream_t s)
+                        const Dyn *dyns, int V, int cpt, int bs, int grid, hipStream_t s)
 {
-#define BL_CALL(VV, CC) hipLaunchKernelGGL((k_batch_local<T, VV, CC>), dim3(grid), dim3(BLOCK), 0, s, items, nitems, Ps, st, dyns)
-#define BL_CPT(VV)                      \
-    if (cpt == 1) BL_CALL(VV, 1);       \
-    else if (cpt == 2) BL_CALL(VV, 2);  \
-    else BL_CALL(VV, 4);
-    if (V == 4) { BL_CPT(4) }
-    else if (V == 1) { BL_CPT(1) }
-    else if (V == 2) { BL_CPT(2) }
-    else { BL_CPT(0) }
+#define BL_CALL(VV, CC, BB) \
+    hipLaunchKernelGGL((k_batch_local<T, VV, CC, BB>), dim3(grid), dim3(BB), 0, s, items, nitems, Ps, st, dyns)
+#define BL_CPT(VV, BB)                      \
+    if (cpt == 1) BL_CALL(VV, 1, BB);       \
+    else if (cpt == 2) BL_CALL(VV, 2, BB);  \
+    else BL_CALL(VV, 4, BB);
+#define BL_BS(VV)                                  \
+    if (bs == 128) { BL_CPT(VV, 128) }             \
+    else if (bs == 512) { BL_CPT(VV, 512) }        \
+    else { BL_CPT(VV, 256) }
+    if (V == 4) { BL_BS(4) }
+    else if (V == 1) { BL_BS(1) }
+    else if (V == 2) { BL_BS(2) }
+    else { BL_CPT(0, 256) }  // the term-list variant: 256 threads only
+#undef BL_BS
 #undef BL_CPT
 #undef BL_CALL
 }
@@ -1589,7 +1595,7 @@ void launch_normalise(int64_t N, T *m, hipStream_t s)
     template void launch_axpy<T>(const T *, double, const T *, T *, int64_t, hipStream_t);        \
     template int launch_maxabs<T>(const T *, int64_t, double *, hipStream_t);                    \
     template void launch_batch_local<T>(const int2 *, int, const FieldParams<T> *const *, const BatchStage<T> *, \
-                                        const Dyn *, int, int, int, hipStream_t);                              \
+                                        const Dyn *, int, int, int, int, hipStream_t);                              \
     template int launch_sd_g<T>(int64_t, const T *, const T *, const T *, T *, double *, hipStream_t); \
     template void launch_sd_update<T>(int64_t, const FieldParams<T> *, const T *, T *, T *, const T *,        \
                                       const double *, hipStream_t);                                         \

@@ -985,6 +985,13 @@ class Batch:
         check(_lib.load().mmx_batch_info(self._h, C.byref(c), C.byref(g), C.byref(w), t, 12))
         return c.value, g.value, w.value, list(t)
 
+    def shape(self):
+        """The whole tuned launch shape: dict(block_threads, cells_per_thread,
+        grid, demag_groups)."""
+        b, c, g, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(_lib.load().mmx_batch_shape(self._h, C.byref(b), C.byref(c), C.byref(g), C.byref(n)))
+        return dict(block_threads=b.value, cells_per_thread=c.value, grid=g.value, demag_groups=n.value)
+
 
 # ---- nudged elastic band -----------------------------------------------------
 def _slerp(x1, x2, t):

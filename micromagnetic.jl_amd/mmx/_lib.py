@@ -96,6 +96,7 @@ SIGNATURES = {
     "mmx_batch_destroy": [_vp],
     "mmx_batch_advance": [_vp, C.c_int64],
     "mmx_batch_info": [_vp, _i32p, _i32p, _i32p, C.POINTER(C.c_float), C.c_int32],
+    "mmx_batch_shape": [_vp, _i32p, _i32p, _i32p, _i32p],
     "mmx_neb_create": [C.POINTER(_vp), _vp, _dp, C.c_int, C.c_double, C.c_int],
     "mmx_neb_destroy": [_vp],
     "mmx_neb_set_driver": [_vp, C.c_double, C.c_double, C.c_int, C.c_double],

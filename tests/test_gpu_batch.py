@@ -47,7 +47,11 @@ def test_batch_matches_members_alone(integrator, thermal):
     alone = _build(7, 9, integrator, thermal=thermal)
     b = mmx.Batch([s for s, _ in batch_members])
     cpt, grid, items, _ = b.info()
-    print(f"[batch] tuned: {cpt} cells/thread, grid {grid}, {items} work items", flush=True)
+    shape = b.shape()
+    print(f"[batch] tuned: {shape['block_threads']} threads/block, {cpt} cells/thread, grid {grid}, {items} work items",
+          flush=True)
+    assert shape["block_threads"] in (128, 256, 512) and shape["cells_per_thread"] == cpt
+    assert shape["demag_groups"] == 0
     for n in (7, 4, 1):  # odd and even counts, several calls
         b.advance(n)
         for s, _ in alone:
@@ -157,6 +161,7 @@ def test_batch_demag_groups_match_members_alone(integrator, thermal):
     mem = _mixed(11, integrator, thermal)
     ref = _mixed(11, integrator, thermal)
     b = mmx.Batch([s for s, _ in mem])
+    assert b.shape()["demag_groups"] == 3
     for n in (5, 2, 1):
         b.advance(n)
         for s, _ in ref:
