@@ -474,8 +474,11 @@ end
 """
     batch_advance!(sims, nsteps)
 
-Advance several attached simulations (RungeKutta or Heun, no demag) together:
-one launch per stage over all of them, replayed from a HIP graph.
+Advance several attached simulations (all RungeKutta or all Heun) together:
+one launch per stage over those without demag, one launch per demag pass and
+stage for each group of those with demag that share a geometry, replayed
+from a HIP graph.  Each simulation ends where `run_sim` stepping it alone
+would leave it.
 """
 function batch_advance!(sims::AbstractVector, nsteps::Integer)
     hs = Ptr{Cvoid}[contexts[s] for s in sims]
