@@ -102,6 +102,67 @@ def test_film_256x256x32_fields_energies_rk4(oracle):
     assert e3 <= TOL
 
 
+def _grf(rng, nx, ny, dx, corr, mean, rel):
+    """Seeded Gaussian random field: white noise filtered by a Gaussian of
+    correlation length corr, unit variance, then mean * (1 + rel * g)."""
+    g = rng.standard_normal((ny, nx))
+    kx = np.fft.fftfreq(nx, d=dx) * 2 * np.pi
+    ky = np.fft.fftfreq(ny, d=dx) * 2 * np.pi
+    K2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    g = np.real(np.fft.ifft2(np.fft.fft2(g) * np.exp(-K2 * corr * corr / 4)))
+    g = (g - g.mean()) / g.std()
+    return mean * (1 + rel * g)
+
+
+@pytest.mark.timeout(600)
+def test_differentiation_film_128x128x8_forward(oracle):
+    """The differentiation workload's forward path at its size: 128x128x8, 2 nm
+    cells, Ku(x,y) and D(x,y) Gaussian random fields (means 6e5 J/m^3 and
+    2.5e-3 J/m^2, 20 % std, 10 nm correlation), exchange + interfacial DMI +
+    anisotropy + demag (transform lengths 128 / 256 / 16, the slot-form kernels
+    with per-cell parameters); H_eff, dm/dt and 4 RK4 steps at dt 5e-14."""
+    import mmx
+
+    note("128x128x8 spatial Ku / D film with demag: fields and 4 RK4 steps")
+    nx, ny, nz, d = 128, 128, 8, 2e-9
+    N = nx * ny * nz
+    rng = np.random.default_rng(5005)
+    Ku = np.tile(_grf(rng, nx, ny, d, 10e-9, 6e5, 0.2).reshape(-1), nz)
+    D = np.tile(_grf(np.random.default_rng(5006), nx, ny, d, 10e-9, 2.5e-3, 0.2).reshape(-1), nz)
+    r = np.random.default_rng(5009)
+    m0 = np.array([0.1, 0.1, 1.0]) + 0.3 * r.standard_normal((N, 3))
+    m0 = (m0 / np.linalg.norm(m0, axis=1, keepdims=True)).reshape(-1)
+    mesh = mmx.FDMesh(dx=d, dy=d, dz=d, nx=nx, ny=ny, nz=nz)
+    g = mmx.Sim(mesh)
+    o = oracle.OracleSim(nx, ny, nz, d, d, d)
+    mmx.set_Ms(g, 6e5)
+    o.set_Ms(np.full(N, 6e5))
+    mmx.init_m0(g, m0)
+    o.init_m0(m0)
+    g.driver.alpha, g.driver.gamma, g.driver.precession = 0.1, 2.21e5, True
+    o.set_llg(0.1, 2.21e5, True)
+    mmx.add_exch(g, 1.3e-11)
+    o.add_exch(1.3e-11)
+    mmx.add_dmi(g, D, type="interfacial")
+    o.add_dmi(D, type="interfacial")
+    mmx.add_anis(g, Ku, axis=(0, 0, 1))
+    o.add_anis(Ku, axis=(0, 0, 1))
+    mmx.add_demag(g)
+    o.add_demag(0, 0, 0)
+    eH = relerr(mmx.effective_field(g), o.effective_field())
+    eR = relerr(mmx.llg_rhs(g), o.llg_rhs())
+    print(f"H_eff {eH:.2e}  dm/dt {eR:.2e}", flush=True)
+    assert eH <= TOL and eR <= TOL
+    mmx.set_driver(g, integrator="RungeKutta", step=5e-14)
+    o.set_integrator(oracle.RK4, 5e-14, 0)
+    mmx.run_step(g, n=4)
+    for _ in range(4):
+        o.advance_step()
+    e4 = np.max(np.abs(g.spin - o.spin))
+    print(f"4 RK4 steps: max |dm| {e4:.2e}", flush=True)
+    assert e4 <= TOL
+
+
 def dipole_xx(x, y, z, V):
     """Point-dipole Nxx of src/atomistic/demag.jl:106-113 times V/(4 pi)."""
     R = x * x + y * y + z * z
