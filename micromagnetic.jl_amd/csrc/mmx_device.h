@@ -857,11 +857,15 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
     const int N = (int)P.N;
     const T alpha_ = P.alpha, llg_a_ = P.llg_a;
     const bool prec_ = (P.flags & FL_PREC) != 0;
-    // operands of the stage combination, loaded with the field's loads
+    // operands of the stage combination, loaded with the field's loads; the
+    // first stage of a step evaluates at its base state (y0 == m), whose
+    // values the field reads anyway
     T y0v[3] = {0, 0, 0}, kpv[3][3];
+    const bool y0m = E.y0 == m;
     if constexpr (MODE != EPI_ERR) {
+        if (!y0m)
 #pragma unroll
-        for (int c = 0; c < 3; c++) y0v[c] = gld(E.y0, c * N + I);
+            for (int c = 0; c < 3; c++) y0v[c] = gld(E.y0, c * N + I);
     }
     if constexpr (MODE == EPI_RK4_FINAL) {
 #pragma unroll
@@ -873,6 +877,13 @@ __device__ __forceinline__ void finish_fast(const MMX_CONST FieldParams<T> &P, c
     T mx, my, mz, Hx, Hy, Hz;
     uint32_t msk;
     cell_field_fast<T, SIG, TILE>(P, dyn, I, i, j, k, m, hdx, hdy, hdz, mx, my, mz, Hx, Hy, Hz, msk, tl);
+    if constexpr (MODE != EPI_ERR) {
+        if (y0m) {
+            y0v[0] = mx;
+            y0v[1] = my;
+            y0v[2] = mz;
+        }
+    }
     // the earlier stages' k of a DOPRI54 / BS23 / Heun combination or error
     // estimate: all issued here together (uniform guards, no use inside),
     // after the field so that they do not add to its register peak
