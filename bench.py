@@ -568,6 +568,42 @@ def bench_adjoint(mmx, precision, nsteps=2000, every=50, n_fd=8, seed=5005):
     return out
 
 
+def bench_diff_forward(mmx, precision, nsteps=2000, warmup=20, seed=5005):
+    """Forward trajectory of the differentiation workload alone: the same
+    128x128x8 film as bench_adjoint (spatial Ku and D, exchange + interfacial
+    DMI + anisotropy + demag, RK4 dt 5e-14), nsteps plain RK4 steps with no
+    recording and no backward pass."""
+    nx, ny, nz, d = 128, 128, 8, 2e-9
+    N = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    Ku = np.tile(grf(rng, nx, ny, d, 10e-9, 6e5, 0.2).reshape(-1), nz)
+    D = np.tile(grf(np.random.default_rng(seed + 1), nx, ny, d, 10e-9, 2.5e-3, 0.2).reshape(-1), nz)
+    r = np.random.default_rng(seed + 4)
+    m0 = np.array([0.1, 0.1, 1.0]) + 0.3 * r.standard_normal((N, 3))
+    m0 = (m0 / np.linalg.norm(m0, axis=1, keepdims=True)).reshape(-1)
+    mesh = mmx.FDMesh(dx=d, dy=d, dz=d, nx=nx, ny=ny, nz=nz)
+    sim = mmx.Sim(mesh, integrator="RungeKutta", precision=precision)
+    mmx.set_Ms(sim, 5.8e5)
+    mmx.init_m0(sim, m0, norm=False)
+    mmx.add_exch(sim, 1.5e-11)
+    mmx.add_dmi(sim, D, type="interfacial", name="D")
+    mmx.add_anis(sim, Ku, axis=(0, 0, 1), name="Ku")
+    mmx.add_demag(sim)
+    mmx.set_driver(sim, integrator="RungeKutta", step=5e-14)
+    L = mmx._lib.load()
+    mmx._lib.check(L.mmx_advance_step(sim._h, warmup))
+    mmx._lib.check(L.mmx_synchronize(sim._h))
+    t0 = time.perf_counter()
+    mmx._lib.check(L.mmx_advance_step(sim._h, nsteps))
+    mmx._lib.check(L.mmx_synchronize(sim._h))
+    t = time.perf_counter() - t0
+    mz = float(sim.spin.reshape(-1, 3)[:, 2].mean())
+    return {"workload": f"128x128x8 film, 2 nm, exch+iDMI(x,y)+Ku(x,y)+demag, RK4 dt 5e-14, {nsteps} steps "
+                        f"(forward only, no recording)",
+            "value": round(N * 4 * nsteps / t / 1e6, 1), "unit": "Mcell-evals/s",
+            "ms_per_step": round(t / nsteps * 1e3, 4), "mean_mz_end": round(mz, 6)}
+
+
 def read_traffic(kernel_pass):
     """HBM bytes per launch of a pass from the committed PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
@@ -723,6 +759,12 @@ def run_forms(args, out_path):
         if "value" in res.get(k, {}):
             log(f"[bench] {k}: {res[k]['value']} Mcell-evals/s over the 96 systems "
                 f"(back to back {res[k]['back_to_back_value']})")
+    leg("diff_forward_fp64", lambda: bench_diff_forward(mmx, mmx.F64))
+    leg("diff_forward_fp32", lambda: bench_diff_forward(mmx, mmx.F32))
+    for k in ("diff_forward_fp64", "diff_forward_fp32"):
+        if "value" in res.get(k, {}):
+            log(f"[bench] {k}: {res[k]['value']} Mcell-evals/s, {res[k]['ms_per_step']} ms per RK4 step "
+                f"(128x128x8 differentiation film, forward only)")
     if ADJOINT_ENABLED:
         log("[bench] differentiation form (FP64 with finite-difference probes, FP32)")
         leg("adjoint_fp64", lambda: bench_adjoint(mmx, mmx.F64, n_fd=args.fd_cells))

@@ -647,6 +647,7 @@ struct Engine : CtxBase {
         G.Py2 = G.Py / 2 + 1;
         G.Pz2 = G.Pz / 2 + 1;
         G.ktw = convz_grouped_tensor(G, (int)sizeof(T));
+        G.p5rx = p5_rows(G, 1);
         if (G.H > 2048 || G.Py > 2048 || G.Pz > 2048) {
             fail(MMX_EINVAL, "mesh %dx%dx%d exceeds the demag transform limits of this build", nx, ny, nz);
             throw HipError{MMX_EINVAL};
@@ -720,7 +721,7 @@ struct Engine : CtxBase {
             HIPCHECK(hipStreamSynchronize(C->stream));
         }
         // the last pass writes one reduction slot group per block
-        const int64_t slots = std::max<int64_t>(max_local_blocks(), p5_blocks(G)) + 64;
+        const int64_t slots = std::max<int64_t>(max_local_blocks(), p5_grid(G)) + 64;
         if ((int64_t)partial.n < slots) partial.alloc(slots);
         Abuf.alloc((int64_t)3 * nz * ny * G.KS);
         Bbuf.alloc((int64_t)3 * nz * G.Py * G.KS);

@@ -308,6 +308,7 @@ struct DemagGeom {
     int Lmax, lgLmax;   // twiddle table length
     int Py2, Pz2;       // Py/2+1, Pz/2+1 (folded tensor extents)
     int ktw;            // 1: folded tensor grouped by 4 kx per (kyf, kzf run), for the wave-form z pass
+    int p5rx;           // rows per block of the last pass (at most its shape's RX; p5_rows)
 };
 
 }  // namespace mmx

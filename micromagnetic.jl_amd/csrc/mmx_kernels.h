@@ -133,11 +133,11 @@ template <class T>
 void launch_normalise(int64_t N, T *m, hipStream_t s);
 
 int max_local_blocks();
-// blocks of one launch of the last demag pass for this geometry
+// blocks of one launch of the last demag pass for this geometry (G.p5rx rows each)
 int p5_grid(const DemagGeom &G);
+// rows per block of the last pass for a launch over `copies` systems of G
+int p5_rows(const DemagGeom &G, int copies);
 // streaming copy of n16 16-byte words (HBM ceiling probe)
 void launch_stream_copy(const void *src, void *dst, int64_t n16, int grid, hipStream_t s);
-// number of blocks (reduction slot groups) the last demag pass launches
-int p5_blocks(const DemagGeom &G);
 
 }  // namespace mmx

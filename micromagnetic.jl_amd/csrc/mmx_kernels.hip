@@ -1370,16 +1370,6 @@ void launch_sum_images(const T *v, int64_t N, int n, int nb, double *partial, do
     hipLaunchKernelGGL(k_segment_sum, dim3(n), dim3(BLOCK), 0, s, partial, nb, out);
 }
 
-#ifdef MMX_KT_MAIN
-int p5_blocks(const DemagGeom &G)
-{
-    int rx = 1;
-#define RX_CALL(H) rx = Shape<H>::RX;
-    MMX_LG_SWITCH(G.lgH, RX_CALL)
-#undef RX_CALL
-    return (G.ny * ((G.nz + rx - 1) / rx) + 7) / 8 * 8;
-}
-#endif
 
 
 #ifdef MMX_KT_MAIN

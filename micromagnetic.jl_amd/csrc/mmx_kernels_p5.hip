@@ -14,14 +14,29 @@
 namespace mmx {
 
 #ifdef MMX_KT_MAIN_P5
-// blocks of one last-pass launch: RX rows per block, groups rounded up to the 8 XCDs
+// blocks of one last-pass launch: G.p5rx rows per block, groups rounded up to
+// the 8 XCDs
 int p5_grid(const DemagGeom &G)
 {
-    int g = 0;
-#define P5G_CALL(H) g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;
-    MMX_LG_SWITCH(G.lgH, P5G_CALL)
-#undef P5G_CALL
-    return g;
+    return (G.ny * ((G.nz + G.p5rx - 1) / G.p5rx) + 7) / 8 * 8;
+}
+
+// Rows per block of the last pass for a launch over `copies` systems of one
+// geometry: the largest of RX, RX/2, ..., 1 that gives the launch at least
+// MMX_P5_MIN_BLOCKS blocks (2 per CU), else the one that gives the most.  A
+// small mesh (128x128x8: 256 blocks at RX = 4) otherwise leaves each CU one
+// block of three waves, and the pass waits on memory latency.
+int p5_rows(const DemagGeom &G, int copies)
+{
+    int rx = 1;
+#define P5R_CALL(H) rx = Shape<H>::RX;
+    MMX_LG_SWITCH(G.lgH, P5R_CALL)
+#undef P5R_CALL
+    auto blocks = [&](int r) { return (int64_t)copies * G.ny * ((G.nz + r - 1) / r); };
+    int best = rx;
+    for (int r = rx / 2; r >= 1 && blocks(best) < MMX_P5_MIN_BLOCKS; r /= 2)
+        if (blocks(r) > blocks(best)) best = r;
+    return best;
 }
 #endif
 
@@ -36,7 +51,7 @@ int launch_p5(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P, con
     int g = 0;
 #define P5_CALL(H)                                                                                   \
     {                                                                                                \
-        g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;                         \
+        g = p5_grid(G);                                                                              \
         const int v = variant(E.mode, fast, E.np);                                                   \
         if (v == 4)                                                                                  \
             hipLaunchKernelGGL((k_p5<T, H, 4>), dim3(g), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, tw); \
@@ -62,11 +77,13 @@ void launch_p5_images(const DemagGeom &G, const cplx<T> *A, const FieldParams<T>
 {
 #define P5I_CALL(H)                                                                                  \
     {                                                                                                \
-        const int g = (G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8;               \
-        hipLaunchKernelGGL((k_p5<T, H, 0, 0u, true>), dim3(g, nimg), dim3(Shape<H>::B5), 0, s, G, A, P, dyn, E, m, \
+        const int g = p5_grid(Gi);                                                                   \
+        hipLaunchKernelGGL((k_p5<T, H, 0, 0u, true>), dim3(g, nimg), dim3(Shape<H>::B5), 0, s, Gi, A, P, dyn, E, m, \
                            tw);                                                                      \
     }
-    MMX_LG_SWITCH(G.lgH, P5I_CALL)
+    DemagGeom Gi = G;
+    Gi.p5rx = p5_rows(G, nimg);
+    MMX_LG_SWITCH(Gi.lgH, P5I_CALL)
 #undef P5I_CALL
 }
 
@@ -77,18 +94,20 @@ void launch_p5_batch(const DemagGeom &G, const cplx<T> *A, int64_t astride, cons
 {
 #define P5B_CALL(H)                                                                                  \
     {                                                                                                \
-        const dim3 g((G.ny * ((G.nz + Shape<H>::RX - 1) / Shape<H>::RX) + 7) / 8 * 8, nmem);         \
+        const dim3 g(p5_grid(Gb), nmem);                                                             \
         const dim3 b(Shape<H>::B5);                                                                  \
         if (v == 4)                                                                                  \
-            hipLaunchKernelGGL((k_p5_batch<T, H, 4>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 4>), g, b, 0, s, Gb, A, astride, gidx, Ps, st, dyns, tw); \
         else if (v == 1)                                                                             \
-            hipLaunchKernelGGL((k_p5_batch<T, H, 1>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 1>), g, b, 0, s, Gb, A, astride, gidx, Ps, st, dyns, tw); \
         else if (v == 2)                                                                             \
-            hipLaunchKernelGGL((k_p5_batch<T, H, 2>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 2>), g, b, 0, s, Gb, A, astride, gidx, Ps, st, dyns, tw); \
         else                                                                                         \
-            hipLaunchKernelGGL((k_p5_batch<T, H, 0>), g, b, 0, s, G, A, astride, gidx, Ps, st, dyns, tw); \
+            hipLaunchKernelGGL((k_p5_batch<T, H, 0>), g, b, 0, s, Gb, A, astride, gidx, Ps, st, dyns, tw); \
     }
-    MMX_LG_SWITCH(G.lgH, P5B_CALL)
+    DemagGeom Gb = G;
+    Gb.p5rx = p5_rows(G, nmem);
+    MMX_LG_SWITCH(Gb.lgH, P5B_CALL)
 #undef P5B_CALL
 }
 

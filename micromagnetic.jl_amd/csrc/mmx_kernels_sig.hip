@@ -29,7 +29,7 @@ static int p5_h(int v, const DemagGeom &G, const cplx<T> *A, const FieldParams<T
 {
 #define P5S(HH)                                                                                      \
     case HH: {                                                                                       \
-        const int g = (G.ny * ((G.nz + Shape<HH>::RX - 1) / Shape<HH>::RX) + 7) / 8 * 8;              \
+        const int g = p5_grid(G);                                                                    \
         return p5_v<T, HH, SIG>(v, g, G, A, P, dyn, E, m, tw, s);                                     \
     }
     switch (G.H) {

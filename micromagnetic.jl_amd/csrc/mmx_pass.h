@@ -143,6 +143,10 @@ struct RowWave {
 #ifndef MMX_P5_WPE
 #define MMX_P5_WPE 3
 #endif
+// launches of the last pass with fewer blocks take fewer rows per block (p5_rows)
+#ifndef MMX_P5_MIN_BLOCKS
+#define MMX_P5_MIN_BLOCKS 512
+#endif
 // the term-list variants (V = 0, 5) hold more live values: 2 waves per SIMD
 // leave them their registers without scratch spills
 #define MMX_P5_ATTR __attribute__((amdgpu_waves_per_eu((V == 0 || V == 5) ? 2 : MMX_P5_WPE)))
@@ -162,12 +166,14 @@ __device__ __forceinline__ void p5_body(const DemagGeom &G, const cplx<T> *__res
     __shared__ cplx<T> lds[3 * RX * HS];
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const int nrows_tot = ny * nz;
-    // A block takes RX rows (x lines) at one y and consecutive z.  Groups are
+    // A block takes rx <= RX rows (x lines) at one y and consecutive z (rows
+    // r >= rx of its layout stay idle in the transforms).  Groups are
     // ordered y-major, z-minor, and each XCD (launch indices go round-robin
     // over the 8 XCDs) gets one contiguous run of groups: the z+-1 rows a cell
     // reads were just fetched by this or the previous block, the y+-1 rows a
     // few blocks earlier, so both come from that XCD's L2.
-    const int gz = (nz + RX - 1) / RX;
+    const int rx = G.p5rx;
+    const int gz = (nz + rx - 1) / rx;
     const int ngroups = ny * gz;
     const int per = (ngroups + 7) / 8;
     const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -175,8 +181,8 @@ __device__ __forceinline__ void p5_body(const DemagGeom &G, const cplx<T> *__res
         if (threadIdx.x == 0 && (E.mode == EPI_ERR || E.mode == EPI_ENERGY)) E.partial[blockIdx.x] = 0.0;
         return;
     }
-    const int y0 = lb / gz, z0 = (lb % gz) * RX;
-    const int nrows = min(RX, nz - z0);
+    const int y0 = lb / gz, z0 = (lb % gz) * rx;
+    const int nrows = min(rx, nz - z0);
     const int ls = threadIdx.x / TPS, j = threadIdx.x % TPS;
     const int c = ls / RX, r = ls % RX;
     const bool active = r < nrows;

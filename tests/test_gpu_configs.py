@@ -13,6 +13,9 @@ against the CPU oracle or fixtures the oracle wrote).
   the FP64 oracle (tests/golden/cfg3_fields.npz), the FP64 path to 1e-12 at
   the same cells, and the FP32 trajectory error per Dormand-Prince step
   against the FP64 path.
+* 128x128x8 film with Ku(x,y) and D(x,y) random fields and demag (the
+  differentiation workload's forward path): FP64 to 1e-12, FP32 against the
+  FP64 oracle to 1e-5 in H_eff and dm/dt.
 Each case prints its name before it touches the GPU.
 """
 import json
@@ -161,6 +164,52 @@ def test_differentiation_film_128x128x8_forward(oracle):
     e4 = np.max(np.abs(g.spin - o.spin))
     print(f"4 RK4 steps: max |dm| {e4:.2e}", flush=True)
     assert e4 <= TOL
+
+
+@pytest.mark.timeout(600)
+def test_differentiation_film_128x128x8_forward_fp32(oracle):
+    """The same 128x128x8 film through the FP32 kernels against the FP64
+    oracle: H_eff and dm/dt within 1e-5 (max-norm relative, the bar of the
+    512x512x64 FP32 case), and m after 4 RK4 steps within 1e-6."""
+    import mmx
+
+    note("128x128x8 spatial Ku / D film with demag, FP32: fields and 4 RK4 steps")
+    nx, ny, nz, d = 128, 128, 8, 2e-9
+    N = nx * ny * nz
+    Ku = np.tile(_grf(np.random.default_rng(5005), nx, ny, d, 10e-9, 6e5, 0.2).reshape(-1), nz)
+    D = np.tile(_grf(np.random.default_rng(5006), nx, ny, d, 10e-9, 2.5e-3, 0.2).reshape(-1), nz)
+    r = np.random.default_rng(5009)
+    m0 = np.array([0.1, 0.1, 1.0]) + 0.3 * r.standard_normal((N, 3))
+    m0 = (m0 / np.linalg.norm(m0, axis=1, keepdims=True)).reshape(-1)
+    mesh = mmx.FDMesh(dx=d, dy=d, dz=d, nx=nx, ny=ny, nz=nz)
+    g = mmx.Sim(mesh, precision=mmx.F32)
+    o = oracle.OracleSim(nx, ny, nz, d, d, d)
+    mmx.set_Ms(g, 6e5)
+    o.set_Ms(np.full(N, 6e5))
+    mmx.init_m0(g, m0)
+    o.init_m0(m0)
+    g.driver.alpha, g.driver.gamma, g.driver.precession = 0.1, 2.21e5, True
+    o.set_llg(0.1, 2.21e5, True)
+    mmx.add_exch(g, 1.3e-11)
+    o.add_exch(1.3e-11)
+    mmx.add_dmi(g, D, type="interfacial")
+    o.add_dmi(D, type="interfacial")
+    mmx.add_anis(g, Ku, axis=(0, 0, 1))
+    o.add_anis(Ku, axis=(0, 0, 1))
+    mmx.add_demag(g)
+    o.add_demag(0, 0, 0)
+    eH = relerr(mmx.effective_field(g), o.effective_field())
+    eR = relerr(mmx.llg_rhs(g), o.llg_rhs())
+    print(f"FP32 H_eff {eH:.2e}  dm/dt {eR:.2e}", flush=True)
+    assert eH <= 1e-5 and eR <= 1e-5
+    mmx.set_driver(g, integrator="RungeKutta", step=5e-14)
+    o.set_integrator(oracle.RK4, 5e-14, 0)
+    mmx.run_step(g, n=4)
+    for _ in range(4):
+        o.advance_step()
+    e4 = np.max(np.abs(g.spin - o.spin))
+    print(f"FP32 4 RK4 steps: max |dm| {e4:.2e}", flush=True)
+    assert e4 <= 1e-6
 
 
 def dipole_xx(x, y, z, V):
