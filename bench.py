@@ -799,6 +799,9 @@ def main():
     ap.add_argument("--term-list", action="store_true",
                     help="time the same film through the term-list kernels (a second, zero Zeeman term: a kind "
                          "that appears twice takes the term list) and exit")
+    ap.add_argument("--diff-film", action="store_true",
+                    help="time the forward RK4 steps of the 128x128x8 differentiation film (spatial Ku and D, "
+                         "demag; FP64 and FP32) and exit")
     ap.add_argument("--fd-cells", type=int, default=64, help="finite-difference probes per gradient quantity")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the library's streaming copy of 1 GiB three times (calibrates HBM counters)")
@@ -846,6 +849,12 @@ def main():
         log(f"[bench] term-list FP64: {N * 4 * args.steps / el / 1e6:.1f} Mcell-evals/s, "
             f"{el / args.steps * 1e3:.3f} ms/step; " +
             ", ".join(f"{k} {v[0] * 1e3:.1f}us" for k, v in passes.items() if v[0] is not None))
+        return
+    if args.diff_film:
+        for prec, name in ((mmx.F64, "FP64"), (mmx.F32, "FP32")):
+            r = bench_diff_forward(mmx, prec, nsteps=args.steps, warmup=args.warmup)
+            log(f"[bench] differentiation film {name}: {r['value']} Mcell-evals/s, {r['ms_per_step']} ms per RK4 "
+                f"step ({r['workload']})")
         return
     errors = {}
     log(f"[bench] FP64 set-up {nx}x{ny}x{nz}, images {args.images}")

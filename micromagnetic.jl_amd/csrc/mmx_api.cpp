@@ -388,7 +388,8 @@ struct Engine : CtxBase {
             for (int i = 0; i < P.nterms; i++) o |= (uint32_t)P.ord[i] << (3 * i);
             P.ordp = o;
         }
-        // term signature: every term uniform, no thermal term, added in slot order
+        // term signature: no thermal term, added in slot order, every term uniform
+        // but interfacial DMI and uniaxial Ku, which may be per cell
         {
             bool ok = P.fast != 0;
             uint32_t sig = 0;
@@ -396,11 +397,13 @@ struct Engine : CtxBase {
             for (int i = 0; i < P.nterms && ok; i++) {
                 const HostTerm &h = C->terms[i];
                 const bool uni = h.kind == K_EXCH || h.kind == K_DMI_BULK || h.kind == K_ZEEMAN_U ||
-                                 h.kind == K_DEMAG || (h.kind == K_DMI_INT && h.uniform) ||
-                                 (h.kind == K_ANIS && h.uniform) || (h.kind == K_CUBIC && h.uniform);
+                                 h.kind == K_DEMAG || h.kind == K_DMI_INT || h.kind == K_ANIS ||
+                                 (h.kind == K_CUBIC && h.uniform);
                 ok = uni && P.ord[i] > last;
                 last = P.ord[i];
                 sig |= 1u << P.ord[i];
+                if (h.kind == K_DMI_INT && !h.uniform) sig |= SG_ID_CELL;
+                if (h.kind == K_ANIS && !h.uniform) sig |= SG_KU_CELL;
             }
             P.sig = ok ? sig : 0u;
         }

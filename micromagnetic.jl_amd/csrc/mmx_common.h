@@ -134,18 +134,22 @@ enum SlotFlags : uint32_t {
 
 enum SlotKind : int { SL_EXCH = 0, SL_DMIB, SL_DMII, SL_ANIS, SL_CUBIC, SL_ZEE, SL_DEMAG, SL_THERM };
 
-// A term signature: the set of slots present (bit = SlotKind) when every term
-// has uniform parameters (no per-cell arrays), there is no thermal term and
-// the terms were added in slot order.  Kernels compiled for a signature know
+// A term signature: the set of slots present (bit = SlotKind) when there is
+// no thermal term, the terms were added in slot order and every term has
+// uniform parameters (no per-cell arrays) except that interfacial DMI and the
+// uniaxial Ku may be given per cell (bits SG_ID_CELL, SG_KU_CELL).  Kernels compiled for a signature know
 // the switches and the insertion order at compile time; signature 0 reads
 // them from the parameter block.
 constexpr uint32_t SG_EX = 1u << SL_EXCH, SG_BD = 1u << SL_DMIB, SG_ID = 1u << SL_DMII, SG_AN = 1u << SL_ANIS,
                    SG_CU = 1u << SL_CUBIC, SG_ZE = 1u << SL_ZEE, SG_DM = 1u << SL_DEMAG;
+constexpr uint32_t SG_ID_CELL = 1u << 8, SG_KU_CELL = 1u << 9;
 // the signatures the kernels are compiled for
 constexpr uint32_t SIG_CHIRAL = SG_EX | SG_ID | SG_AN | SG_ZE | SG_DM;    // film with iDMI, Ku, field, demag
 constexpr uint32_t SIG_SOFT = SG_EX | SG_DM;                              // exchange + demag
 constexpr uint32_t SIG_BULK = SG_EX | SG_BD | SG_CU | SG_DM;              // bulk DMI, cubic, demag
 constexpr uint32_t SIG_CHIRAL_LOCAL = SG_EX | SG_ID | SG_AN | SG_ZE;      // as SIG_CHIRAL without demag
+// film with D(x,y) and Ku(x,y) maps and demag, no field
+constexpr uint32_t SIG_CHIRAL_CELL = SG_EX | SG_ID | SG_AN | SG_DM | SG_ID_CELL | SG_KU_CELL;
 
 constexpr uint32_t sig_flags(uint32_t sig)
 {
@@ -153,8 +157,8 @@ constexpr uint32_t sig_flags(uint32_t sig)
     for (int b = 0; b < 8; b++) n += (sig >> b) & 1u;
     if (sig & SG_EX) f |= 1u << FL_EX_SHIFT;
     if (sig & SG_BD) f |= 1u << FL_BD_SHIFT;
-    if (sig & SG_ID) f |= 1u << FL_ID_SHIFT;
-    if (sig & SG_AN) f |= FL_AN;
+    if (sig & SG_ID) f |= ((sig & SG_ID_CELL) ? 2u : 1u) << FL_ID_SHIFT;
+    if (sig & SG_AN) f |= FL_AN | ((sig & SG_KU_CELL) ? (uint32_t)FL_AN_KU : 0u);
     if (sig & SG_CU) f |= FL_CU;
     if (sig & SG_ZE) f |= 1u << FL_ZE_SHIFT;
     return f | (n << FL_NT_SHIFT);

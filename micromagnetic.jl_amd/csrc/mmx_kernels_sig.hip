@@ -55,6 +55,7 @@ int launch_p5_sig(const DemagGeom &G, const cplx<T> *A, const FieldParams<T> *P,
     case SIG_CHIRAL: return p5_h<T, SIG_CHIRAL>(v, G, A, P, dyn, E, m, tw, s);
     case SIG_SOFT: return p5_h<T, SIG_SOFT>(v, G, A, P, dyn, E, m, tw, s);
     case SIG_BULK: return p5_h<T, SIG_BULK>(v, G, A, P, dyn, E, m, tw, s);
+    case SIG_CHIRAL_CELL: return p5_h<T, SIG_CHIRAL_CELL>(v, G, A, P, dyn, E, m, tw, s);
     default: return -1;
     }
 }
