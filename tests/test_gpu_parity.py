@@ -426,8 +426,9 @@ def test_fp32_error_vs_fp64_oracle(oracle):
 
 SIG_CASES = [
     # (name, mesh, pbc, terms, integrator) -- kernels compiled for term signatures
-    # (bulk DMI + cubic + demag; the chiral film without demag), the order and
-    # per-cell parameters that take the signature-free kernels
+    # (bulk DMI + cubic + demag; the chiral film without demag; D and Ku maps
+    # with demag), the order and per-cell parameters that take the
+    # signature-free kernels
     ("bulk-cubic-demag", (12, 10, 4), "xy",
      [("exch", dict(A=1.3e-11)), ("dmi", dict(D=1e-3)), ("cubic", dict(Kc=-5e4)), ("demag", {})], "RungeKutta"),
     ("bulk-cubic-demag dopri", (12, 10, 4), "",
@@ -442,6 +443,13 @@ SIG_CASES = [
      [("exch", dict(A=1.5e-11)), ("dmi", dict(D=3e-3, type="interfacial")),
       ("anis", dict(Ku=np.linspace(6e5, 9e5, 480))), ("zeeman", dict(H=(0, 0, 79577.47))), ("demag", {})],
      "RungeKutta"),
+    # D(x,y) and Ku(x,y) maps with demag and no field: a compiled signature
+    ("per-cell D and Ku", (12, 10, 4), "",
+     [("exch", dict(A=1.3e-11)), ("dmi", dict(D=np.linspace(2e-3, 3e-3, 480), type="interfacial")),
+      ("anis", dict(Ku=np.linspace(9e5, 6e5, 480))), ("demag", {})], "RungeKutta"),
+    ("per-cell D and Ku dopri", (12, 10, 4), "xy",
+     [("exch", dict(A=1.3e-11)), ("dmi", dict(D=np.linspace(2e-3, 3e-3, 480), type="interfacial")),
+      ("anis", dict(Ku=np.linspace(9e5, 6e5, 480))), ("demag", {})], "DormandPrince"),
 ]
 
 
